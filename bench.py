@@ -1,0 +1,222 @@
+"""Benchmark of the C3D10 solve path on one MI355X.
+
+Workload (default): the C3D10 I-beam, 3 m long, 0.2 x 0.02 flanges, 0.012 web,
+0.3 deep, 300 hex layers split into 6 tets each (351 585 nodes, 1 054 755
+DOF, 208 800 elements), root clamped, -50 kN shear + 5 kN m torsion at the
+tip.  One step = one solve of the resident system from u = 0 to
+||f - K u|| / ||f|| <= 1e-8 (FP64-verified) with the mixed-precision solver:
+FP64 iterative refinement around an FP32 smoothed-aggregation AMG-PCG.
+
+The printed JSON line carries value = seconds per solve (lower is better),
+the same solve with the all-FP64 AMG-PCG and the speedup, the FP32 / FP64 BSR
+SpMV bandwidth, a roofline object for the FP32 3x3 BSR SpMV kernel (the
+dominant kernel of the solve) and a CPU baseline (the FP64 oracle on the host
+cores, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "s to rel-res 1e-8 + BSR SpMV GB/s; mixed-prec vs FP64 PCG, same MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(nnzb: int, nb: int, b: int, sv: int, sx: int) -> int:
+    """Algorithmic bytes of one BSR(b) SpMV: values + block column indices +
+    row pointers + x read once + y written once."""
+    return nnzb * (b * b * sv + 4) + 4 * (nb + 1) + 2 * b * nb * sx
+
+
+def time_spmv(K, precision: str, reps: int) -> float:
+    """Average device time (s) of one SpMV launch, HIP events on the launch stream."""
+    import torch
+    dt = torch.float32 if precision == "fp32" else torch.float64
+    n = K.shape[0]
+    x = torch.randn(n, dtype=torch.float64, device=K.device).to(dt)
+    y = torch.zeros(n, dtype=dt, device=K.device)
+    for _ in range(3):
+        K.spmv(x, y, precision=precision)
+    st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    st.record()
+    for _ in range(reps):
+        K.spmv(x, y, precision=precision)
+    en.record()
+    torch.cuda.synchronize()
+    return st.elapsed_time(en) / 1e3 / reps
+
+
+def cpu_baseline(model, gpu_jacobi_iters: int | None, sample_iters: int = 60) -> dict:
+    """FP64 oracle (OpenMP C) on the host: assembly + a bounded number of
+    Jacobi-PCG iterations of the reference algorithm on the same system; the
+    per-iteration time is scaled by the iteration count the same algorithm
+    needs to reach 1e-8 (measured on the GPU)."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    E, nu = model.element_moduli()
+    t0 = time.perf_counter()
+    K, F = O.assemble_csr(model.mesh_nodes, model.solid_elements, E, nu, model.fixed_dof_mask(3),
+                          model.load_vector(3))
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, it, rel = O.pcg(K, F, rtol=1e-30, max_iter=sample_iters)
+    t_it = (time.perf_counter() - t0) / max(it, 1)
+    need = gpu_jacobi_iters
+    out = {"unit": "s", "cores": threads, "kind": "port",
+           "sample": (f"oracle FP64 Jacobi-PCG (reference algorithm, solid_solver.py:197-308), {it} iterations "
+                      f"on the same I-beam system ({t_it * 1e3:.1f} ms/iteration), assembly {t_asm:.1f} s; "
+                      f"scaled to the {need} iterations the same algorithm takes to 1e-8")}
+    out["value"] = round(t_it * need, 3) if need else None
+    out["seconds_per_iteration"] = t_it
+    out["assembly_sec"] = t_asm
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", choices=["ibeam", "box"], default="ibeam")
+    ap.add_argument("--layers", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-jacobi", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--inner-rtol", type=float, default=1e-3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how the "
+              "solve would shard); rerun with --gpus 1", file=sys.stderr)
+        return 2
+    import torch
+    from wnfea_amd.mesh.rcm import apply_rcm_to_model
+    from wnfea_amd.mesh.solid_generator import generate_c3d10_ibeam, generate_c3d10_structured_block
+    from wnfea_amd.solver.solid_solver import (assemble_c3d10_bsr, build_amg, ir_device, pcg_device,
+                                               true_residual)
+
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    torch.cuda.set_device(0)
+    t0 = time.perf_counter()
+    if args.config == "ibeam":
+        model, meta = generate_c3d10_ibeam(n_layers=args.layers, length=3.0 * args.layers / 300)
+        workload = "c3d10_ibeam_3m_1.05M_dof" if args.layers == 300 else f"c3d10_ibeam_{args.layers}_layers"
+    else:
+        model, meta = generate_c3d10_structured_block(1.0, 0.1, 0.1, 40, 4, 4, tip_load_total=-1000.0,
+                                                      tets_per_hex=6, load_direction="z")
+        workload = "c3d10_box_40x4x4_19683_dof"
+    t_mesh = time.perf_counter() - t0
+    log(f"[bench] mesh {meta['n_dofs']} DOF {meta['n_elements']} elements ({t_mesh:.1f}s)")
+    t0 = time.perf_counter()
+    model, n2o, _ = apply_rcm_to_model(model)
+    t_rcm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    system = assemble_c3d10_bsr(model, want_fp64_equilibrated=True)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    info = system.K.info()
+    log(f"[bench] rcm {t_rcm:.1f}s assembly {t_asm:.1f}s nnzb {info['nnzb']} slots {info['slots']}")
+    t0 = time.perf_counter()
+    amg = build_amg(system)
+    t_amg = time.perf_counter() - t0
+    st = amg.stats()
+    log(f"[bench] amg setup {t_amg:.1f}s levels {[l['n'] for l in st['levels']]} "
+        f"op-complexity {st['operator_complexity']:.2f}")
+
+    def mixed_solve():
+        return ir_device(system, tol=1e-8, inner_rtol=args.inner_rtol, amg=amg)
+
+    def fp64_solve():
+        return pcg_device(system, rtol=1e-8 * 0.5, max_iter=2000, preconditioner="amg",
+                          precision="fp64_equilibrated", amg=amg)
+
+    for _ in range(args.warmup):
+        mixed_solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = [mixed_solve() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    u, outer, inner, rel, _ = res[-1]
+    rel_true = true_residual(system, u)
+    log(f"[bench] mixed: {t_mixed * 1e3:.1f} ms/solve outer {outer} inner {inner} rel {rel_true:.2e}")
+    # all-FP64 AMG-PCG on the same hierarchy
+    fp64_solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        x64, it64, _, _ = fp64_solve()
+    torch.cuda.synchronize()
+    t_fp64 = (time.perf_counter() - t0) / max(1, args.steps // 2)
+    rel64 = true_residual(system, x64)
+    log(f"[bench] fp64 AMG-PCG: {t_fp64 * 1e3:.1f} ms/solve iters {it64} rel {rel64:.2e}")
+    err_u = float(torch.linalg.norm(u - x64) / torch.linalg.norm(x64))
+    # the reference's algorithm on the GPU: FP64 Jacobi-PCG
+    jac = None
+    if not args.no_jacobi:
+        xj, itj, _, tj = pcg_device(system, rtol=1e-8, max_iter=60000, preconditioner="jacobi", precision="fp64",
+                                    check_every=64)
+        relj = true_residual(system, xj)
+        jac = {"seconds": round(tj, 4), "iterations": itj, "rel_res": relj}
+        log(f"[bench] fp64 Jacobi-PCG: {tj:.2f}s iters {itj} rel {relj:.2e}")
+    # SpMV bandwidth and roofline of the FP32 3x3 kernel
+    nnzb, nb = info["nnzb"], system.n_nodes
+    t32 = time_spmv(system.K, "fp32", args.spmv_reps)
+    t64 = time_spmv(system.K, "fp64", args.spmv_reps)
+    b32 = spmv_bytes(nnzb, nb, 3, 4, 4)
+    b64 = spmv_bytes(nnzb, nb, 3, 8, 8)
+    gbs32, gbs64 = b32 / t32 / 1e9, b64 / t64 / 1e9
+    log(f"[bench] spmv fp32 {t32 * 1e6:.1f} us {gbs32:.0f} GB/s | fp64 {t64 * 1e6:.1f} us {gbs64:.0f} GB/s")
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(model, jac["iterations"] if jac else None)
+        log(f"[bench] cpu baseline {cpu['value']} s ({cpu['cores']} threads)")
+    out = {
+        "metric": METRIC,
+        "value": round(t_mixed, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_mixed * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-inner/f64-refinement",
+        "data": "synthetic (structured C3D10 mesh generated in-process, deterministic loads)",
+        "config": {"workload": workload, "n_dofs": meta["n_dofs"], "n_elements": meta["n_elements"],
+                   "nnzb": nnzb, "tol": 1e-8, "solver": "IR(FP64) + FP32 SA-AMG-PCG",
+                   "parallelism": "single-gpu"},
+        "dof_per_s": round(meta["n_dofs"] / t_mixed, 1),
+        "rel_res_fp64_verified": rel_true,
+        "outer_steps": outer,
+        "inner_iterations": inner,
+        "fp64_amg_pcg": {"seconds": round(t_fp64, 5), "iterations": it64, "rel_res": rel64},
+        "speedup_mixed_vs_fp64_amg_pcg": round(t_fp64 / t_mixed, 3),
+        "fp64_jacobi_pcg": jac,
+        "speedup_mixed_vs_fp64_jacobi_pcg": round(jac["seconds"] / t_mixed, 2) if jac else None,
+        "displacement_rel_err_mixed_vs_fp64": err_u,
+        "spmv_gbs": {"fp32": round(gbs32, 1), "fp64": round(gbs64, 1), "fp32_us": round(t32 * 1e6, 2),
+                     "fp64_us": round(t64 * 1e6, 2), "bytes_fp32": b32, "bytes_fp64": b64},
+        "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "sell_spmv_kernel<3,3,float,float,1,OpAxpby> (FP32 3x3 BSR SpMV)"},
+        "setup": {"mesh_sec": round(t_mesh, 2), "rcm_sec": round(t_rcm, 2), "assembly_sec": round(t_asm, 2),
+                  "amg_setup_sec": round(t_amg, 2), "amg": st},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,222 @@
+/*
+ * libwnfea_hip.so -- MI355X (gfx950) solve path of WNFEA's C3D10 solver.
+ *
+ * Conventions
+ *   - Return value: 0 ok, <0 argument/state error (-1 bad argument, -2 bad
+ *     size, -3 bad mode, -4 not ready, -5 bad mesh), >0 a hipError_t code.
+ *     No entry point exits or aborts.
+ *   - "device" arrays are device pointers (e.g. torch tensors' data_ptr());
+ *     "host" arrays are host pointers.  `stream` is a hipStream_t (NULL =
+ *     default stream); kernels are enqueued on it.
+ *   - Handles own their device buffers.  A handle is not thread-safe: use one
+ *     handle per thread.
+ *   - Block matrices use the standard block-CSR layout at the boundary
+ *     (rowptr[nb+1], colidx[nnzb], values[nnzb][br*bc] row-major blocks) and
+ *     a sliced-ELL layout inside the library.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository).  The reference's stateless exports copy host
+ * arrays in and out per call; these take device arrays and a stream instead.
+ */
+#ifndef WNFEA_HIP_H
+#define WNFEA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library ABI version (1).  No reference counterpart. */
+int wnfea_abi_version(void);
+
+/* out[0] CUs, out[1] wavefront size, out[2] gfx arch number (0x950 = 2384),
+ * out[3] L2 bytes, out[4] LDS bytes per block, out[5] HIP runtime version.
+ * Replaces hip_get_device_summary (wnfea/solver/native/hip_kernels.cpp:1541). */
+int wnfea_device_info(long long *out);
+
+/* ---------------------------------------------------------------- elements */
+
+/* Batched C3D10 element stiffness, FP64.  conn[M][10] int32 (device),
+ * nodes[N][3], E[M], nu[M] (device) -> Ke[M][30][30] (device, DOF 3*node+comp).
+ * *bad (device int, zeroed by the caller) receives e+1 of an element with
+ * det J <= 0.  Replaces the per-element element_stiffness_c3d10 loop of
+ * assemble_c3d10_sparse_3dof (wnfea/elements/c3d10.py:219-249,
+ * wnfea/solver/solid_solver.py:72-82). */
+int wnfea_c3d10_ke(int M, const int *conn, const double *nodes, const double *E, const double *nu, double *Ke,
+                   int *bad, void *stream);
+
+/* Matrix-free internal forces F = sum_e K_e u_e, FP64, deterministic.
+ * U and F have `stride` (3 or 6) doubles per node (device); fe_work[M][30]
+ * is device scratch; n2e_ptr/n2e is the node->(element*10+local) incidence
+ * from wnfea_node_incidence (device); accumulate != 0 adds to F.
+ * Replaces hip_compute_c3d10_forces (wnfea/solver/native/hip_kernels.cpp:1633)
+ * and compute_c3d10_internal_forces_native (wnfea/solver/native/kernels.cpp:244). */
+int wnfea_c3d10_fint(int M, int n_nodes, const int *conn, const double *nodes, const double *E, const double *nu,
+                     const double *U, int stride, const int *n2e_ptr, const int *n2e, double *fe_work, double *F,
+                     int accumulate, void *stream);
+
+/* Gauss-point stresses (optional gp_sig[M][4][6], gp_vm[M][4]), cell means
+ * cell_sig[M][6], cell_vm[M], and (if nodal_vm) the nodal average of cell_vm.
+ * U has 3 doubles per node.  Replaces element_stresses_c3d10 and
+ * compute_c3d10_stress_field (wnfea/elements/c3d10.py:286-318,
+ * wnfea/solver/solid_solver.py:311-365). */
+int wnfea_c3d10_stress(int M, int n_nodes, const int *conn, const double *nodes, const double *E, const double *nu,
+                       const double *U, const int *n2e_ptr, const int *n2e, double *gp_sig, double *gp_vm,
+                       double *cell_sig, double *cell_vm, double *nodal_vm, int *bad, void *stream);
+
+/* Adjoint sensitivity with respect to per-element Young's modulus:
+ * grad[e] = -(1/E_e) lam_e^T K_e u_e (K is linear in E_e).  u, lam 3 per node.
+ * No reference counterpart (the reference has no sensitivities); it serves the
+ * compliance / tip-displacement gradients of the adjoint configuration. */
+int wnfea_adjoint_grad(int M, const int *conn, const double *nodes, const double *E, const double *nu,
+                       const double *u, const double *lam, double *fe_work, double *grad, void *stream);
+
+/* ---------------------------------------------------------------- host setup */
+
+/* Reverse Cuthill-McKee order (host arrays): perm[new] = old, for a symmetric
+ * CSR graph with sorted columns and the seed order numpy.argsort(degrees)
+ * uses.  Replaces scipy.sparse.csgraph.reverse_cuthill_mckee as called by
+ * compute_rcm_order (wnfea/mesh/rcm.py:73-92). */
+int wnfea_rcm(int n, const int *ptr, const int *ind, const int *seed_order, int *perm);
+
+/* Node graph of an element mesh (host arrays; npe nodes per element): sorted
+ * unique neighbours including the node itself.  With node_free (host uint8),
+ * off-diagonal pairs touching a fully fixed node are dropped (the Dirichlet
+ * pattern rule).  Returns nnz, or -1 if cap is too small.  Replaces
+ * compute_node_adjacency_matrix (wnfea/mesh/rcm.py:20-70) and the COO->CSR
+ * pattern + eliminate_zeros of assemble_c3d10_sparse_3dof
+ * (wnfea/solver/solid_solver.py:50-64,84,95-127). */
+long long wnfea_node_graph(int n_nodes, int M, int npe, const int *conn, const unsigned char *node_free, int *rowptr,
+                           int *colidx, long long cap);
+
+/* Per-block contribution lists for deterministic assembly (host arrays):
+ * cptr[nnzb+1] (int64), codes = e*npe*npe + a*npe + b in element order.
+ * Returns the number of codes, or -1 if cap is too small.  Replaces the COO
+ * duplicate summation of assemble_c3d10_sparse_3dof
+ * (wnfea/solver/solid_solver.py:62-84). */
+long long wnfea_contributions(int n_nodes, int M, int npe, const int *conn, const int *rowptr, const int *colidx,
+                              long long *cptr, int *codes, long long cap);
+
+/* Node -> (element*npe + local) incidence in element order (host arrays).
+ * Serves the deterministic gathers of wnfea_c3d10_fint/_stress; replaces the
+ * atomic scatter of c3d10_internal_forces_kernel
+ * (wnfea/solver/native/hip_kernels.cpp:418-423). */
+int wnfea_node_incidence(int n_nodes, int M, int npe, const int *conn, int *ptr, int *codes);
+
+/* Smoothed-aggregation aggregates of a strength graph (host; CSR without the
+ * diagonal).  agg[i] = aggregate id or -1 (isolated).  Returns the count.
+ * Replaces aggregate_beam_nodes (wnfea/solver/amg_preconditioner.py:72-124). */
+long long wnfea_sa_aggregate(int n, const int *ptr, const int *ind, int *agg);
+
+/* Tentative prolongator (host): per-aggregate orthonormalisation of the
+ * near-null-space B[n_nodes*b][k]; P as block CSR (b x k blocks, one per
+ * aggregated node) and the coarse near-null-space Bc[n_agg*k][k].  Replaces
+ * the per-aggregate QR of _build_hierarchy
+ * (wnfea/solver/amg_preconditioner.py:241-281). */
+int wnfea_sa_tentative(int n_nodes, int b, int k, const int *agg, int n_agg, const double *B, int *p_rowptr,
+                       int *p_colidx, double *p_vals, double *Bc);
+
+/* ---------------------------------------------------------------- block matrices */
+
+/* Create a br x bc block matrix from its host block-CSR structure; sigma is
+ * the row-sorting window of the sliced-ELL layout (multiple of 64; <=0 =
+ * 4096).  Returns NULL on failure.  Replaces BSR6x6Matrix / csr_to_bsr6x6
+ * (wnfea/solver/bsr_matrix.py:19-122) and the device uploads of
+ * hip_spmv_bsr6x6_fp32/fp64 (wnfea/solver/native/hip_kernels.cpp:2975-3087). */
+void *wnfea_bsr_create(int br, int bc, int n_rows, int n_cols, const int *rowptr, const int *colidx, int sigma);
+void wnfea_bsr_destroy(void *h);
+
+/* out[0..6]: n_slices, slots, nnzb, sigma, has FP64, has FP32, has FP64 equilibrated. */
+int wnfea_bsr_info(void *h, long long *out);
+
+/* SpMV launch shape: waves per slice (1, 2, 4) and XCD-contiguous slice mapping. */
+int wnfea_bsr_tune(void *h, int waves, int xcd_remap);
+
+/* Values from the standard layout (device FP64 [nnzb][br*bc]); which: 1 FP64, 2 FP32, 3 both. */
+int wnfea_bsr_set_values(void *h, const double *vals_std, int which, void *stream);
+
+/* FP64 values back in the standard layout (device).  Replaces bsr6x6_to_csr's
+ * value source (wnfea/solver/bsr_matrix.py:125-152). */
+int wnfea_bsr_get_values(void *h, double *vals_std, void *stream);
+
+/* Assemble 3x3 blocks from Ke (device) via the contribution lists (device
+ * rowidx/colidx per block, cptr, codes), applying the Dirichlet rule per
+ * scalar entry from node_fix (device uint8 bits per node; NULL = no BCs);
+ * optionally also writes vals_std.  Replaces assemble_c3d10_sparse_3dof's
+ * value path (wnfea/solver/solid_solver.py:66-127). */
+int wnfea_bsr_assemble(void *h, const int *rowidx, const int *colidx, const long long *cptr, const int *codes,
+                       const double *Ke, const unsigned char *node_fix, double *vals_std, void *stream);
+
+/* Scalar diagonal (device) given diag_block[i] = standard index of block (i,i). */
+int wnfea_bsr_diag(void *h, const long long *diag_block, double *diag, void *stream);
+
+/* FP32 (and optionally FP64) values of S K S with S = diag(scale) (device).
+ * Replaces the equilibration of BlockBeamAMGPreconditioner
+ * (wnfea/solver/amg_preconditioner.py:180-185). */
+int wnfea_bsr_equilibrate(void *h, const int *rowidx, const int *colidx, const double *scale, int want64e,
+                          void *stream);
+
+/* y = alpha A x + beta y; prec 0 FP64, 1 FP32 (FP32 vectors), 2 FP64
+ * equilibrated.  Replaces hip_spmv_bsr6x6_fp32 / _fp64
+ * (wnfea/solver/native/hip_kernels.cpp:2975,3032) and bsr6x6_spmv_cpu
+ * (wnfea/solver/bsr_matrix.py:155-193). */
+int wnfea_bsr_spmv(void *h, int prec, const void *x, void *y, double alpha, double beta, void *stream);
+
+/* y = A x and *out = x . (A x) (FP64 reduction, host scalar; synchronises).
+ * Replaces spmv_csr_dot_kernel's fused product (hip_kernels.cpp:1171-1228). */
+int wnfea_bsr_spmv_dot(void *h, int prec, const void *x, void *y, double *out, void *stream);
+
+/* ---------------------------------------------------------------- AMG */
+
+/* Replaces hip_amg_create / hip_amg_set_level / hip_amg_set_coarse_cholesky /
+ * hip_amg_apply_vcycle / hip_amg_destroy
+ * (wnfea/solver/native/hip_kernels.cpp:1891,1924,2061,2519,2541). */
+void *wnfea_amg_create(int n_levels);
+void wnfea_amg_destroy(void *h);
+
+/* Level data (host arrays).  Level 0 passes fine_bsr (its FP32 / FP64
+ * equilibrated values are the operator) and NULL A arrays; other levels pass
+ * A as br x br block CSR.  dinv: point-Jacobi inverse diagonal; Chebyshev on
+ * [lmax/ratio, lmax] of the given degree; P (br x bc_coarse blocks) and
+ * R = P^T (bc_coarse x br) except on the coarsest level. */
+int wnfea_amg_set_level(void *h, int lvl, void *fine_bsr, int br, int n_rows, const int *a_rowptr,
+                        const int *a_colidx, const double *a_vals, const double *dinv, double lmax, double ratio,
+                        int degree, int bc_coarse, int n_coarse_rows, const int *p_rowptr, const int *p_colidx,
+                        const double *p_vals, const int *r_rowptr, const int *r_colidx, const double *r_vals,
+                        int sigma);
+
+/* Coarsest-level dense inverse (host, n x n row-major FP64). */
+int wnfea_amg_set_coarse(void *h, int n, const double *inv);
+
+/* Allocate cycle vectors (prec_mask bit0 FP64, bit1 FP32) and set the SpMV launch shape. */
+int wnfea_amg_finalize(void *h, int prec_mask, int waves, int xcd_remap);
+
+/* z = M r, one V-cycle (device vectors); prec 0 FP64, 1 FP32. */
+int wnfea_amg_vcycle(void *h, int prec, const void *r, void *z, void *stream);
+
+/* ---------------------------------------------------------------- solvers */
+
+void *wnfea_solver_create(void);
+void wnfea_solver_destroy(void *h);
+
+/* PCG from x = 0 until ||r|| <= rtol ||b||; prec as wnfea_bsr_spmv; pkind 0
+ * none, 1 Jacobi (dinv in the vectors' precision), 2 AMG.  Iterations run in
+ * device-scalar chunks of check_every between host checks.  Replaces
+ * hip_amg_solve_pcg(_device) (wnfea/solver/native/hip_kernels.cpp:2384-2517)
+ * and solve_c3d10_pcg's FP64 branch (wnfea/solver/solid_solver.py:302-308). */
+int wnfea_pcg(void *solver, void *bsr, int prec, int pkind, const void *dinv, void *amg, const void *b, void *x,
+              double rtol, int max_iter, int check_every, int *iters, double *rel, void *stream);
+
+/* FP64 iterative refinement: r = f - K u (FP64), inner FP32 PCG on S K S to
+ * inner_rtol, u += S d, until ||f - K u|| <= tol ||f||.  Replaces
+ * solve_c3d10_pcg's mixed_fp32 branch (wnfea/solver/solid_solver.py:271-300). */
+int wnfea_ir(void *solver, void *bsr, const double *scale, int pkind, const float *dinv32, void *amg,
+             const double *f, double *u, double tol, double inner_rtol, int max_outer, int max_inner,
+             int check_every, int zero_init, int *outer, int *inner, double *rel, void *stream);
+
+/* r = f - K u (FP64; r may be NULL) and out[0] = ||r||, out[1] = ||f||. */
+int wnfea_residual(void *solver, void *bsr, const double *u, const double *f, double *r, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WNFEA_HIP_H */

@@ -1,0 +1,123 @@
+"""Element, assembly and SpMV kernels on the GPU against the CPU FP64 oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_tets(n, seed=0):
+    rng = np.random.default_rng(seed)
+    base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    out = []
+    while len(out) < n:
+        A = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+        if np.linalg.det(A) <= 0.2:
+            continue
+        c = base @ A.T * rng.uniform(0.01, 2.0) + rng.standard_normal(3)
+        mids = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+        X = np.vstack([c] + [0.5 * (c[a] + c[b]) + 0.02 * rng.standard_normal(3) * 0.1 for a, b in mids])
+        out.append(X)
+    return np.array(out)
+
+
+def test_ke_matches_oracle():
+    from oracle import oracle as O
+    from wnfea_amd.elements.c3d10 import element_stiffness_c3d10
+    X = _random_tets(300)
+    E = np.random.default_rng(1).uniform(1e9, 3e11, len(X))
+    nu = np.random.default_rng(2).uniform(0.1, 0.45, len(X))
+    Ke = np.stack([element_stiffness_c3d10(X[i], E[i], nu[i]) for i in range(3)])
+    from wnfea_amd.elements import c3d10
+    Kb = c3d10.element_stiffness_c3d10(X, E, nu)
+    for i in range(len(X)):
+        Ko = O.element_stiffness(X[i], E[i], nu[i])
+        assert np.abs(Kb[i] - Ko).max() <= 1e-12 * np.abs(Ko).max()
+        assert np.array_equal(Kb[i], Kb[i].T)
+    assert np.allclose(Ke, Kb[:3], rtol=0, atol=0)
+
+
+def test_fint_and_stress_match_oracle():
+    from oracle import oracle as O
+    from wnfea_amd.elements import c3d10
+    X = _random_tets(64, seed=3)
+    rng = np.random.default_rng(4)
+    U = rng.standard_normal((64, 30)) * 1e-3
+    f = c3d10.element_internal_forces_c3d10(X, U, 2.1e11, 0.3)
+    s, vm, _ = c3d10.element_stresses_c3d10(X, U, 2.1e11, 0.3)
+    for i in range(64):
+        fo = O.element_internal_forces(X[i], U[i], 2.1e11, 0.3)
+        assert np.linalg.norm(f[i] - fo) <= 1e-12 * np.linalg.norm(fo)
+        so, vo, _ = O.element_stresses(X[i], U[i], 2.1e11, 0.3)
+        assert np.abs(s[i] - so).max() <= 1e-10 * np.abs(so).max()
+        assert np.abs(vm[i] - vo).max() <= 1e-10 * np.abs(vo).max()
+
+
+def test_inverted_element_raises():
+    from wnfea_amd.elements import c3d10
+    X = _random_tets(1, seed=5)[0]
+    X[[1, 2]] = X[[2, 1]]
+    with pytest.raises(ValueError):
+        c3d10.element_stiffness_c3d10(X, 2.1e11, 0.3)
+
+
+def test_assembly_pattern_and_values_c0(c0_model, c0_oracle):
+    """Block pattern bit-exact with the oracle's CSR-derived pattern; values to FP64 round-off."""
+    from oracle import oracle as O
+    from wnfea_amd.solver.solid_solver import assemble_c3d10_bsr, assemble_c3d10_sparse_3dof
+    _, mr, _ = c0_model
+    K, F, _ = c0_oracle
+    rp, ci, vals = O.csr_to_bsr(K, 3)
+    s = assemble_c3d10_bsr(mr)
+    assert np.array_equal(s.K.rowptr, rp) and np.array_equal(s.K.colidx, ci)
+    v = s.K.get_values().cpu().numpy()
+    assert np.abs(v - vals).max() <= 1e-12 * np.abs(vals).max()
+    assert np.array_equal(s.F.cpu().numpy(), F)
+    Kc, Fc, fixed = assemble_c3d10_sparse_3dof(mr)
+    assert fixed == mr.fixed_dof_list(3)
+    assert abs(Kc - K).max() <= 1e-12 * abs(K).max()
+    assert np.array_equal(Fc, F)
+
+
+def test_spmv_fp64_fp32_c0(c0_oracle):
+    import torch
+    from oracle import oracle as O
+    from wnfea_amd.solver.bsr_matrix import DeviceBSR, csr_to_bsr
+    K, _, _ = c0_oracle
+    Kb = csr_to_bsr(K, 3)
+    x = np.random.default_rng(42).standard_normal(K.shape[0])
+    y_ref = O.csr_spmv(K, x)
+    for waves in (1, 2, 4):
+        for xcd in (0, 1):
+            m = DeviceBSR.from_host(Kb, which=3, sigma=1024)
+            m.tune(waves, xcd)
+            xd = torch.as_tensor(x, device=m.device)
+            y = m.spmv(xd).cpu().numpy()
+            assert np.linalg.norm(y - y_ref) <= 1e-14 * np.linalg.norm(y_ref) * 10
+            y32 = m.spmv(xd.float(), precision="fp32").cpu().numpy()
+            assert np.linalg.norm(y32 - y_ref) <= 1e-5 * np.linalg.norm(y_ref)
+            y2 = m.spmv(xd, torch.ones_like(xd), alpha=2.0, beta=-1.0).cpu().numpy()
+            assert np.linalg.norm(y2 - (2 * y_ref - 1)) <= 1e-13 * np.linalg.norm(y_ref)
+            m.close()
+
+
+@pytest.mark.parametrize("br,bc", [(6, 6), (3, 6), (6, 3)])
+def test_spmv_rect_blocks(br, bc):
+    import scipy.sparse as sp
+    import torch
+    from wnfea_amd.solver.bsr_matrix import DeviceBSR
+    rng = np.random.default_rng(br * 10 + bc)
+    nr, nc = 700, 450
+    A = sp.random(nr, nc, density=0.02, random_state=7, format="csr")
+    A.data[:] = 1
+    rowptr, colidx = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    vals = rng.standard_normal((A.nnz, br * bc))
+    m = DeviceBSR(br, bc, nr, nc, rowptr, colidx, sigma=256)
+    m.set_values(torch.as_tensor(vals, device=m.device), which=3)
+    x = rng.standard_normal(nc * bc)
+    y = m.spmv(torch.as_tensor(x, device=m.device)).cpu().numpy()
+    Bs = sp.bsr_matrix((vals.reshape(-1, br, bc), colidx, rowptr), shape=(nr * br, nc * bc))
+    y_ref = Bs @ x
+    assert np.linalg.norm(y - y_ref) <= 1e-13 * np.linalg.norm(y_ref)
+    back = m.get_values().cpu().numpy()
+    assert np.array_equal(back, vals)
+    m.close()

@@ -1,0 +1,385 @@
+// C-ABI of the gfx950 solve-path library (libwnfea_hip.so).
+//
+// Conventions: every entry point returns int status (0 ok, <0 argument error,
+// >0 HIP error code); device arrays are plain device pointers (torch tensors'
+// data_ptr() on the Python side); `stream` is a hipStream_t passed as void*
+// (null = default stream).  Host-side setup functions take host pointers and
+// say so.  Handles own their device buffers; one handle must not be used from
+// two threads at once.  Declarations and the reference interface each entry
+// replaces are in include/wnfea_hip.h.
+#include <cstring>
+#include <memory>
+#include <vector>
+#include "wn.hpp"
+
+namespace wn {
+// setup.cpp
+void rcm_order(int n, const int* ptr, const int* ind, const int* seed_order, int* perm);
+long long node_graph(int n_nodes, int M, int npe, const int* conn, const unsigned char* node_free, int* rowptr,
+                     int* colidx, long long cap);
+long long contributions(int n_nodes, int M, int npe, const int* conn, const int* rowptr, const int* colidx,
+                        long long* cptr, int* codes, long long cap);
+void node_incidence(int n_nodes, int M, int npe, const int* conn, int* ptr, int* codes);
+long long sa_aggregate(int n, const int* ptr, const int* ind, int* agg);
+int sa_tentative(int n_nodes, int b, int k, const int* agg, int n_agg, const double* B, int* p_rowptr, int* p_colidx,
+                 double* p_vals, double* Bc);
+// solver.hip
+struct Amg;
+struct AmgLevel;
+struct Solver;
+template <typename T> int amg_apply(Amg& g, const T* r, T* z, const int* skip, hipStream_t s);
+template <typename T>
+int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, const T* b, T* x, double rtol,
+            int max_iter, int check_every, const SpmvTune& tune0, hipStream_t s, int* iters, double* rel);
+int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float* dinv32, Amg* amg, const double* f,
+           double* u, double tol, double inner_rtol, int max_outer, int max_inner, int check_every,
+           const SpmvTune& tune, hipStream_t s, int* outer_out, int* inner_out, double* rel_out, int zero_init);
+Solver* solver_new();
+void solver_delete(Solver*);
+Workspace& solver_ws(Solver&);
+Amg* amg_new();
+void amg_delete(Amg*);
+int amg_resize(Amg&, int n_levels);
+int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const int* a_rowptr, const int* a_colidx,
+                  const double* a_vals, const double* dinv, double lmax, double ratio, int degree, int bc_coarse,
+                  int n_coarse_rows, const int* p_rowptr, const int* p_colidx, const double* p_vals,
+                  const int* r_rowptr, const int* r_colidx, const double* r_vals, int sigma);
+int amg_set_coarse(Amg& g, int n, const double* inv);
+int amg_finalize(Amg& g, int prec_mask, const SpmvTune& t);
+}  // namespace wn
+
+using namespace wn;
+
+struct WnBsr {
+  SellMatrix m;
+  SpmvTune tune;
+  std::unique_ptr<Solver, void (*)(Solver*)> scratch{nullptr, solver_delete};
+};
+
+static hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+extern "C" {
+
+int wnfea_abi_version(void) { return 1; }
+
+// out[0] CUs, out[1] wavefront size, out[2] gfx arch number (e.g. 950),
+// out[3] L2 bytes, out[4] LDS bytes per block, out[5] runtime version.
+int wnfea_device_info(long long* out) {
+  if (!out) return kBadArg;
+  int dev = 0;
+  WN_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  WN_HIP(hipGetDeviceProperties(&p, dev));
+  int rt = 0;
+  WN_HIP(hipRuntimeGetVersion(&rt));
+  out[0] = p.multiProcessorCount;
+  out[1] = p.warpSize;
+  int arch = 0;
+  if (std::strncmp(p.gcnArchName, "gfx", 3) == 0) arch = static_cast<int>(std::strtol(p.gcnArchName + 3, nullptr, 16));
+  out[2] = arch;
+  out[3] = p.l2CacheSize;
+  out[4] = static_cast<long long>(p.sharedMemPerBlock);
+  out[5] = rt;
+  return kOk;
+}
+
+// ------------------------------------------------------------ element kernels
+int wnfea_c3d10_ke(int M, const int* conn, const double* nodes, const double* E, const double* nu, double* Ke,
+                   int* bad, void* stream) {
+  if (M < 0 || (M > 0 && (!conn || !nodes || !E || !nu || !Ke || !bad))) return kBadArg;
+  return launch_ke(M, conn, nodes, E, nu, Ke, bad, S(stream));
+}
+
+int wnfea_c3d10_fint(int M, int n_nodes, const int* conn, const double* nodes, const double* E, const double* nu,
+                     const double* U, int stride, const int* n2e_ptr, const int* n2e, double* fe_work, double* F,
+                     int accumulate, void* stream) {
+  if (M < 0 || n_nodes < 0 || (stride != 3 && stride != 6)) return kBadArg;
+  if (M > 0 && (!conn || !nodes || !E || !nu || !U || !n2e_ptr || !n2e || !fe_work || !F)) return kBadArg;
+  int r = launch_fint_elem(M, conn, nodes, E, nu, U, stride, fe_work, S(stream));
+  if (r) return r;
+  return launch_gather_node(n_nodes, n2e_ptr, n2e, fe_work, F, stride, accumulate, S(stream));
+}
+
+int wnfea_c3d10_stress(int M, int n_nodes, const int* conn, const double* nodes, const double* E, const double* nu,
+                       const double* U, const int* n2e_ptr, const int* n2e, double* gp_sig, double* gp_vm,
+                       double* cell_sig, double* cell_vm, double* nodal_vm, int* bad, void* stream) {
+  if (M < 0 || n_nodes < 0) return kBadArg;
+  if (M > 0 && (!conn || !nodes || !E || !nu || !U || !cell_sig || !cell_vm || !bad)) return kBadArg;
+  int r = launch_stress(M, conn, nodes, E, nu, U, gp_sig, gp_vm, cell_sig, cell_vm, bad, S(stream));
+  if (r) return r;
+  if (nodal_vm) {
+    if (!n2e_ptr || !n2e) return kBadArg;
+    return launch_nodal_average(n_nodes, n2e_ptr, n2e, cell_vm, nodal_vm, S(stream));
+  }
+  return kOk;
+}
+
+int wnfea_adjoint_grad(int M, const int* conn, const double* nodes, const double* E, const double* nu,
+                       const double* u, const double* lam, double* fe_work, double* grad, void* stream) {
+  if (M < 0 || (M > 0 && (!conn || !nodes || !E || !nu || !u || !lam || !fe_work || !grad))) return kBadArg;
+  int r = launch_fint_elem(M, conn, nodes, E, nu, u, 3, fe_work, S(stream));
+  if (r) return r;
+  return launch_adjoint_dot(M, conn, E, fe_work, lam, grad, S(stream));
+}
+
+// ------------------------------------------------------------ host setup
+int wnfea_rcm(int n, const int* ptr, const int* ind, const int* seed_order, int* perm) {
+  if (n < 0 || (n > 0 && (!ptr || !ind || !seed_order || !perm))) return kBadArg;
+  rcm_order(n, ptr, ind, seed_order, perm);
+  return kOk;
+}
+
+long long wnfea_node_graph(int n_nodes, int M, int npe, const int* conn, const unsigned char* node_free, int* rowptr,
+                           int* colidx, long long cap) {
+  if (n_nodes < 0 || M < 0 || npe <= 0 || !rowptr || (M > 0 && !conn)) return kBadArg;
+  return node_graph(n_nodes, M, npe, conn, node_free, rowptr, colidx, cap);
+}
+
+long long wnfea_contributions(int n_nodes, int M, int npe, const int* conn, const int* rowptr, const int* colidx,
+                              long long* cptr, int* codes, long long cap) {
+  if (n_nodes < 0 || M < 0 || npe <= 0 || !rowptr || !cptr || (M > 0 && (!conn || !codes))) return kBadArg;
+  if ((long long)M * npe * npe > 2147483647LL) return kBadSize;
+  return contributions(n_nodes, M, npe, conn, rowptr, colidx, cptr, codes, cap);
+}
+
+int wnfea_node_incidence(int n_nodes, int M, int npe, const int* conn, int* ptr, int* codes) {
+  if (n_nodes < 0 || M < 0 || npe <= 0 || !ptr || (M > 0 && (!conn || !codes))) return kBadArg;
+  node_incidence(n_nodes, M, npe, conn, ptr, codes);
+  return kOk;
+}
+
+long long wnfea_sa_aggregate(int n, const int* ptr, const int* ind, int* agg) {
+  if (n < 0 || (n > 0 && (!ptr || !agg))) return kBadArg;
+  return sa_aggregate(n, ptr, ind, agg);
+}
+
+int wnfea_sa_tentative(int n_nodes, int b, int k, const int* agg, int n_agg, const double* B, int* p_rowptr,
+                       int* p_colidx, double* p_vals, double* Bc) {
+  if (n_nodes < 0 || b <= 0 || k <= 0 || n_agg < 0 || !agg || !B || !p_rowptr || !p_colidx || !p_vals || !Bc)
+    return kBadArg;
+  return sa_tentative(n_nodes, b, k, agg, n_agg, B, p_rowptr, p_colidx, p_vals, Bc);
+}
+
+// ------------------------------------------------------------ block matrices
+void* wnfea_bsr_create(int br, int bc, int n_rows, int n_cols, const int* rowptr, const int* colidx, int sigma) {
+  auto* h = new WnBsr();
+  if (h->m.build(br, bc, n_rows, n_cols, rowptr, colidx, sigma) != kOk) {
+    h->m.release();
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+void wnfea_bsr_destroy(void* h) {
+  if (!h) return;
+  auto* b = static_cast<WnBsr*>(h);
+  b->scratch.reset();
+  b->m.release();
+  delete b;
+}
+
+// out: n_slices, slots, nnzb, sigma, has_v64, has_v32, has_v64e
+int wnfea_bsr_info(void* h, long long* out) {
+  if (!h || !out) return kBadArg;
+  const SellMatrix& m = static_cast<WnBsr*>(h)->m;
+  out[0] = m.n_slices;
+  out[1] = m.slots;
+  out[2] = m.nnzb;
+  out[3] = m.sigma;
+  out[4] = m.d_v64 != nullptr;
+  out[5] = m.d_v32 != nullptr;
+  out[6] = m.d_v64e != nullptr;
+  return kOk;
+}
+
+int wnfea_bsr_tune(void* h, int waves, int xcd_remap) {
+  if (!h || (waves != 1 && waves != 2 && waves != 4)) return kBadArg;
+  auto* b = static_cast<WnBsr*>(h);
+  b->tune.waves = waves;
+  b->tune.xcd_remap = xcd_remap ? 1 : 0;
+  return kOk;
+}
+
+// which: bit0 FP64 values, bit1 FP32 values (both from the same FP64 input)
+int wnfea_bsr_set_values(void* h, const double* vals_std, int which, void* stream) {
+  if (!h || (!vals_std && static_cast<WnBsr*>(h)->m.nnzb > 0) || !(which & 3)) return kBadArg;
+  return sell_from_std(static_cast<WnBsr*>(h)->m, vals_std, which & 1, which & 2, S(stream));
+}
+
+int wnfea_bsr_get_values(void* h, double* vals_std, void* stream) {
+  if (!h || !vals_std) return kBadArg;
+  return sell_to_std(static_cast<WnBsr*>(h)->m, vals_std, S(stream));
+}
+
+int wnfea_bsr_assemble(void* h, const int* rowidx, const int* colidx, const long long* cptr, const int* codes,
+                       const double* Ke, const unsigned char* node_fix, double* vals_std, void* stream) {
+  if (!h) return kBadArg;
+  auto* b = static_cast<WnBsr*>(h);
+  if (b->m.nnzb > 0 && (!rowidx || !colidx || !cptr || !codes || !Ke)) return kBadArg;
+  return assemble_gather(b->m, rowidx, colidx, cptr, codes, Ke, node_fix, vals_std, S(stream));
+}
+
+int wnfea_bsr_diag(void* h, const long long* diag_block, double* diag, void* stream) {
+  if (!h || !diag_block || !diag) return kBadArg;
+  return extract_diag(static_cast<WnBsr*>(h)->m, diag_block, diag, S(stream));
+}
+
+int wnfea_bsr_equilibrate(void* h, const int* rowidx, const int* colidx, const double* scale, int want64e,
+                          void* stream) {
+  if (!h || !rowidx || !colidx || !scale) return kBadArg;
+  return equilibrate(static_cast<WnBsr*>(h)->m, rowidx, colidx, scale, want64e, S(stream));
+}
+
+// prec: 0 = FP64 primary values, 1 = FP32 values, 2 = FP64 equilibrated values
+int wnfea_bsr_spmv(void* h, int prec, const void* x, void* y, double alpha, double beta, void* stream) {
+  if (!h || !x || !y) return kBadArg;
+  auto* b = static_cast<WnBsr*>(h);
+  switch (prec) {
+    case 0:
+      return spmv_axpby<double, double>(b->m, static_cast<const double*>(x), static_cast<double*>(y), alpha, beta,
+                                        b->tune, S(stream));
+    case 1:
+      return spmv_axpby<float, float>(b->m, static_cast<const float*>(x), static_cast<float*>(y), alpha, beta,
+                                      b->tune, S(stream));
+    case 2:
+      if (!b->m.d_v64e) return kNotReady;
+      return spmv_axpby<double, double>(b->m.equilibrated(), static_cast<const double*>(x), static_cast<double*>(y),
+                                        alpha, beta, b->tune, S(stream));
+    default:
+      return kBadMode;
+  }
+}
+
+// y = A x and out[0] = x . (A x), FP64 (prec 0/2) or FP32 values with FP64 dot (prec 1)
+int wnfea_bsr_spmv_dot(void* h, int prec, const void* x, void* y, double* out, void* stream) {
+  if (!h || !x || !y || !out) return kBadArg;
+  auto* b = static_cast<WnBsr*>(h);
+  if (!b->scratch) b->scratch.reset(solver_new());
+  Workspace& w = solver_ws(*b->scratch);
+  int r;
+  if (prec == 1)
+    r = spmv_dot<float>(b->m, static_cast<const float*>(x), static_cast<float*>(y), w, 0, b->tune, S(stream));
+  else if (prec == 0 || prec == 2)
+    r = spmv_dot<double>(prec == 2 ? b->m.equilibrated() : b->m, static_cast<const double*>(x),
+                         static_cast<double*>(y), w, 0, b->tune, S(stream));
+  else
+    return kBadMode;
+  if (r) return r;
+  WN_HIP(hipMemcpyAsync(out, w.sc + kScGamma, sizeof(double), hipMemcpyDeviceToHost, S(stream)));
+  WN_HIP(hipStreamSynchronize(S(stream)));
+  return kOk;
+}
+
+// ------------------------------------------------------------ AMG
+void* wnfea_amg_create(int n_levels) {
+  if (n_levels <= 0 || n_levels > 32) return nullptr;
+  Amg* g = amg_new();
+  if (amg_resize(*g, n_levels) != kOk) {
+    amg_delete(g);
+    return nullptr;
+  }
+  return g;
+}
+
+void wnfea_amg_destroy(void* h) {
+  if (h) amg_delete(static_cast<Amg*>(h));
+}
+
+int wnfea_amg_set_level(void* h, int lvl, void* fine_bsr, int br, int n_rows, const int* a_rowptr,
+                        const int* a_colidx, const double* a_vals, const double* dinv, double lmax, double ratio,
+                        int degree, int bc_coarse, int n_coarse_rows, const int* p_rowptr, const int* p_colidx,
+                        const double* p_vals, const int* r_rowptr, const int* r_colidx, const double* r_vals,
+                        int sigma) {
+  if (!h || !dinv) return kBadArg;
+  SellMatrix* fine = fine_bsr ? &static_cast<WnBsr*>(fine_bsr)->m : nullptr;
+  return amg_set_level(*static_cast<Amg*>(h), lvl, fine, br, n_rows, a_rowptr, a_colidx, a_vals, dinv, lmax, ratio,
+                       degree, bc_coarse, n_coarse_rows, p_rowptr, p_colidx, p_vals, r_rowptr, r_colidx, r_vals,
+                       sigma);
+}
+
+int wnfea_amg_set_coarse(void* h, int n, const double* inv) {
+  if (!h || n <= 0 || !inv) return kBadArg;
+  return amg_set_coarse(*static_cast<Amg*>(h), n, inv);
+}
+
+// prec_mask bit0: allocate FP64 cycle vectors, bit1: FP32
+int wnfea_amg_finalize(void* h, int prec_mask, int waves, int xcd_remap) {
+  if (!h || !(prec_mask & 3) || (waves != 1 && waves != 2 && waves != 4)) return kBadArg;
+  SpmvTune t;
+  t.waves = waves;
+  t.xcd_remap = xcd_remap ? 1 : 0;
+  return amg_finalize(*static_cast<Amg*>(h), prec_mask, t);
+}
+
+// z = M r (one V-cycle), prec 0 = FP64, 1 = FP32
+int wnfea_amg_vcycle(void* h, int prec, const void* r, void* z, void* stream) {
+  if (!h || !r || !z) return kBadArg;
+  Amg& g = *static_cast<Amg*>(h);
+  if (prec == 0) return amg_apply<double>(g, static_cast<const double*>(r), static_cast<double*>(z), nullptr, S(stream));
+  if (prec == 1) return amg_apply<float>(g, static_cast<const float*>(r), static_cast<float*>(z), nullptr, S(stream));
+  return kBadMode;
+}
+
+// ------------------------------------------------------------ solvers
+void* wnfea_solver_create(void) { return solver_new(); }
+void wnfea_solver_destroy(void* h) {
+  if (h) solver_delete(static_cast<Solver*>(h));
+}
+
+// PCG on A x = b from x = 0 until ||r|| <= rtol ||b|| (recursive residual).
+// prec: 0 FP64 primary values, 1 FP32 values (FP32 vectors, FP64 dots),
+// 2 FP64 equilibrated values.  pkind: 0 none, 1 Jacobi (dinv, same precision
+// as the vectors), 2 AMG (amg handle).
+int wnfea_pcg(void* solver, void* bsr, int prec, int pkind, const void* dinv, void* amg, const void* b, void* x,
+              double rtol, int max_iter, int check_every, int* iters, double* rel, void* stream) {
+  if (!solver || !bsr || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2) return kBadArg;
+  if ((pkind == 1 && !dinv) || (pkind == 2 && !amg)) return kBadArg;
+  auto* B = static_cast<WnBsr*>(bsr);
+  Solver& Sv = *static_cast<Solver*>(solver);
+  Amg* g = static_cast<Amg*>(amg);
+  if (prec == 1)
+    return pcg_run<float>(Sv, B->m, pkind, static_cast<const float*>(dinv), g, static_cast<const float*>(b),
+                          static_cast<float*>(x), rtol, max_iter, check_every, B->tune, S(stream), iters, rel);
+  if (prec == 0 || prec == 2) {
+    if (prec == 2 && !B->m.d_v64e) return kNotReady;
+    return pcg_run<double>(Sv, prec == 2 ? B->m.equilibrated() : B->m, pkind, static_cast<const double*>(dinv), g,
+                           static_cast<const double*>(b), static_cast<double*>(x), rtol, max_iter, check_every,
+                           B->tune, S(stream), iters, rel);
+  }
+  return kBadMode;
+}
+
+// FP64 iterative refinement with an FP32 inner PCG (see solver.hip).
+int wnfea_ir(void* solver, void* bsr, const double* scale, int pkind, const float* dinv32, void* amg,
+             const double* f, double* u, double tol, double inner_rtol, int max_outer, int max_inner,
+             int check_every, int zero_init, int* outer, int* inner, double* rel, void* stream) {
+  if (!solver || !bsr || !scale || !f || !u || pkind < 0 || pkind > 2) return kBadArg;
+  if ((pkind == 1 && !dinv32) || (pkind == 2 && !amg)) return kBadArg;
+  auto* B = static_cast<WnBsr*>(bsr);
+  return ir_run(*static_cast<Solver*>(solver), B->m, scale, pkind, dinv32, static_cast<Amg*>(amg), f, u, tol,
+                inner_rtol, max_outer, max_inner, check_every, B->tune, S(stream), outer, inner, rel, zero_init);
+}
+
+// r = f - K u (FP64 primary values) and out[0] = ||r||, out[1] = ||f||.
+int wnfea_residual(void* solver, void* bsr, const double* u, const double* f, double* r, double* out, void* stream) {
+  if (!solver || !bsr || !u || !f || !out) return kBadArg;
+  auto* B = static_cast<WnBsr*>(bsr);
+  Solver& Sv = *static_cast<Solver*>(solver);
+  Workspace& w = solver_ws(Sv);
+  int e = w.alloc(B->m.n_slices + 8);
+  if (e) return e;
+  const long long n = (long long)B->m.n_rows * B->m.br;
+  SpmvTune t = B->tune;
+  t.skip = nullptr;
+  if ((e = spmv_residual64(B->m, u, f, r, nullptr, nullptr, w, w.sc + kScTrue, t, S(stream)))) return e;
+  if ((e = launch_dot<double>(n, f, f, w, w.sc + kScF2, S(stream)))) return e;
+  double h[kScCount];
+  WN_HIP(hipMemcpyAsync(h, w.sc, sizeof(double) * kScCount, hipMemcpyDeviceToHost, S(stream)));
+  WN_HIP(hipStreamSynchronize(S(stream)));
+  out[0] = std::sqrt(h[kScTrue]);
+  out[1] = std::sqrt(h[kScF2]);
+  return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,414 @@
+// C3D10 element kernels for gfx950 (FP64).
+//
+// Element formulation follows wnfea/elements/c3d10.py: Abaqus node order
+// (corners 1-4, midsides 1-2, 2-3, 3-1, 1-4, 2-4, 3-4; :35-106), 4-point
+// Hammer rule a = (5-sqrt5)/20, b = (5+3sqrt5)/20, w = 1/24 (:22-32), isotropic
+// D (:192-216), Ke = sum_g w B^T D B det J with DOF order 3*node+comp
+// (:219-249), f_int = sum_g w B^T sigma det J (:252-283), Gauss-point stress
+// and von Mises (:286-318).
+//
+// Mapping: one 64-lane wavefront per element.  Lanes 0..39 each own one
+// (Gauss point, node) pair and compute that node's Cartesian gradient g_a at
+// that point (the 3x3 Jacobian is recomputed per lane, 30 FMAs, instead of a
+// cross-lane broadcast).  The isotropic block formula
+//   K_ab = sum_g w detJ (lam g_a g_b^T + mu g_b g_a^T + mu (g_a . g_b) I)
+// is algebraically B_a^T D B_b, so each of the 55 lanes owning an upper
+// block pair (a <= b) produces a full 3x3 block and also writes its transpose:
+// Ke comes out exactly symmetric, as the reference's 0.5 (Ke + Ke^T).
+#include "common.hpp"
+
+namespace wn {
+
+__constant__ double c_gauss[4][3];
+__constant__ double c_dndxi[4][3][10];  // natural derivatives per Gauss point
+
+struct ElemConsts {
+  double gauss[4][3];
+  double dndxi[4][3][10];
+};
+
+static ElemConsts make_consts() {
+  ElemConsts c{};
+  const double a = (5.0 - sqrt(5.0)) / 20.0, b = (5.0 + 3.0 * sqrt(5.0)) / 20.0;
+  const double pts[4][3] = {{a, a, a}, {b, a, a}, {a, b, a}, {a, a, b}};
+  for (int g = 0; g < 4; ++g) {
+    for (int k = 0; k < 3; ++k) c.gauss[g][k] = pts[g][k];
+    const double L1 = 1.0 - pts[g][0] - pts[g][1] - pts[g][2], L2 = pts[g][0], L3 = pts[g][1], L4 = pts[g][2];
+    // dN_i/dL_k for the 10 shape functions, then d/dxi_m = d/dL_{m+1} - d/dL_1
+    double dl[4][10] = {};
+    dl[0][0] = 4 * L1 - 1; dl[1][1] = 4 * L2 - 1; dl[2][2] = 4 * L3 - 1; dl[3][3] = 4 * L4 - 1;
+    dl[0][4] = 4 * L2; dl[1][4] = 4 * L1;
+    dl[1][5] = 4 * L3; dl[2][5] = 4 * L2;
+    dl[0][6] = 4 * L3; dl[2][6] = 4 * L1;
+    dl[0][7] = 4 * L4; dl[3][7] = 4 * L1;
+    dl[1][8] = 4 * L4; dl[3][8] = 4 * L2;
+    dl[2][9] = 4 * L4; dl[3][9] = 4 * L3;
+    for (int m = 0; m < 3; ++m)
+      for (int i = 0; i < 10; ++i) c.dndxi[g][m][i] = dl[m + 1][i] - dl[0][i];
+  }
+  return c;
+}
+
+// Uploads the quadrature tables to the current device once per process
+// (not thread-safe; call from the thread that owns the device).
+int upload_element_consts(hipStream_t s) {
+  static unsigned done_mask = 0;
+  int dev = 0;
+  WN_HIP(hipGetDevice(&dev));
+  if (dev < 32 && ((done_mask >> dev) & 1u)) return kOk;
+  const ElemConsts c = make_consts();
+  WN_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_gauss), c.gauss, sizeof(c.gauss), 0, hipMemcpyHostToDevice, s));
+  WN_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dndxi), c.dndxi, sizeof(c.dndxi), 0, hipMemcpyHostToDevice, s));
+  WN_HIP(hipStreamSynchronize(s));
+  if (dev < 32) done_mask |= 1u << dev;
+  return kOk;
+}
+
+// Cartesian gradient of node `a` at Gauss point `g`; returns det J.
+__device__ __forceinline__ double node_gradient(const double (*X)[3], int g, int a, double grad[3]) {
+  double J[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < 10; ++i) s = fma(c_dndxi[g][r][i], X[i][c], s);
+      J[r][c] = s;
+    }
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  const double id = 1.0 / det;
+  const double inv[3][3] = {
+      {c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id, (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+      {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+      {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id, (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+  const double d0 = c_dndxi[g][0][a], d1 = c_dndxi[g][1][a], d2 = c_dndxi[g][2][a];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) grad[r] = inv[r][0] * d0 + inv[r][1] * d1 + inv[r][2] * d2;
+  return det;
+}
+
+constexpr int kElemPerBlock = 4;  // 4 wavefronts, one element each
+
+// Ke[e][30][30] (row-major, DOF 3*node+comp).  bad_flag gets (e+1) of an element
+// with det J <= 0 (the smallest such index is not guaranteed; any one is reported).
+__global__ __launch_bounds__(256) void c3d10_ke_kernel(int M, const int* __restrict__ conn,
+                                                       const double* __restrict__ nodes,
+                                                       const double* __restrict__ E,
+                                                       const double* __restrict__ nu, double* __restrict__ Ke,
+                                                       int* __restrict__ bad_flag) {
+  __shared__ double sX[kElemPerBlock][10][3];
+  __shared__ double sG[kElemPerBlock][4][10][3];  // w detJ folded in separately
+  __shared__ double sW[kElemPerBlock][4];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int e = blockIdx.x * kElemPerBlock + w;
+  const bool live = e < M;
+  if (live && lane < 30) {
+    const int node = conn[(size_t)e * 10 + lane / 3];
+    sX[w][lane / 3][lane % 3] = nodes[(size_t)node * 3 + lane % 3];
+  }
+  __syncthreads();
+  if (live && lane < 40) {
+    const int g = lane / 10, a = lane % 10;
+    double grad[3];
+    const double det = node_gradient(sX[w], g, a, grad);
+    sG[w][g][a][0] = grad[0];
+    sG[w][g][a][1] = grad[1];
+    sG[w][g][a][2] = grad[2];
+    if (a == 0) {
+      sW[w][g] = det / 24.0;
+      if (!(det > 0.0)) atomicMax(bad_flag, e + 1);
+    }
+  }
+  __syncthreads();
+  if (!live || lane >= 55) return;
+  // lane -> upper-triangle pair (a, b), a <= b
+  int a = 0, rem = lane;
+  while (rem >= 10 - a) { rem -= 10 - a; ++a; }
+  const int b = a + rem;
+  const double Ev = E[e], nv = nu[e];
+  const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
+  const double lam = f * nv, mu = Ev / (2.0 * (1.0 + nv));
+  double K[3][3] = {};
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const double wd = sW[w][g];
+    const double ga[3] = {sG[w][g][a][0], sG[w][g][a][1], sG[w][g][a][2]};
+    const double gb[3] = {sG[w][g][b][0], sG[w][g][b][1], sG[w][g][b][2]};
+    const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double v = lam * ga[r] * gb[c] + mu * ga[c] * gb[r];
+        if (r == c) v += mu * dot;
+        K[r][c] = fma(wd, v, K[r][c]);
+      }
+  }
+  double* out = Ke + (size_t)e * 900;
+  if (a == b) {  // diagonal block: mirror so that rounding cannot break symmetry
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) out[(3 * a + r) * 30 + 3 * a + c] = 0.5 * (K[r][c] + K[c][r]);
+    return;
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      out[(3 * a + r) * 30 + 3 * b + c] = K[r][c];
+      out[(3 * b + c) * 30 + 3 * a + r] = K[r][c];
+    }
+}
+
+// Per-element internal forces fe[e][30] = Ke(e) u_e, matrix-free (no Ke), plus
+// optional Gauss-point stresses.  One thread per element, FP64.  U is read with
+// DOF stride `stride` (3 for solid vectors, 6 for the reference's mixed layout).
+// A Gauss point with |det J| < 1e-15 is skipped (kernels.cpp:312 semantics).
+__global__ __launch_bounds__(128) void c3d10_fint_elem_kernel(int M, const int* __restrict__ conn,
+                                                              const double* __restrict__ nodes,
+                                                              const double* __restrict__ E,
+                                                              const double* __restrict__ nu,
+                                                              const double* __restrict__ U, int stride,
+                                                              double* __restrict__ fe) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M) return;
+  double X[10][3], u[10][3], fo[10][3] = {};
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const int n = conn[(size_t)e * 10 + i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      X[i][c] = nodes[(size_t)n * 3 + c];
+      u[i][c] = U[(size_t)n * stride + c];
+    }
+  }
+  const double Ev = E[e], nv = nu[e];
+  const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
+  const double c11 = f * (1.0 - nv), c12 = f * nv, c44 = Ev / (2.0 * (1.0 + nv));
+  for (int g = 0; g < 4; ++g) {
+    double J[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 10; ++i) s = fma(c_dndxi[g][r][i], X[i][c], s);
+        J[r][c] = s;
+      }
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    if (fabs(det) < 1e-15) continue;
+    const double id = 1.0 / det;
+    const double inv[3][3] = {
+        {c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id, (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+        {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+        {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id, (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+    double gr[10][3];
+    double eps[6] = {};
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      const double d0 = c_dndxi[g][0][i], d1 = c_dndxi[g][1][i], d2 = c_dndxi[g][2][i];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) gr[i][r] = inv[r][0] * d0 + inv[r][1] * d1 + inv[r][2] * d2;
+      eps[0] += gr[i][0] * u[i][0];
+      eps[1] += gr[i][1] * u[i][1];
+      eps[2] += gr[i][2] * u[i][2];
+      eps[3] += gr[i][1] * u[i][0] + gr[i][0] * u[i][1];
+      eps[4] += gr[i][2] * u[i][1] + gr[i][1] * u[i][2];
+      eps[5] += gr[i][2] * u[i][0] + gr[i][0] * u[i][2];
+    }
+    const double sig[6] = {c11 * eps[0] + c12 * (eps[1] + eps[2]), c11 * eps[1] + c12 * (eps[0] + eps[2]),
+                           c11 * eps[2] + c12 * (eps[0] + eps[1]), c44 * eps[3], c44 * eps[4], c44 * eps[5]};
+    const double wd = det / 24.0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      fo[i][0] += wd * (gr[i][0] * sig[0] + gr[i][1] * sig[3] + gr[i][2] * sig[5]);
+      fo[i][1] += wd * (gr[i][1] * sig[1] + gr[i][0] * sig[3] + gr[i][2] * sig[4]);
+      fo[i][2] += wd * (gr[i][2] * sig[2] + gr[i][1] * sig[4] + gr[i][0] * sig[5]);
+    }
+  }
+  double* o = fe + (size_t)e * 30;
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[3 * i + c] = fo[i][c];
+}
+
+// Deterministic node-wise gather of element vectors: out[n*stride + c] =
+// (accumulate ? out : 0) + sum over the node's (element, local) entries in
+// element order of fe[e][3*local + c].
+__global__ void gather_node_kernel(int n_nodes, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
+                                   const double* __restrict__ fe, double* __restrict__ out, int stride,
+                                   int accumulate) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; ++k) {
+    const int code = n2e[k];  // e * 10 + local
+    const double* p = fe + (size_t)(code / 10) * 30 + 3 * (code % 10);
+    s0 += p[0];
+    s1 += p[1];
+    s2 += p[2];
+  }
+  double* o = out + (size_t)n * stride;
+  if (accumulate) {
+    s0 += o[0];
+    s1 += o[1];
+    s2 += o[2];
+  }
+  o[0] = s0;
+  o[1] = s1;
+  o[2] = s2;
+}
+
+// Gauss-point stresses -> cell averages (sigma, von Mises) per element; u has
+// 3 DOF per node.  Mirrors solid_solver.py:339-355 (mean of the 4 Gauss-point
+// stresses and of the 4 Gauss-point von Mises values).
+__global__ __launch_bounds__(128) void c3d10_stress_kernel(int M, const int* __restrict__ conn,
+                                                           const double* __restrict__ nodes,
+                                                           const double* __restrict__ E,
+                                                           const double* __restrict__ nu,
+                                                           const double* __restrict__ U,
+                                                           double* __restrict__ gp_sig,  // [M][4][6] or null
+                                                           double* __restrict__ gp_vm,   // [M][4] or null
+                                                           double* __restrict__ cell_sig,
+                                                           double* __restrict__ cell_vm, int* __restrict__ bad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M) return;
+  double X[10][3], u[10][3];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const int n = conn[(size_t)e * 10 + i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      X[i][c] = nodes[(size_t)n * 3 + c];
+      u[i][c] = U[(size_t)n * 3 + c];
+    }
+  }
+  const double Ev = E[e], nv = nu[e];
+  const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
+  const double c11 = f * (1.0 - nv), c12 = f * nv, c44 = Ev / (2.0 * (1.0 + nv));
+  double sacc[6] = {}, vacc = 0.0;
+  for (int g = 0; g < 4; ++g) {
+    double eps[6] = {};
+    double det = 0.0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      double gr[3];
+      det = node_gradient(X, g, i, gr);
+      eps[0] += gr[0] * u[i][0];
+      eps[1] += gr[1] * u[i][1];
+      eps[2] += gr[2] * u[i][2];
+      eps[3] += gr[1] * u[i][0] + gr[0] * u[i][1];
+      eps[4] += gr[2] * u[i][1] + gr[1] * u[i][2];
+      eps[5] += gr[2] * u[i][0] + gr[0] * u[i][2];
+    }
+    if (!(det > 0.0)) atomicMax(bad, e + 1);
+    const double s[6] = {c11 * eps[0] + c12 * (eps[1] + eps[2]), c11 * eps[1] + c12 * (eps[0] + eps[2]),
+                         c11 * eps[2] + c12 * (eps[0] + eps[1]), c44 * eps[3], c44 * eps[4], c44 * eps[5]};
+    const double vm = sqrt(0.5 * ((s[0] - s[1]) * (s[0] - s[1]) + (s[1] - s[2]) * (s[1] - s[2]) +
+                                  (s[2] - s[0]) * (s[2] - s[0]) + 6.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5])));
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      sacc[r] += s[r];
+      if (gp_sig) gp_sig[((size_t)e * 4 + g) * 6 + r] = s[r];
+    }
+    if (gp_vm) gp_vm[(size_t)e * 4 + g] = vm;
+    vacc += vm;
+  }
+#pragma unroll
+  for (int r = 0; r < 6; ++r) cell_sig[(size_t)e * 6 + r] = sacc[r] / 4.0;
+  cell_vm[e] = vacc / 4.0;
+}
+
+// Nodal average of cell von Mises over incident elements (deterministic gather).
+__global__ void nodal_average_kernel(int n_nodes, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
+                                     const double* __restrict__ cell_vm, double* __restrict__ nodal_vm) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  const int b = n2e_ptr[n], t = n2e_ptr[n + 1];
+  double s = 0.0;
+  for (int k = b; k < t; ++k) s += cell_vm[n2e[k] / 10];
+  nodal_vm[n] = (t > b) ? s / (t - b) : 0.0;
+}
+
+// Adjoint sensitivity of J(u) with dK/dE_e = K_e/E_e (K linear in E_e):
+//   dJ/dE_e = -(1/E_e) lambda_e^T K_e u_e, with fe = K_e u_e from the f_int kernel.
+__global__ void adjoint_dot_kernel(int M, const int* __restrict__ conn, const double* __restrict__ E,
+                                   const double* __restrict__ fe, const double* __restrict__ lam,
+                                   double* __restrict__ grad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M) return;
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const int n = conn[(size_t)e * 10 + i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s += lam[(size_t)n * 3 + c] * fe[(size_t)e * 30 + 3 * i + c];
+  }
+  grad[e] = -s / E[e];
+}
+
+// ---------------------------------------------------------------- host wrappers
+int launch_ke(int M, const int* conn, const double* nodes, const double* E, const double* nu, double* Ke, int* bad,
+              hipStream_t s) {
+  if (M <= 0) return kOk;
+  int r = upload_element_consts(s);
+  if (r) return r;
+  hipLaunchKernelGGL(c3d10_ke_kernel, dim3(ceil_div(M, kElemPerBlock)), dim3(256), 0, s, M, conn, nodes, E, nu, Ke,
+                     bad);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_fint_elem(int M, const int* conn, const double* nodes, const double* E, const double* nu, const double* U,
+                     int stride, double* fe, hipStream_t s) {
+  if (M <= 0) return kOk;
+  int r = upload_element_consts(s);
+  if (r) return r;
+  hipLaunchKernelGGL(c3d10_fint_elem_kernel, dim3(ceil_div(M, 128)), dim3(128), 0, s, M, conn, nodes, E, nu, U,
+                     stride, fe);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_gather_node(int n_nodes, const int* n2e_ptr, const int* n2e, const double* fe, double* out, int stride,
+                       int accumulate, hipStream_t s) {
+  if (n_nodes <= 0) return kOk;
+  hipLaunchKernelGGL(gather_node_kernel, dim3(ceil_div(n_nodes, 256)), dim3(256), 0, s, n_nodes, n2e_ptr, n2e, fe,
+                     out, stride, accumulate);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_stress(int M, const int* conn, const double* nodes, const double* E, const double* nu, const double* U,
+                  double* gp_sig, double* gp_vm, double* cell_sig, double* cell_vm, int* bad, hipStream_t s) {
+  if (M <= 0) return kOk;
+  int r = upload_element_consts(s);
+  if (r) return r;
+  hipLaunchKernelGGL(c3d10_stress_kernel, dim3(ceil_div(M, 128)), dim3(128), 0, s, M, conn, nodes, E, nu, U, gp_sig,
+                     gp_vm, cell_sig, cell_vm, bad);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_nodal_average(int n_nodes, const int* n2e_ptr, const int* n2e, const double* cell_vm, double* nodal_vm,
+                         hipStream_t s) {
+  if (n_nodes <= 0) return kOk;
+  hipLaunchKernelGGL(nodal_average_kernel, dim3(ceil_div(n_nodes, 256)), dim3(256), 0, s, n_nodes, n2e_ptr, n2e,
+                     cell_vm, nodal_vm);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_adjoint_dot(int M, const int* conn, const double* E, const double* fe, const double* lam, double* grad,
+                       hipStream_t s) {
+  if (M <= 0) return kOk;
+  hipLaunchKernelGGL(adjoint_dot_kernel, dim3(ceil_div(M, 256)), dim3(256), 0, s, M, conn, E, fe, lam, grad);
+  return static_cast<int>(hipGetLastError());
+}
+
+}  // namespace wn

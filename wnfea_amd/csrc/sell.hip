@@ -1,0 +1,408 @@
+// Sliced-ELL block matrices: host-side layout plan, value conversion and
+// assembly kernels, and the SpMV instantiations used by the solvers.
+#include <algorithm>
+#include <numeric>
+#include "wn.hpp"
+
+namespace wn {
+
+int Workspace::alloc(int need) {
+  if (need <= max_blocks && partials) return kOk;
+  release();
+  max_blocks = std::max(need, 4096);
+  WN_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)max_blocks));
+  WN_HIP(hipMalloc(&counter, sizeof(unsigned int) * 16));
+  WN_HIP(hipMalloc(&sc, sizeof(double) * kScCount));
+  WN_HIP(hipMemset(counter, 0, sizeof(unsigned int) * 16));
+  WN_HIP(hipMemset(sc, 0, sizeof(double) * kScCount));
+  return kOk;
+}
+
+void Workspace::release() {
+  if (partials) (void)hipFree(partials);
+  if (counter) (void)hipFree(counter);
+  if (sc) (void)hipFree(sc);
+  partials = nullptr;
+  counter = nullptr;
+  sc = nullptr;
+  max_blocks = 0;
+}
+
+void SellMatrix::release() {
+  if (eq_alias) {
+    delete eq_alias;  // non-owning view: nothing of its own to free
+    eq_alias = nullptr;
+  }
+  for (void* p : {(void*)d_off, (void*)d_len, (void*)d_row, (void*)d_col, (void*)d_slot, (void*)d_v64, (void*)d_v64e,
+                  (void*)d_v32})
+    if (p) (void)hipFree(p);
+  d_off = nullptr;
+  d_len = d_row = d_col = nullptr;
+  d_slot = nullptr;
+  d_v64 = nullptr;
+  d_v64e = nullptr;
+  d_v32 = nullptr;
+}
+
+SellMatrix& SellMatrix::equilibrated() {
+  if (!d_v64e) return *this;
+  if (!eq_alias) eq_alias = new SellMatrix();
+  SellMatrix& a = *eq_alias;
+  a.br = br; a.bc = bc; a.n_rows = n_rows; a.n_cols = n_cols; a.n_slices = n_slices; a.sigma = sigma;
+  a.nnzb = nnzb; a.slots = slots; a.d_off = d_off; a.d_len = d_len; a.d_row = d_row; a.d_col = d_col;
+  a.d_slot = d_slot; a.d_v64 = d_v64e; a.d_v64e = nullptr; a.d_v32 = d_v32;
+  return a;
+}
+
+// Host plan: stable sort of rows by length (descending) inside windows of
+// `sigma` rows, slices of 64 consecutive sorted rows, column-major slots.
+int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* rowptr, const int* colidx, int sigma_) {
+  if (br_ <= 0 || bc_ <= 0 || n_rows_ < 0 || n_cols_ < 0 || !rowptr || (!colidx && rowptr[n_rows_] > 0))
+    return kBadArg;
+  release();
+  br = br_;
+  bc = bc_;
+  n_rows = n_rows_;
+  n_cols = n_cols_;
+  sigma = sigma_ <= 0 ? 4096 : sigma_;
+  sigma = ((sigma + 63) / 64) * 64;
+  nnzb = rowptr[n_rows];
+  h_rowptr.assign(rowptr, rowptr + n_rows + 1);
+  h_colidx.assign(colidx, colidx + nnzb);
+  n_slices = (n_rows + 63) / 64;
+  std::vector<int> order(n_rows);
+  std::iota(order.begin(), order.end(), 0);
+  for (int w0 = 0; w0 < n_rows; w0 += sigma) {
+    const int w1 = std::min(n_rows, w0 + sigma);
+    std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
+      return (rowptr[a + 1] - rowptr[a]) > (rowptr[b + 1] - rowptr[b]);
+    });
+  }
+  std::vector<long long> off(n_slices + 1, 0);
+  std::vector<int> len(n_slices * 64, 0), row(n_slices * 64, -1);
+  for (int s = 0; s < n_slices; ++s) {
+    int mx = 0;
+    for (int l = 0; l < 64; ++l) {
+      const int k = s * 64 + l;
+      if (k < n_rows) {
+        const int r = order[k];
+        row[k] = r;
+        len[k] = rowptr[r + 1] - rowptr[r];
+        mx = std::max(mx, len[k]);
+      }
+    }
+    off[s + 1] = off[s] + 64LL * mx;
+  }
+  slots = off[n_slices];
+  std::vector<int> col(std::max<long long>(slots, 1), 0);
+  std::vector<long long> slot(std::max<long long>(nnzb, 1), 0);
+  for (int s = 0; s < n_slices; ++s)
+    for (int l = 0; l < 64; ++l) {
+      const int k = s * 64 + l;
+      if (row[k] < 0) continue;
+      const int r = row[k];
+      for (int j = 0; j < len[k]; ++j) {
+        const long long sl = off[s] + 64LL * j + l;
+        col[sl] = colidx[rowptr[r] + j];
+        slot[rowptr[r] + j] = sl;
+      }
+      // padding slots keep column 0 and value 0; lanes stop at their own
+      // length, so they are never read.
+    }
+  WN_HIP(hipMalloc(&d_off, sizeof(long long) * off.size()));
+  WN_HIP(hipMalloc(&d_len, sizeof(int) * len.size()));
+  WN_HIP(hipMalloc(&d_row, sizeof(int) * row.size()));
+  WN_HIP(hipMalloc(&d_col, sizeof(int) * col.size()));
+  WN_HIP(hipMalloc(&d_slot, sizeof(long long) * slot.size()));
+  WN_HIP(hipMemcpy(d_off, off.data(), sizeof(long long) * off.size(), hipMemcpyHostToDevice));
+  WN_HIP(hipMemcpy(d_len, len.data(), sizeof(int) * len.size(), hipMemcpyHostToDevice));
+  WN_HIP(hipMemcpy(d_row, row.data(), sizeof(int) * row.size(), hipMemcpyHostToDevice));
+  WN_HIP(hipMemcpy(d_col, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
+  WN_HIP(hipMemcpy(d_slot, slot.data(), sizeof(long long) * slot.size(), hipMemcpyHostToDevice));
+  return kOk;
+}
+
+int SellMatrix::ensure_values(int want64, int want32) {
+  const size_t nv = (size_t)std::max<long long>(slots, 1) * br * bc;
+  if (want64 && !d_v64) {
+    WN_HIP(hipMalloc(&d_v64, sizeof(double) * nv));
+    WN_HIP(hipMemset(d_v64, 0, sizeof(double) * nv));
+  }
+  if (want32 && !d_v32) {
+    WN_HIP(hipMalloc(&d_v32, sizeof(float) * nv));
+    WN_HIP(hipMemset(d_v32, 0, sizeof(float) * nv));
+  }
+  return kOk;
+}
+
+__device__ __forceinline__ long long value_base(long long slot, int bb) {
+  return slot * bb - (slot & 63) * (bb - 1);
+}
+
+__global__ void std_to_sell_kernel(long long nnzb, int bb, const long long* __restrict__ slot,
+                                   const double* __restrict__ vs, double* __restrict__ v64, float* __restrict__ v32) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= nnzb * bb) return;
+  const long long k = t / bb;
+  const int e = (int)(t - k * bb);
+  const long long p = value_base(slot[k], bb) + 64LL * e;
+  const double v = vs[t];
+  if (v64) v64[p] = v;
+  if (v32) v32[p] = float(v);
+}
+
+__global__ void sell_to_std_kernel(long long nnzb, int bb, const long long* __restrict__ slot,
+                                   const double* __restrict__ v64, double* __restrict__ vs) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= nnzb * bb) return;
+  const long long k = t / bb;
+  const int e = (int)(t - k * bb);
+  vs[t] = v64[value_base(slot[k], bb) + 64LL * e];
+}
+
+int sell_from_std(SellMatrix& A, const double* vals_std, int to64, int to32, hipStream_t s) {
+  int r = A.ensure_values(to64, to32);
+  if (r) return r;
+  const long long n = A.nnzb * A.br * A.bc;
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(std_to_sell_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, A.nnzb, A.br * A.bc, A.d_slot,
+                     vals_std, to64 ? A.d_v64 : nullptr, to32 ? A.d_v32 : nullptr);
+  return static_cast<int>(hipGetLastError());
+}
+
+int sell_to_std(const SellMatrix& A, double* vals_std, hipStream_t s) {
+  if (!A.d_v64) return kNotReady;
+  const long long n = A.nnzb * A.br * A.bc;
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(sell_to_std_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, A.nnzb, A.br * A.bc, A.d_slot,
+                     A.d_v64, vals_std);
+  return static_cast<int>(hipGetLastError());
+}
+
+// Deterministic assembly of 3x3 blocks.  Block k = (i, j) sums the entries of
+// its contributions in list order (element order, then local pair order), each
+// code = e*100 + a*10 + b addressing Ke[e][3a+r][3b+c].  Dirichlet semantics of
+// solid_solver.py:95-127 per scalar entry: a fixed row or column DOF zeroes the
+// entry, and the diagonal entry of a fixed DOF becomes 1.  node_fix[n] holds the
+// fixed-DOF bits of node n (bit c = DOF c fixed).
+__global__ void assemble_gather_kernel(long long nnzb, const int* __restrict__ rowidx, const int* __restrict__ colidx,
+                                       const long long* __restrict__ cptr, const int* __restrict__ codes,
+                                       const double* __restrict__ Ke, const unsigned char* __restrict__ node_fix,
+                                       const long long* __restrict__ slot, double* __restrict__ v64,
+                                       double* __restrict__ vstd) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= nnzb) return;
+  double acc[9] = {};
+  for (long long q = cptr[k]; q < cptr[k + 1]; ++q) {
+    const int code = codes[q];
+    const int e = code / 100, ab = code - e * 100, a = ab / 10, b = ab - a * 10;
+    const double* kp = Ke + (size_t)e * 900 + (3 * a) * 30 + 3 * b;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) acc[r * 3 + c] += kp[r * 30 + c];
+  }
+  const int i = rowidx[k], j = colidx[k];
+  const unsigned fi = node_fix ? node_fix[i] : 0u, fj = node_fix ? node_fix[j] : 0u;
+  const long long base = value_base(slot[k], 9);
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double v = acc[r * 3 + c];
+      if (((fi >> r) & 1u) || ((fj >> c) & 1u)) v = (i == j && r == c) ? 1.0 : 0.0;
+      v64[base + 64LL * (r * 3 + c)] = v;
+      if (vstd) vstd[k * 9 + r * 3 + c] = v;
+    }
+}
+
+int assemble_gather(SellMatrix& A, const int* d_rowidx, const int* d_colidx, const long long* cptr, const int* codes,
+                    const double* Ke, const unsigned char* node_fix, double* vals_std, hipStream_t s) {
+  if (A.br != 3 || A.bc != 3) return kBadArg;
+  int r = A.ensure_values(1, 0);
+  if (r) return r;
+  if (A.nnzb == 0) return kOk;
+  hipLaunchKernelGGL(assemble_gather_kernel, dim3(ceil_div(A.nnzb, 256)), dim3(256), 0, s, A.nnzb, d_rowidx, d_colidx,
+                     cptr, codes, Ke, node_fix, A.d_slot, A.d_v64, vals_std);
+  return static_cast<int>(hipGetLastError());
+}
+
+__global__ void extract_diag_kernel(int n_rows, int b, const long long* __restrict__ diag_block,
+                                    const long long* __restrict__ slot, const double* __restrict__ v64,
+                                    double* __restrict__ diag) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= (long long)n_rows * b) return;
+  const long long i = t / b;
+  const int r = (int)(t - i * b);
+  const long long k = diag_block[i];
+  diag[t] = (k >= 0) ? v64[value_base(slot[k], b * b) + 64LL * (r * b + r)] : 0.0;
+}
+
+int extract_diag(const SellMatrix& A, const long long* diag_block, double* diag, hipStream_t s) {
+  if (A.br != A.bc || !A.d_v64) return kBadArg;
+  const long long n = (long long)A.n_rows * A.br;
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(extract_diag_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, A.n_rows, A.br, diag_block,
+                     A.d_slot, A.d_v64, diag);
+  return static_cast<int>(hipGetLastError());
+}
+
+// v32 (and v64e) = s_i * v64 * s_j per scalar entry (symmetric diagonal equilibration).
+__global__ void equilibrate_kernel(long long nnzb, int b, const int* __restrict__ rowidx,
+                                   const int* __restrict__ colidx, const long long* __restrict__ slot,
+                                   const double* __restrict__ scale, const double* __restrict__ v64,
+                                   float* __restrict__ v32, double* __restrict__ v64e) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const int bb = b * b;
+  if (t >= nnzb * bb) return;
+  const long long k = t / bb;
+  const int e = (int)(t - k * bb), r = e / b, c = e - r * b;
+  const long long p = value_base(slot[k], bb) + 64LL * e;
+  const double v = scale[(long long)rowidx[k] * b + r] * v64[p] * scale[(long long)colidx[k] * b + c];
+  v32[p] = float(v);
+  if (v64e) v64e[p] = v;
+}
+
+int equilibrate(SellMatrix& A, const int* d_rowidx, const int* d_colidx, const double* scale, int want64e,
+                hipStream_t s) {
+  if (A.br != A.bc || !A.d_v64) return kBadArg;
+  int r = A.ensure_values(0, 1);
+  if (r) return r;
+  const size_t nv = (size_t)std::max<long long>(A.slots, 1) * A.br * A.bc;
+  if (want64e && !A.d_v64e) {
+    WN_HIP(hipMalloc(&A.d_v64e, sizeof(double) * nv));
+    WN_HIP(hipMemset(A.d_v64e, 0, sizeof(double) * nv));
+  }
+  const long long n = A.nnzb * A.br * A.bc;
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(equilibrate_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, A.nnzb, A.br, d_rowidx, d_colidx,
+                     A.d_slot, scale, A.d_v64, A.d_v32, want64e ? A.d_v64e : nullptr);
+  return static_cast<int>(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- SpMV dispatch
+template <int BR, int BC, typename VT, typename XT, class Op>
+static int launch_sell(const SellMatrix& A, const XT* x, const Op& op, const SpmvTune& t, hipStream_t s) {
+  if (A.n_slices == 0) return kOk;
+  const SellView<VT> v = A.view<VT>();
+  if (!v.val) return kNotReady;
+  const int per = (A.n_slices + 7) / 8;
+  const int grid = t.xcd_remap ? per * 8 : A.n_slices;
+  switch (t.waves) {
+    case 2:
+      hipLaunchKernelGGL((sell_spmv_kernel<BR, BC, VT, XT, 2, Op>), dim3(grid), dim3(128), 0, s, v, x, op, t.xcd_remap, t.skip);
+      break;
+    case 4:
+      hipLaunchKernelGGL((sell_spmv_kernel<BR, BC, VT, XT, 4, Op>), dim3(grid), dim3(256), 0, s, v, x, op, t.xcd_remap, t.skip);
+      break;
+    default:
+      hipLaunchKernelGGL((sell_spmv_kernel<BR, BC, VT, XT, 1, Op>), dim3(grid), dim3(64), 0, s, v, x, op, t.xcd_remap, t.skip);
+  }
+  return static_cast<int>(hipGetLastError());
+}
+
+#define WN_BLOCK_DISPATCH(A, CALL)                     \
+  do {                                                 \
+    if ((A).br == 3 && (A).bc == 3) { CALL(3, 3); }    \
+    else if ((A).br == 6 && (A).bc == 6) { CALL(6, 6); } \
+    else if ((A).br == 3 && (A).bc == 6) { CALL(3, 6); } \
+    else if ((A).br == 6 && (A).bc == 3) { CALL(6, 3); } \
+    else return kBadArg;                               \
+  } while (0)
+
+template <typename VT, typename XT>
+int spmv_axpby(const SellMatrix& A, const XT* x, XT* y, double alpha, double beta, const SpmvTune& t, hipStream_t s) {
+  using AT = typename AccT<VT>::type;
+#define CALL(R, C)                                                                 \
+  {                                                                                \
+    OpAxpby<R, XT, AT> op{y, AT(alpha), AT(beta)};                                 \
+    return launch_sell<R, C, VT, XT>(A, x, op, t, s);                              \
+  }
+  WN_BLOCK_DISPATCH(A, CALL);
+#undef CALL
+}
+template int spmv_axpby<double, double>(const SellMatrix&, const double*, double*, double, double, const SpmvTune&,
+                                        hipStream_t);
+template int spmv_axpby<float, float>(const SellMatrix&, const float*, float*, double, double, const SpmvTune&,
+                                      hipStream_t);
+
+template <typename VT>
+int spmv_dot(const SellMatrix& A, const VT* p, VT* y, Workspace& w, int with_alpha, const SpmvTune& t,
+             hipStream_t s) {
+  using AT = typename AccT<VT>::type;
+  if (A.br != A.bc) return kBadArg;
+  int r = w.alloc(A.n_slices + 8);
+  if (r) return r;
+#define CALL(R, C)                                                                                          \
+  {                                                                                                         \
+    OpSpmvDot<R, VT, AT> op{y, p, w.partials, w.counter, w.sc + kScGamma, w.sc + kScRho,                    \
+                            with_alpha ? w.sc + kScAlpha : nullptr};                                        \
+    return launch_sell<R, C, VT, VT>(A, p, op, t, s);                                                       \
+  }
+  if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
+#undef CALL
+}
+template int spmv_dot<double>(const SellMatrix&, const double*, double*, Workspace&, int, const SpmvTune&,
+                              hipStream_t);
+template int spmv_dot<float>(const SellMatrix&, const float*, float*, Workspace&, int, const SpmvTune&, hipStream_t);
+
+int spmv_residual64(const SellMatrix& A, const double* x, const double* b, double* r, float* rs, const double* scale,
+                    Workspace& w, double* out, const SpmvTune& t, hipStream_t s) {
+  if (A.br != A.bc) return kBadArg;
+  int e = w.alloc(A.n_slices + 8);
+  if (e) return e;
+#define CALL(R, C)                                                         \
+  {                                                                        \
+    OpResidual<R, double> op{b, r, rs, scale, w.partials, w.counter, out}; \
+    return launch_sell<R, C, double, double>(A, x, op, t, s);              \
+  }
+  if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
+#undef CALL
+}
+
+template <typename T>
+int spmv_cheb(const SellMatrix& A, const T* x_in, T* x_out, const T* b, const T* dinv, T* d, double c1, double c2,
+              const SpmvTune& t, hipStream_t s) {
+  if (A.br != A.bc) return kBadArg;
+#define CALL(R, C)                                                  \
+  {                                                                 \
+    OpCheb<R, T> op{b, dinv, x_in, x_out, d, T(c1), T(c2)};         \
+    return launch_sell<R, C, T, T>(A, x_in, op, t, s);              \
+  }
+  if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
+#undef CALL
+}
+template int spmv_cheb<float>(const SellMatrix&, const float*, float*, const float*, const float*, float*, double,
+                              double, const SpmvTune&, hipStream_t);
+template int spmv_cheb<double>(const SellMatrix&, const double*, double*, const double*, const double*, double*,
+                               double, double, const SpmvTune&, hipStream_t);
+
+template <typename T>
+int spmv_defect(const SellMatrix& A, const T* x, const T* b, T* res, const SpmvTune& t, hipStream_t s) {
+  if (A.br != A.bc) return kBadArg;
+#define CALL(R, C)                                     \
+  {                                                    \
+    OpDefect<R, T> op{b, res};                         \
+    return launch_sell<R, C, T, T>(A, x, op, t, s);    \
+  }
+  if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
+#undef CALL
+}
+template int spmv_defect<float>(const SellMatrix&, const float*, const float*, float*, const SpmvTune&, hipStream_t);
+template int spmv_defect<double>(const SellMatrix&, const double*, const double*, double*, const SpmvTune&,
+                                 hipStream_t);
+
+template <typename T>
+int spmv_addto(const SellMatrix& A, const T* e, T* x, const SpmvTune& t, hipStream_t s) {
+#define CALL(R, C)                                  \
+  {                                                 \
+    OpAddTo<R, T> op{x};                            \
+    return launch_sell<R, C, T, T>(A, e, op, t, s); \
+  }
+  WN_BLOCK_DISPATCH(A, CALL);
+#undef CALL
+}
+template int spmv_addto<float>(const SellMatrix&, const float*, float*, const SpmvTune&, hipStream_t);
+template int spmv_addto<double>(const SellMatrix&, const double*, double*, const SpmvTune&, hipStream_t);
+
+}  // namespace wn

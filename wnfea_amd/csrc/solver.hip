@@ -1,0 +1,454 @@
+// Smoothed-aggregation AMG V-cycle and the PCG / iterative-refinement drivers.
+//
+// V-cycle (per level l < coarsest), working precision T:
+//   pre-smooth  : Chebyshev(deg) on D^-1 A from x = 0   (1 elementwise + deg-1 SpMV kernels)
+//   defect      : res = b - A x                        (1 SpMV)
+//   restrict    : b_{l+1} = R res                      (1 SpMV with R = P^T stored explicitly)
+//   recurse
+//   prolongate  : x += P x_{l+1}                       (1 SpMV)
+//   post-smooth : Chebyshev(deg) from x                (deg SpMV kernels)
+// Coarsest level: x = A_c^{-1} b with the FP64 inverse (dense GEMV in FP64 for
+// every working precision).  Pre- and post-smoothers use the same polynomial,
+// so the cycle is a symmetric preconditioner for PCG.
+//
+// PCG keeps rho/alpha/beta/||r||^2 on the device; a done flag raised by the
+// x/r update makes every later kernel of the loop return at once, so the host
+// enqueues iterations in chunks and checks the flag only between chunks.
+#include <cmath>
+#include <memory>
+#include <utility>
+#include "wn.hpp"
+
+namespace wn {
+
+struct AmgLevel {
+  SellMatrix* A = nullptr;  // not owned for level 0
+  bool own_A = false;
+  SellMatrix P, R;
+  bool has_PR = false;
+  int n = 0;  // scalar rows
+  double* dinv64 = nullptr;
+  float* dinv32 = nullptr;
+  double lmax = 2.0, ratio = 30.0;
+  int degree = 2;
+  // level vectors (b, x, tmp, d, res) per precision
+  double* v64[5] = {};
+  float* v32[5] = {};
+};
+
+struct Amg {
+  std::vector<std::unique_ptr<AmgLevel>> L;
+  int n_coarse = 0;
+  double* inv = nullptr;  // n_coarse^2, row-major
+  SpmvTune tune;
+  ~Amg() { release(); }
+  void release() {
+    for (auto& lp : L) {
+      AmgLevel& l = *lp;
+      if (l.own_A && l.A) {
+        l.A->release();
+        delete l.A;
+      }
+      l.P.release();
+      l.R.release();
+      for (auto* p : l.v64) if (p) (void)hipFree(p);
+      for (auto* p : l.v32) if (p) (void)hipFree(p);
+      if (l.dinv64) (void)hipFree(l.dinv64);
+      if (l.dinv32) (void)hipFree(l.dinv32);
+    }
+    L.clear();
+    if (inv) (void)hipFree(inv);
+    inv = nullptr;
+  }
+};
+
+template <typename T> T** level_vecs(AmgLevel& l);
+template <> double** level_vecs<double>(AmgLevel& l) { return l.v64; }
+template <> float** level_vecs<float>(AmgLevel& l) { return l.v32; }
+template <typename T> T* level_dinv(AmgLevel& l);
+template <> double* level_dinv<double>(AmgLevel& l) { return l.dinv64; }
+template <> float* level_dinv<float>(AmgLevel& l) { return l.dinv32; }
+
+int amg_alloc_vectors(Amg& g, int prec_mask) {
+  for (auto& lp : g.L) {
+    AmgLevel& l = *lp;
+    const size_t n = std::max(l.n, 1);
+    for (int k = 0; k < 5; ++k) {
+      if ((prec_mask & 1) && !l.v64[k]) WN_HIP(hipMalloc(&l.v64[k], sizeof(double) * n));
+      if ((prec_mask & 2) && !l.v32[k]) WN_HIP(hipMalloc(&l.v32[k], sizeof(float) * n));
+    }
+  }
+  return kOk;
+}
+
+// d = c2 * dinv .* b;  x = d
+template <typename T>
+__global__ void cheb_first_kernel(long long n, const T* __restrict__ b, const T* __restrict__ dinv, T* __restrict__ d,
+                                  T* __restrict__ x, T c2, const int* skip) {
+  if (skip && *skip) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const T v = c2 * dinv[i] * b[i];
+    d[i] = v;
+    x[i] = v;
+  }
+}
+
+// x = inv * b on the coarsest level: one 256-thread block per row, FP64.
+template <typename T>
+__global__ __launch_bounds__(256) void coarse_gemv_kernel(int n, const double* __restrict__ inv,
+                                                          const T* __restrict__ b, T* __restrict__ x,
+                                                          const int* skip) {
+  if (skip && *skip) return;
+  const int i = blockIdx.x;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) s += inv[(size_t)i * n + j] * double(b[j]);
+  __shared__ double sm[4];
+  s = block_sum(s, sm);
+  if (threadIdx.x == 0) x[i] = T(s);
+}
+
+struct ChebCoef {
+  double c1[16], c2[16];
+};
+
+static ChebCoef cheb_coefs(double lmax, double ratio, int deg) {
+  ChebCoef c{};
+  const double lmin = lmax / ratio;
+  const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin), sig = th / de;
+  double rho = 1.0 / sig;
+  c.c1[0] = 0.0;
+  c.c2[0] = 1.0 / th;
+  for (int k = 1; k < deg && k < 16; ++k) {
+    const double rn = 1.0 / (2.0 * sig - rho);
+    c.c1[k] = rn * rho;
+    c.c2[k] = 2.0 * rn / de;
+    rho = rn;
+  }
+  return c;
+}
+
+template <typename T>
+static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t s) {
+  AmgLevel& L = *g.L[l];
+  SpmvTune t = g.tune;
+  t.skip = skip;
+  const long long n = L.n;
+  if (l == (int)g.L.size() - 1) {
+    if (L.n != g.n_coarse || !g.inv) return kNotReady;
+    hipLaunchKernelGGL(coarse_gemv_kernel<T>, dim3(std::max(L.n, 1)), dim3(256), 0, s, L.n, g.inv, b, x, skip);
+    return static_cast<int>(hipGetLastError());
+  }
+  T** v = level_vecs<T>(L);
+  T* tmp = v[2];
+  T* d = v[3];
+  T* res = v[4];
+  const T* dinv = level_dinv<T>(L);
+  if (!tmp || !dinv || !L.has_PR) return kNotReady;
+  const int deg = std::max(1, std::min(L.degree, 16));
+  const ChebCoef c = cheb_coefs(L.lmax, L.ratio, deg);
+  const SellMatrix& A = *L.A;
+  int r;
+  // Buffer plan: the post-smoother (deg SpMV steps, ping-pong) must end in x,
+  // so it starts in `post`; the pre-smoother (1 elementwise + deg-1 SpMV
+  // steps) is started so that it ends in `post` too, and the prolongation
+  // adds in place there.  No copies.
+  T* post = (deg % 2 == 0) ? x : tmp;
+  T* cur = ((deg - 1) % 2 == 0) ? post : (post == x ? tmp : x);
+  T* oth = (cur == x) ? tmp : x;
+  hipLaunchKernelGGL(cheb_first_kernel<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, b, dinv, d, cur, T(c.c2[0]), skip);
+  if ((r = static_cast<int>(hipGetLastError()))) return r;
+  for (int k = 1; k < deg; ++k) {
+    if ((r = spmv_cheb<T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
+    std::swap(cur, oth);
+  }
+  // defect and restriction
+  if ((r = spmv_defect<T>(A, cur, b, res, t, s))) return r;
+  AmgLevel& N = *g.L[l + 1];
+  T** nv = level_vecs<T>(N);
+  if ((r = spmv_axpby<T, T>(L.R, res, nv[0], 1.0, 0.0, t, s))) return r;
+  if ((r = vcycle<T>(g, l + 1, nv[0], nv[1], skip, s))) return r;
+  if ((r = spmv_addto<T>(L.P, nv[1], cur, t, s))) return r;  // cur == post here
+  oth = (cur == x) ? tmp : x;
+  for (int k = 0; k < deg; ++k) {
+    if ((r = spmv_cheb<T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
+    std::swap(cur, oth);
+  }
+  return kOk;
+}
+
+template <typename T>
+int amg_apply(Amg& g, const T* r, T* z, const int* skip, hipStream_t s) {
+  if (g.L.empty()) return kNotReady;
+  return vcycle<T>(g, 0, r, z, skip, s);
+}
+template int amg_apply<float>(Amg&, const float*, float*, const int*, hipStream_t);
+template int amg_apply<double>(Amg&, const double*, double*, const int*, hipStream_t);
+
+// ------------------------------------------------------------------ PCG
+template <typename T>
+struct KVec {
+  T *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  long long n = 0;
+  int ensure(long long n_) {
+    if (n_ <= n && r) return kOk;
+    release();
+    n = n_;
+    const size_t bytes = sizeof(T) * std::max<long long>(n, 1);
+    WN_HIP(hipMalloc(&r, bytes));
+    WN_HIP(hipMalloc(&z, bytes));
+    WN_HIP(hipMalloc(&p, bytes));
+    WN_HIP(hipMalloc(&q, bytes));
+    return kOk;
+  }
+  void release() {
+    for (T* v : {r, z, p, q})
+      if (v) (void)hipFree(v);
+    r = z = p = q = nullptr;
+    n = 0;
+  }
+};
+
+struct Solver {
+  Workspace ws;
+  int* flags = nullptr;  // [0] done, [1] iterations
+  int* h_flags = nullptr;
+  double* h_sc = nullptr;
+  KVec<double> k64;
+  KVec<float> k32;
+  double* r64 = nullptr;  // FP64 residual of the refinement loop
+  float* rhs32 = nullptr;
+  float* d32 = nullptr;
+  long long n_ir = 0;
+  int init() {
+    if (flags) return kOk;
+    WN_HIP(hipMalloc(&flags, sizeof(int) * 4));
+    WN_HIP(hipMemset(flags, 0, sizeof(int) * 4));
+    WN_HIP(hipHostMalloc(&h_flags, sizeof(int) * 4, 0));
+    WN_HIP(hipHostMalloc(&h_sc, sizeof(double) * kScCount, 0));
+    return ws.alloc(4096);
+  }
+  ~Solver() {
+    ws.release();
+    k64.release();
+    k32.release();
+    if (flags) (void)hipFree(flags);
+    if (h_flags) (void)hipHostFree(h_flags);
+    if (h_sc) (void)hipHostFree(h_sc);
+    for (void* p : {(void*)r64, (void*)rhs32, (void*)d32})
+      if (p) (void)hipFree(p);
+  }
+};
+
+__global__ void pcg_start_kernel(double* sc, int* flags, double rtol2) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc[kScRR0] = sc[kScRR];
+    sc[kScTol2] = rtol2 * sc[kScRR];
+    flags[0] = (sc[kScRR] == 0.0) ? 1 : 0;
+    flags[1] = 0;
+  }
+}
+
+// Preconditioner kinds: 0 none, 1 Jacobi (dinv), 2 AMG.
+template <typename T>
+int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, const T* b, T* x, double rtol,
+            int max_iter, int check_every, const SpmvTune& tune0, hipStream_t s, int* iters, double* rel) {
+  KVec<T>& k = *reinterpret_cast<KVec<T>*>(std::is_same<T, double>::value ? (void*)&S.k64 : (void*)&S.k32);
+  const long long n = (long long)A.n_rows * A.br;
+  int r;
+  if ((r = S.init())) return r;
+  if ((r = k.ensure(n))) return r;
+  if ((r = S.ws.alloc(A.n_slices + 8))) return r;
+  SpmvTune t = tune0;
+  t.skip = S.flags;
+  if (check_every <= 0) check_every = 8;
+  // x = 0, r = b, rr = b.b
+  if ((r = launch_fill<T>(n, x, T(0), s))) return r;
+  if ((r = launch_cast_scale<T, T>(n, b, nullptr, 1.0, k.r, s))) return r;
+  if ((r = launch_dot<T>(n, k.r, k.r, S.ws, S.ws.sc + kScRR, s))) return r;
+  hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(64), 0, s, S.ws.sc, S.flags, rtol * rtol);
+  int done_iters = 0;
+  for (int it = 0; it < max_iter;) {
+    const int chunk = std::min(check_every, max_iter - it);
+    for (int c = 0; c < chunk; ++c, ++it) {
+      const int first = (it == 0);
+      if (pkind == 1) {
+        if ((r = launch_jacobi_dot<T>(n, k.r, dinv, k.z, S.ws, first, S.flags, s))) return r;
+      } else if (pkind == 2) {
+        if ((r = amg_apply<T>(*amg, k.r, k.z, S.flags, s))) return r;
+        if ((r = launch_rho<T>(n, k.r, k.z, S.ws, first, S.flags, s))) return r;
+      } else {
+        if ((r = launch_rho<T>(n, k.r, k.r, S.ws, first, S.flags, s))) return r;
+      }
+      if ((r = launch_update_p<T>(n, pkind ? k.z : k.r, k.p, S.ws, S.flags, s))) return r;
+      if ((r = spmv_dot<T>(A, k.p, k.q, S.ws, 1, t, s))) return r;
+      if ((r = launch_update_xr<T>(n, x, k.r, k.p, k.q, S.ws, S.flags, s))) return r;
+    }
+    WN_HIP(hipMemcpyAsync(S.h_flags, S.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    WN_HIP(hipStreamSynchronize(s));
+    done_iters = S.h_flags[1];
+    if (S.h_flags[0]) break;
+  }
+  WN_HIP(hipMemcpyAsync(S.h_sc, S.ws.sc, sizeof(double) * kScCount, hipMemcpyDeviceToHost, s));
+  WN_HIP(hipStreamSynchronize(s));
+  if (iters) *iters = done_iters;
+  if (rel) *rel = S.h_sc[kScRR0] > 0.0 ? std::sqrt(S.h_sc[kScRR] / S.h_sc[kScRR0]) : 0.0;
+  return kOk;
+}
+template int pcg_run<double>(Solver&, const SellMatrix&, int, const double*, Amg*, const double*, double*, double, int,
+                             int, const SpmvTune&, hipStream_t, int*, double*);
+template int pcg_run<float>(Solver&, const SellMatrix&, int, const float*, Amg*, const float*, float*, double, int, int,
+                            const SpmvTune&, hipStream_t, int*, double*);
+
+// ------------------------------------------------------------------ refinement
+// FP64 iterative refinement around an FP32 (equilibrated) inner PCG:
+//   r = f - K u (FP64, K = A.d_v64);  rhat = s .* r;  solve Ahat dhat = rhat/|rhat|
+//   in FP32 (A.d_v32 = S K S) to relative inner_rtol;  u += |rhat| s .* dhat.
+// Stops when ||f - K u|| <= tol ||f||.  Returns total inner iterations.
+int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float* dinv32, Amg* amg, const double* f,
+           double* u, double tol, double inner_rtol, int max_outer, int max_inner, int check_every,
+           const SpmvTune& tune, hipStream_t s, int* outer_out, int* inner_out, double* rel_out, int zero_init) {
+  const long long n = (long long)A.n_rows * A.br;
+  int r;
+  if ((r = S.init())) return r;
+  if ((r = S.ws.alloc(A.n_slices + 8))) return r;
+  if (S.n_ir < n) {
+    for (void* p : {(void*)S.r64, (void*)S.rhs32, (void*)S.d32})
+      if (p) (void)hipFree(p);
+    WN_HIP(hipMalloc(&S.r64, sizeof(double) * n));
+    WN_HIP(hipMalloc(&S.rhs32, sizeof(float) * n));
+    WN_HIP(hipMalloc(&S.d32, sizeof(float) * n));
+    S.n_ir = n;
+  }
+  if (!A.d_v64 || !A.d_v32) return kNotReady;
+  if (zero_init && (r = launch_fill<double>(n, u, 0.0, s))) return r;
+  if ((r = launch_dot<double>(n, f, f, S.ws, S.ws.sc + kScF2, s))) return r;
+  int total_inner = 0, outer = 0;
+  double rel = 0.0;
+  SpmvTune t = tune;
+  t.skip = nullptr;
+  for (;; ++outer) {
+    if ((r = spmv_residual64(A, u, f, S.r64, S.rhs32, scale, S.ws, S.ws.sc + kScTrue, t, s))) return r;
+    WN_HIP(hipMemcpyAsync(S.h_sc, S.ws.sc, sizeof(double) * kScCount, hipMemcpyDeviceToHost, s));
+    WN_HIP(hipStreamSynchronize(s));
+    const double f2 = S.h_sc[kScF2];
+    rel = f2 > 0.0 ? std::sqrt(S.h_sc[kScTrue] / f2) : 0.0;
+    if (rel <= tol || outer >= max_outer || !(rel == rel)) break;
+    // normalised FP32 right-hand side
+    if ((r = launch_dot<float>(n, S.rhs32, S.rhs32, S.ws, S.ws.sc + kScInnerN, s))) return r;
+    if ((r = launch_normalise<float>(n, S.rhs32, S.rhs32, S.ws.sc + kScInnerN, s))) return r;
+    int it = 0;
+    double irel = 0.0;
+    if ((r = pcg_run<float>(S, A, pkind, dinv32, amg, S.rhs32, S.d32, inner_rtol, max_inner, check_every, tune, s,
+                            &it, &irel)))
+      return r;
+    total_inner += it;
+    if ((r = launch_ir_update<float>(n, u, S.d32, scale, S.ws.sc + kScInnerN, s))) return r;
+  }
+  if (outer_out) *outer_out = outer;
+  if (inner_out) *inner_out = total_inner;
+  if (rel_out) *rel_out = rel;
+  return kOk;
+}
+
+}  // namespace wn
+
+namespace wn {
+
+Solver* solver_new() { return new Solver(); }
+void solver_delete(Solver* s) { delete s; }
+Workspace& solver_ws(Solver& s) {
+  (void)s.init();
+  return s.ws;
+}
+
+Amg* amg_new() { return new Amg(); }
+void amg_delete(Amg* g) { delete g; }
+int amg_resize(Amg& g, int n_levels) {
+  g.release();
+  for (int i = 0; i < n_levels; ++i) g.L.emplace_back(new AmgLevel());
+  return kOk;
+}
+
+// Level lvl: operator A (level 0: the fine matrix's equilibrated values, given
+// by `fine`; other levels: host standard-layout block arrays), point-Jacobi
+// inverse diagonal (host), Chebyshev interval [lmax/ratio, lmax] and degree,
+// and -- except on the coarsest level -- the prolongator P (br x bc_coarse
+// blocks) and restriction R = P^T (bc_coarse x br blocks), host arrays.
+int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const int* a_rowptr, const int* a_colidx,
+                  const double* a_vals, const double* dinv, double lmax, double ratio, int degree, int bc_coarse,
+                  int n_coarse_rows, const int* p_rowptr, const int* p_colidx, const double* p_vals,
+                  const int* r_rowptr, const int* r_colidx, const double* r_vals, int sigma) {
+  if (lvl < 0 || lvl >= (int)g.L.size() || br <= 0 || n_rows < 0 || degree < 1 || degree > 16 || !(lmax > 0.0) ||
+      !(ratio > 1.0))
+    return kBadArg;
+  AmgLevel& L = *g.L[lvl];
+  int r;
+  if (fine) {
+    if (fine->br != br || fine->n_rows != n_rows || !fine->d_v32) return kBadArg;
+    L.A = &fine->equilibrated();
+    L.own_A = false;
+  } else {
+    if (!a_rowptr || !a_vals) return kBadArg;
+    L.A = new SellMatrix();
+    L.own_A = true;
+    if ((r = L.A->build(br, br, n_rows, n_rows, a_rowptr, a_colidx, sigma))) return r;
+    double* dv = nullptr;
+    const size_t nv = (size_t)L.A->nnzb * br * br;
+    WN_HIP(hipMalloc(&dv, sizeof(double) * std::max<size_t>(nv, 1)));
+    WN_HIP(hipMemcpy(dv, a_vals, sizeof(double) * nv, hipMemcpyHostToDevice));
+    r = sell_from_std(*L.A, dv, 1, 1, nullptr);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(dv);
+    if (r) return r;
+  }
+  L.n = n_rows * br;
+  L.lmax = lmax;
+  L.ratio = ratio;
+  L.degree = degree;
+  if (!L.dinv64) WN_HIP(hipMalloc(&L.dinv64, sizeof(double) * std::max(L.n, 1)));
+  if (!L.dinv32) WN_HIP(hipMalloc(&L.dinv32, sizeof(float) * std::max(L.n, 1)));
+  std::vector<float> d32(L.n);
+  for (int i = 0; i < L.n; ++i) d32[i] = static_cast<float>(dinv[i]);
+  WN_HIP(hipMemcpy(L.dinv64, dinv, sizeof(double) * L.n, hipMemcpyHostToDevice));
+  WN_HIP(hipMemcpy(L.dinv32, d32.data(), sizeof(float) * L.n, hipMemcpyHostToDevice));
+  L.has_PR = false;
+  if (p_rowptr) {
+    if (!p_vals || !r_rowptr || !r_vals || bc_coarse <= 0 || n_coarse_rows < 0) return kBadArg;
+    auto upload = [&](SellMatrix& M, int rb, int cb, int rows, int cols, const int* rp, const int* ci,
+                      const double* vals) -> int {
+      int e = M.build(rb, cb, rows, cols, rp, ci, sigma);
+      if (e) return e;
+      double* dv = nullptr;
+      const size_t nv = (size_t)M.nnzb * rb * cb;
+      WN_HIP(hipMalloc(&dv, sizeof(double) * std::max<size_t>(nv, 1)));
+      WN_HIP(hipMemcpy(dv, vals, sizeof(double) * nv, hipMemcpyHostToDevice));
+      e = sell_from_std(M, dv, 1, 1, nullptr);
+      (void)hipDeviceSynchronize();
+      (void)hipFree(dv);
+      return e;
+    };
+    if ((r = upload(L.P, br, bc_coarse, n_rows, n_coarse_rows, p_rowptr, p_colidx, p_vals))) return r;
+    if ((r = upload(L.R, bc_coarse, br, n_coarse_rows, n_rows, r_rowptr, r_colidx, r_vals))) return r;
+    L.has_PR = true;
+  }
+  return kOk;
+}
+
+int amg_set_coarse(Amg& g, int n, const double* inv) {
+  if (g.inv) (void)hipFree(g.inv);
+  g.inv = nullptr;
+  WN_HIP(hipMalloc(&g.inv, sizeof(double) * (size_t)n * n));
+  WN_HIP(hipMemcpy(g.inv, inv, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice));
+  g.n_coarse = n;
+  return kOk;
+}
+
+int amg_finalize(Amg& g, int prec_mask, const SpmvTune& t) {
+  g.tune = t;
+  for (size_t l = 0; l + 1 < g.L.size(); ++l)
+    if (!g.L[l]->A || !g.L[l]->has_PR) return kNotReady;
+  if (g.L.empty() || !g.L.back()->A || g.n_coarse != g.L.back()->n) return kNotReady;
+  return amg_alloc_vectors(g, prec_mask);
+}
+
+}  // namespace wn

@@ -1,0 +1,250 @@
+"""Smoothed-aggregation AMG for 3-DOF (C3D10) and 6-DOF (beam) systems.
+
+Setup (host, once per matrix) follows the structure of the reference's
+``BlockBeamAMGPreconditioner`` (``wnfea/solver/amg_preconditioner.py:147-332``):
+symmetric diagonal equilibration D^-1/2 A D^-1/2 (``:180-185``), aggregation,
+per-aggregate QR of the rigid-body near-null-space (``:241-281``, generalised
+from 6-DOF beam nodes to 3-DOF solid nodes: 3 translations + 3 rotations),
+Galerkin coarse operators R A P (``:283-293``), dense coarsest solve.  Unlike
+the reference it is a *smoothed* aggregation AMG:
+
+* strength of connection ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F) on
+  block norms, three-phase aggregation (native, ``wnfea_sa_aggregate``);
+* the tentative prolongator is smoothed, P = (I - omega/lambda D^-1 A) P_tent
+  with omega = 4/3 and lambda the power-iteration estimate of rho(D^-1 A);
+* the coarse near-null-space is the R factor of each aggregate's QR;
+* Chebyshev smoothing on [lambda/ratio, lambda] replaces damped Jacobi.
+
+The V-cycle itself runs on the GPU (``wnfea_amg_vcycle``) in FP32 or FP64;
+the coarsest level is a dense FP64 inverse applied on the GPU.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+
+from .. import _lib
+
+
+def rigid_body_modes(coords: np.ndarray, dofs_per_node: int = 3) -> np.ndarray:
+    """Near-null-space of 3-D elasticity: (n*b, 6) columns [tx, ty, tz, rx, ry, rz].
+
+    For b = 6 (beam nodes) the rotational DOFs get the unit rotations, as in
+    ``amg_preconditioner.py:30-69``."""
+    c = np.asarray(coords, dtype=np.float64)
+    n, b = len(c), dofs_per_node
+    B = np.zeros((n * b, 6))
+    x, y, z = c[:, 0], c[:, 1], c[:, 2]
+    B[0::b, 0] = 1.0
+    B[1::b, 1] = 1.0
+    B[2::b, 2] = 1.0
+    B[1::b, 3], B[2::b, 3] = -z, y
+    B[0::b, 4], B[2::b, 4] = z, -x
+    B[0::b, 5], B[1::b, 5] = -y, x
+    if b == 6:
+        B[3::b, 3] = 1.0
+        B[4::b, 4] = 1.0
+        B[5::b, 5] = 1.0
+    return B
+
+
+def _power_lmax(A: sp.spmatrix, dinv: np.ndarray, iters: int = 15, seed: int = 0) -> float:
+    x = np.random.default_rng(seed).standard_normal(A.shape[0])
+    lam = 1.0
+    for _ in range(iters):
+        y = dinv * (A @ x)
+        ny = np.linalg.norm(y)
+        if ny == 0.0:
+            return 1.0
+        lam = ny / np.linalg.norm(x)
+        x = y / ny
+    return float(lam)
+
+
+def _strength_graph(A: sp.bsr_matrix, theta: float) -> sp.csr_matrix:
+    b = A.blocksize[0]
+    n = A.shape[0] // b
+    nrm = np.sqrt((A.data.reshape(len(A.data), -1) ** 2).sum(axis=1))
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(A.indptr))
+    cols = A.indices.astype(np.int64)
+    dn = np.zeros(n)
+    on_diag = rows == cols
+    dn[rows[on_diag]] = nrm[on_diag]
+    strong = (~on_diag) & (nrm > 0.0) & (nrm >= theta * np.sqrt(dn[rows] * dn[cols]))
+    G = sp.csr_matrix((np.ones(int(strong.sum()), dtype=np.int8), (rows[strong], cols[strong])), shape=(n, n))
+    G.sort_indices()
+    return G
+
+
+@dataclass
+class AMGLevelData:
+    A: sp.bsr_matrix                 # level operator (equilibrated on level 0)
+    dinv: np.ndarray                 # point-Jacobi inverse diagonal
+    lmax: float
+    P: sp.bsr_matrix | None = None
+    R: sp.bsr_matrix | None = None
+    n_aggregates: int = 0
+
+
+def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None = None, theta: float = 0.08,
+             theta_decay: float = 1.0, omega: float = 4.0 / 3.0, coarse_size: int = 600, max_levels: int = 10,
+             power_iters: int = 15, lmax_safety: float = 1.1,
+             dof_scale: np.ndarray | None = None) -> tuple[list[AMGLevelData], np.ndarray]:
+    """Host SA hierarchy of a block matrix.
+
+    When A = S K S is the equilibrated operator (S = diag(1/sqrt|K_ii|)), its
+    near-null-space is S^-1 B: pass dof_scale = sqrt|K_ii| so the rigid-body
+    modes are scaled accordingly.  Returns (levels, coarse_inverse).  Level
+    l > 0 operators are 6x6-block."""
+    L = _lib.load()
+    A = sp.bsr_matrix(A)
+    A.sort_indices()
+    b = A.blocksize[0]
+    B = rigid_body_modes(coords, b)
+    if dof_scale is not None:
+        B *= np.asarray(dof_scale, dtype=np.float64)[:, None]
+    if fixed_mask is not None:
+        B[np.asarray(fixed_mask, dtype=bool)] = 0.0
+    levels: list[AMGLevelData] = []
+    th = theta
+    while True:
+        b = A.blocksize[0]
+        d = A.diagonal()
+        dinv = np.where(np.abs(d) > 0.0, 1.0 / np.where(d == 0.0, 1.0, d), 1.0)
+        lmax = _power_lmax(A, dinv, power_iters) * lmax_safety
+        if A.shape[0] <= coarse_size or len(levels) + 1 >= max_levels:
+            levels.append(AMGLevelData(A=A, dinv=dinv, lmax=lmax))
+            break
+        G = _strength_graph(A, th)
+        n = G.shape[0]
+        agg = np.empty(n, dtype=np.int32)
+        ptr = np.ascontiguousarray(G.indptr, dtype=np.int32)
+        ind = np.ascontiguousarray(G.indices, dtype=np.int32)
+        na = int(L.wnfea_sa_aggregate(n, _lib.hptr(ptr), _lib.hptr(ind), _lib.hptr(agg)))
+        if na <= 0 or na * 6 >= A.shape[0]:
+            levels.append(AMGLevelData(A=A, dinv=dinv, lmax=lmax))
+            break
+        k = B.shape[1]
+        n_blk = int((agg >= 0).sum())
+        p_rowptr = np.zeros(n + 1, dtype=np.int32)
+        p_colidx = np.zeros(max(n_blk, 1), dtype=np.int32)
+        p_vals = np.zeros((max(n_blk, 1), b * k))
+        Bc = np.zeros((na * k, k))
+        Bc_in = np.ascontiguousarray(B, dtype=np.float64)
+        _lib.check(L.wnfea_sa_tentative(n, b, k, _lib.hptr(agg), na, _lib.hptr(Bc_in), _lib.hptr(p_rowptr),
+                                        _lib.hptr(p_colidx), _lib.hptr(p_vals), _lib.hptr(Bc)), "wnfea_sa_tentative")
+        P0 = sp.bsr_matrix((p_vals[:n_blk].reshape(-1, b, k), p_colidx[:n_blk], p_rowptr), shape=(n * b, na * k))
+        # P = P0 - (omega / lmax) D^-1 A P0
+        AP = (A @ P0).tobsr(blocksize=(b, k))
+        Dinv = sp.diags(dinv)
+        P = (P0 - (omega / lmax) * (Dinv @ AP)).tobsr(blocksize=(b, k))
+        P.sum_duplicates()
+        P.sort_indices()
+        R = P.T.tobsr(blocksize=(k, b))
+        R.sort_indices()
+        Ac = (R @ (A @ P).tobsr(blocksize=(b, k))).tobsr(blocksize=(k, k))
+        Ac = (0.5 * (Ac + Ac.T)).tobsr(blocksize=(k, k))
+        Ac.sum_duplicates()
+        Ac.sort_indices()
+        levels.append(AMGLevelData(A=A, dinv=dinv, lmax=lmax, P=P, R=R, n_aggregates=na))
+        A, B = Ac, Bc
+        th *= theta_decay
+    Acd = levels[-1].A.toarray()
+    inv = np.linalg.inv(Acd)
+    inv = 0.5 * (inv + inv.T)
+    return levels, inv
+
+
+class SmoothedAggregationAMG:
+    """GPU V-cycle preconditioner of an equilibrated system (see module doc).
+
+    ``fine`` is the system's DeviceBSR (its FP32 values, and for FP64 cycles
+    its FP64 equilibrated values, are the level-0 operator, shared with PCG).
+    """
+
+    def __init__(self, fine, A_hat: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None = None,
+                 dof_scale: np.ndarray | None = None, theta: float = 0.08, theta_decay: float = 1.0,
+                 degree: int = 2, ratio: float = 30.0, coarse_size: int = 600, max_levels: int = 10,
+                 precisions: int = 3, sigma: int = 4096, waves: int = 1, xcd_remap: bool = True):
+        t0 = time.perf_counter()
+        self.lib = _lib.load()
+        self.levels, inv = sa_setup(A_hat, coords, fixed_mask, theta=theta, theta_decay=theta_decay,
+                                    coarse_size=coarse_size, max_levels=max_levels, dof_scale=dof_scale)
+        self.setup_host_sec = time.perf_counter() - t0
+        self.degree, self.ratio = degree, ratio
+        self.fine = fine
+        nl = len(self.levels)
+        self.handle = self.lib.wnfea_amg_create(nl)
+        if not self.handle:
+            raise _lib.HipError("wnfea_amg_create failed")
+        self._keep = []
+        for l, lv in enumerate(self.levels):
+            b = lv.A.blocksize[0]
+            n_rows = lv.A.shape[0] // b
+            dinv = np.ascontiguousarray(lv.dinv, dtype=np.float64)
+            args_A = (None, None, None)
+            fine_h = None
+            if l == 0:
+                fine_h = fine.handle
+            else:
+                args_A = (np.ascontiguousarray(lv.A.indptr, dtype=np.int32),
+                          np.ascontiguousarray(lv.A.indices, dtype=np.int32),
+                          np.ascontiguousarray(lv.A.data.reshape(len(lv.A.data), -1), dtype=np.float64))
+            if lv.P is not None:
+                kc = lv.P.blocksize[1]
+                nc = lv.P.shape[1] // kc
+                Pa = (np.ascontiguousarray(lv.P.indptr, dtype=np.int32),
+                      np.ascontiguousarray(lv.P.indices, dtype=np.int32),
+                      np.ascontiguousarray(lv.P.data.reshape(len(lv.P.data), -1), dtype=np.float64))
+                Ra = (np.ascontiguousarray(lv.R.indptr, dtype=np.int32),
+                      np.ascontiguousarray(lv.R.indices, dtype=np.int32),
+                      np.ascontiguousarray(lv.R.data.reshape(len(lv.R.data), -1), dtype=np.float64))
+            else:
+                kc, nc, Pa, Ra = 0, 0, (None, None, None), (None, None, None)
+            self._keep.append((args_A, Pa, Ra, dinv))
+            rc = self.lib.wnfea_amg_set_level(
+                self.handle, l, fine_h, b, n_rows, _lib.hptr(args_A[0]), _lib.hptr(args_A[1]), _lib.hptr(args_A[2]),
+                _lib.hptr(dinv), float(lv.lmax), float(ratio), int(degree), int(kc), int(nc),
+                _lib.hptr(Pa[0]), _lib.hptr(Pa[1]), _lib.hptr(Pa[2]), _lib.hptr(Ra[0]), _lib.hptr(Ra[1]),
+                _lib.hptr(Ra[2]), int(sigma))
+            _lib.check(rc, f"wnfea_amg_set_level({l})")
+        inv = np.ascontiguousarray(inv, dtype=np.float64)
+        _lib.check(self.lib.wnfea_amg_set_coarse(self.handle, inv.shape[0], _lib.hptr(inv)), "wnfea_amg_set_coarse")
+        _lib.check(self.lib.wnfea_amg_finalize(self.handle, int(precisions), int(waves), int(bool(xcd_remap))),
+                   "wnfea_amg_finalize")
+        self._keep = None
+        self.setup_sec = time.perf_counter() - t0
+
+    def stats(self) -> dict:
+        out = []
+        for lv in self.levels:
+            out.append({"n": lv.A.shape[0], "nnz": int(lv.A.nnz), "block": lv.A.blocksize[0],
+                        "lmax": lv.lmax, "aggregates": lv.n_aggregates,
+                        "P_nnz": int(lv.P.nnz) if lv.P is not None else 0})
+        nnz0 = out[0]["nnz"]
+        return {"levels": out, "operator_complexity": sum(o["nnz"] for o in out) / max(nnz0, 1),
+                "setup_sec": self.setup_sec, "setup_host_sec": self.setup_host_sec}
+
+    def vcycle(self, r, z=None):
+        """z = M r on the device (FP32 or FP64 by r's dtype)."""
+        import torch
+        prec = 1 if r.dtype == torch.float32 else 0
+        if z is None:
+            z = torch.empty_like(r)
+        _lib.check(self.lib.wnfea_amg_vcycle(self.handle, prec, _lib.dptr(r), _lib.dptr(z), _lib.stream_of()),
+                   "wnfea_amg_vcycle")
+        return z
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.wnfea_amg_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
