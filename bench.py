@@ -91,7 +91,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
-    ap.add_argument("--inner-rtol", type=float, default=1e-3)
+    ap.add_argument("--amg-precision", choices=["fp16", "fp32"], default="fp16")
+    ap.add_argument("--ir", action="store_true", help="also time the restart-IR (FP32 Krylov) scheme")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how the "
@@ -100,7 +102,8 @@ def main() -> int:
     import torch
     from wnfea_amd.mesh.rcm import apply_rcm_to_model
     from wnfea_amd.mesh.solid_generator import generate_c3d10_ibeam, generate_c3d10_structured_block
-    from wnfea_amd.solver.solid_solver import (assemble_c3d10_bsr, build_amg, ir_device, pcg_device,
+    from wnfea_amd.solver.autotune import autotune_spmv
+    from wnfea_amd.solver.solid_solver import (assemble_c3d10_bsr, build_amg, ir_device, pcg_device, solve_mixed,
                                                true_residual)
 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
@@ -124,19 +127,24 @@ def main() -> int:
     t_asm = time.perf_counter() - t0
     info = system.K.info()
     log(f"[bench] rcm {t_rcm:.1f}s assembly {t_asm:.1f}s nnzb {info['nnzb']} slots {info['slots']}")
+    tune = {"waves": 1, "unroll": 1, "xcd_remap": True}
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = autotune_spmv(system.K, "fp32", use_cache=False)
+        tune.pop("table", None)
+        log(f"[bench] autotune {tune} ({time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
-    amg = build_amg(system)
+    amg = build_amg(system, waves=tune["waves"], unroll=tune["unroll"], xcd_remap=tune["xcd_remap"])
     t_amg = time.perf_counter() - t0
     st = amg.stats()
     log(f"[bench] amg setup {t_amg:.1f}s levels {[l['n'] for l in st['levels']]} "
         f"op-complexity {st['operator_complexity']:.2f}")
 
     def mixed_solve():
-        return ir_device(system, tol=1e-8, inner_rtol=args.inner_rtol, amg=amg)
+        return solve_mixed(system, amg, tol=1e-8, amg_precision=args.amg_precision)
 
     def fp64_solve():
-        return pcg_device(system, rtol=1e-8 * 0.5, max_iter=2000, preconditioner="amg",
-                          precision="fp64_equilibrated", amg=amg)
+        return solve_mixed(system, amg, tol=1e-8, amg_precision="fp64")
 
     for _ in range(args.warmup):
         mixed_solve()
@@ -145,20 +153,34 @@ def main() -> int:
     res = [mixed_solve() for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps
-    u, outer, inner, rel, _ = res[-1]
+    u, iters, steps, rel, _ = res[-1]
     rel_true = true_residual(system, u)
-    log(f"[bench] mixed: {t_mixed * 1e3:.1f} ms/solve outer {outer} inner {inner} rel {rel_true:.2e}")
+    log(f"[bench] mixed (FP64 CG + {args.amg_precision} V-cycle): {t_mixed * 1e3:.1f} ms/solve iterations {iters} "
+        f"restarts {steps - 1} rel {rel_true:.2e}")
     # all-FP64 AMG-PCG on the same hierarchy
     fp64_solve()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(max(1, args.steps // 2)):
-        x64, it64, _, _ = fp64_solve()
+        x64, it64, _, _, _ = fp64_solve()
     torch.cuda.synchronize()
     t_fp64 = (time.perf_counter() - t0) / max(1, args.steps // 2)
     rel64 = true_residual(system, x64)
     log(f"[bench] fp64 AMG-PCG: {t_fp64 * 1e3:.1f} ms/solve iters {it64} rel {rel64:.2e}")
     err_u = float(torch.linalg.norm(u - x64) / torch.linalg.norm(x64))
+    other = "fp32" if args.amg_precision == "fp16" else "fp16"
+    solve_mixed(system, amg, tol=1e-8, amg_precision=other)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    uo, ito, _, relo, _ = solve_mixed(system, amg, tol=1e-8, amg_precision=other)
+    torch.cuda.synchronize()
+    t_other = time.perf_counter() - t0
+    log(f"[bench] mixed with {other} V-cycle: {t_other * 1e3:.1f} ms iters {ito} rel {relo:.2e}")
+    ir = None
+    if args.ir:
+        _, ir_outer, ir_inner, ir_rel, ir_t = ir_device(system, tol=1e-8, amg=amg, max_outer=30)
+        ir = {"seconds": round(ir_t, 4), "outer": ir_outer, "inner": ir_inner, "rel_res": ir_rel}
+        log(f"[bench] restart IR (FP32 Krylov): {ir}")
     # the reference's algorithm on the GPU: FP64 Jacobi-PCG
     jac = None
     if not args.no_jacobi:
@@ -191,18 +213,21 @@ def main() -> int:
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-inner/f64-refinement",
+        "dtype": f"f64 CG / {args.amg_precision}-stored f32 AMG",
         "data": "synthetic (structured C3D10 mesh generated in-process, deterministic loads)",
         "config": {"workload": workload, "n_dofs": meta["n_dofs"], "n_elements": meta["n_elements"],
-                   "nnzb": nnzb, "tol": 1e-8, "solver": "IR(FP64) + FP32 SA-AMG-PCG",
+                   "nnzb": nnzb, "tol": 1e-8, "solver": "FP64 CG + low-precision SA-AMG V-cycle",
                    "parallelism": "single-gpu"},
         "dof_per_s": round(meta["n_dofs"] / t_mixed, 1),
         "rel_res_fp64_verified": rel_true,
-        "outer_steps": outer,
-        "inner_iterations": inner,
+        "iterations": iters,
+        "cg_restarts": steps - 1,
+        "vcycle_storage": args.amg_precision,
         "fp64_amg_pcg": {"seconds": round(t_fp64, 5), "iterations": it64, "rel_res": rel64},
         "speedup_mixed_vs_fp64_amg_pcg": round(t_fp64 / t_mixed, 3),
         "fp64_jacobi_pcg": jac,
+        "restart_ir_fp32_krylov": ir,
+        f"mixed_{other}_vcycle": {"seconds": round(t_other, 5), "iterations": ito, "rel_res": relo},
         "speedup_mixed_vs_fp64_jacobi_pcg": round(jac["seconds"] / t_mixed, 2) if jac else None,
         "displacement_rel_err_mixed_vs_fp64": err_u,
         "spmv_gbs": {"fp32": round(gbs32, 1), "fp64": round(gbs64, 1), "fp32_us": round(t32 * 1e6, 2),
@@ -210,6 +235,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "sell_spmv_kernel<3,3,float,float,1,OpAxpby> (FP32 3x3 BSR SpMV)"},
+        "spmv_tune": tune,
         "setup": {"mesh_sec": round(t_mesh, 2), "rcm_sec": round(t_rcm, 2), "assembly_sec": round(t_asm, 2),
                   "amg_setup_sec": round(t_amg, 2), "amg": st},
         "cpu_baseline": cpu,

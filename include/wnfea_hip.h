@@ -119,17 +119,21 @@ int wnfea_sa_tentative(int n_nodes, int b, int k, const int *agg, int n_agg, con
 
 /* Create a br x bc block matrix from its host block-CSR structure; sigma is
  * the row-sorting window of the sliced-ELL layout (multiple of 64; <=0 =
- * 4096).  Returns NULL on failure.  Replaces BSR6x6Matrix / csr_to_bsr6x6
+ * 4096); lanes is the number of wavefront lanes per block row (power of two
+ * <= 64; <=0 = chosen from the shape).  Returns NULL on failure.  Replaces BSR6x6Matrix / csr_to_bsr6x6
  * (wnfea/solver/bsr_matrix.py:19-122) and the device uploads of
  * hip_spmv_bsr6x6_fp32/fp64 (wnfea/solver/native/hip_kernels.cpp:2975-3087). */
-void *wnfea_bsr_create(int br, int bc, int n_rows, int n_cols, const int *rowptr, const int *colidx, int sigma);
+void *wnfea_bsr_create(int br, int bc, int n_rows, int n_cols, const int *rowptr, const int *colidx, int sigma,
+                       int lanes);
 void wnfea_bsr_destroy(void *h);
 
-/* out[0..6]: n_slices, slots, nnzb, sigma, has FP64, has FP32, has FP64 equilibrated. */
+/* out[0..7]: n_slices, slots, nnzb, sigma, has FP64, has FP32, has FP64 equilibrated, lanes per row. */
 int wnfea_bsr_info(void *h, long long *out);
 
-/* SpMV launch shape: waves per slice (1, 2, 4) and XCD-contiguous slice mapping. */
-int wnfea_bsr_tune(void *h, int waves, int xcd_remap);
+/* SpMV launch shape: waves per slice (1, 2, 4), XCD-contiguous slice mapping,
+ * blocks in flight per lane (1, 2, 4).  No reference counterpart (the
+ * reference's launch sizes are fixed heuristics, wnfea/solver/fast_kernels.py:243-260). */
+int wnfea_bsr_tune(void *h, int waves, int xcd_remap, int unroll);
 
 /* Values from the standard layout (device FP64 [nnzb][br*bc]); which: 1 FP64, 2 FP32, 3 both. */
 int wnfea_bsr_set_values(void *h, const double *vals_std, int which, void *stream);
@@ -188,7 +192,7 @@ int wnfea_amg_set_level(void *h, int lvl, void *fine_bsr, int br, int n_rows, co
 int wnfea_amg_set_coarse(void *h, int n, const double *inv);
 
 /* Allocate cycle vectors (prec_mask bit0 FP64, bit1 FP32) and set the SpMV launch shape. */
-int wnfea_amg_finalize(void *h, int prec_mask, int waves, int xcd_remap);
+int wnfea_amg_finalize(void *h, int prec_mask, int waves, int xcd_remap, int unroll);
 
 /* z = M r, one V-cycle (device vectors); prec 0 FP64, 1 FP32. */
 int wnfea_amg_vcycle(void *h, int prec, const void *r, void *z, void *stream);

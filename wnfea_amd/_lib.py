@@ -37,18 +37,18 @@ SIGNATURES = {
     "wnfea_node_incidence": (_i, [_i, _i, _i, _p, _p, _p]),
     "wnfea_sa_aggregate": (_ll, [_i, _p, _p, _p]),
     "wnfea_sa_tentative": (_i, [_i, _i, _i, _p, _i, _p, _p, _p, _p, _p]),
-    "wnfea_bsr_create": (_p, [_i, _i, _i, _i, _p, _p, _i]),
+    "wnfea_bsr_create": (_p, [_i, _i, _i, _i, _p, _p, _i, _i]),
     "wnfea_bsr_destroy": (None, [_p]),
     "wnfea_bsr_info": (_i, [_p, _pll]),
-    "wnfea_bsr_tune": (_i, [_p, _i, _i]),
+    "wnfea_bsr_tune": (_i, [_p, _i, _i, _i]),
     "wnfea_bsr_set_values": (_i, [_p, _p, _i, _p]),
     "wnfea_bsr_get_values": (_i, [_p, _p, _p]),
     "wnfea_bsr_assemble": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "wnfea_bsr_diag": (_i, [_p, _p, _p, _p]),
-    "wnfea_bsr_equilibrate": (_i, [_p, _p, _p, _p, _i, _p]),
+    "wnfea_bsr_equilibrate": (_i, [_p, _p, _p, _p, _i, _i, _p]),
     "wnfea_bsr_spmv": (_i, [_p, _i, _p, _p, _d, _d, _p]),
     "wnfea_bsr_spmv_dot": (_i, [_p, _i, _p, _p, _pd, _p]),
-    "wnfea_amg_create": (_p, [_i]),
+    "wnfea_amg_create": (_p, [_i, _i]),
     "wnfea_amg_destroy": (None, [_p]),
     "wnfea_amg_set_level": (_i, [_p, _i, _p, _i, _i, _p, _p, _p, _p, _d, _d, _i, _i, _i, _p, _p, _p, _p, _p, _p, _i]),
     "wnfea_amg_set_coarse": (_i, [_p, _i, _p]),
@@ -56,8 +56,8 @@ SIGNATURES = {
     "wnfea_amg_vcycle": (_i, [_p, _i, _p, _p, _p]),
     "wnfea_solver_create": (_p, []),
     "wnfea_solver_destroy": (None, [_p]),
-    "wnfea_pcg": (_i, [_p, _p, _i, _i, _p, _p, _p, _p, _d, _i, _i, _pi, _pd, _p]),
-    "wnfea_ir": (_i, [_p, _p, _p, _i, _p, _p, _p, _p, _d, _d, _i, _i, _i, _i, _pi, _pi, _pd, _p]),
+    "wnfea_pcg": (_i, [_p, _p, _i, _i, _p, _p, _i, _p, _p, _p, _d, _i, _i, _pi, _pd, _p]),
+    "wnfea_ir": (_i, [_p, _p, _p, _i, _p, _p, _i, _p, _p, _d, _d, _i, _i, _i, _i, _pi, _pi, _pd, _p]),
     "wnfea_residual": (_i, [_p, _p, _p, _p, _p, _pd, _p]),
 }
 

@@ -34,13 +34,11 @@ __global__ void cast_scale_kernel(long long n, const A* __restrict__ x, const do
   }
 }
 
-// z = dinv .* r, rho = r . z; finalize: beta = rho/rho_prev (0 on the first
-// iteration), rho_prev <- rho.  sc layout: see Scalars in wn.hpp.
+// z = dinv .* r and the partials of rho = r . z (finalised with kFinRho).
 template <typename T>
 __global__ __launch_bounds__(kVecBlock) void jacobi_dot_kernel(long long n, const T* __restrict__ r,
                                                                const T* __restrict__ dinv, T* __restrict__ z,
-                                                               double* partials, unsigned int* counter, double* sc,
-                                                               int first, const int* skip) {
+                                                               double* partials, const int* skip) {
   if (skip && *skip) return;
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -51,49 +49,44 @@ __global__ __launch_bounds__(kVecBlock) void jacobi_dot_kernel(long long n, cons
   __shared__ double sm[kVecBlock / 64];
   const double bs = block_sum(acc, sm);
   double v[1] = {bs};
-  if (grid_reduce_last<1>(v, partials, counter, sc + kScRho) && threadIdx.x == 0) {
-    const double rho = sc[kScRho];
-    sc[kScBeta] = first ? 0.0 : (sc[kScRhoPrev] != 0.0 ? rho / sc[kScRhoPrev] : 0.0);
-    sc[kScRhoPrev] = rho;
-  }
+  store_block_partials<1>(v, partials);
 }
 
-// rho = r . z with the same finalisation (the preconditioner wrote z).
-template <typename T>
-__global__ __launch_bounds__(kVecBlock) void rho_kernel(long long n, const T* __restrict__ r, const T* __restrict__ z,
-                                                        double* partials, unsigned int* counter, double* sc,
-                                                        int first, const int* skip) {
+// partials of rho = r . (S z) (finalised with kFinRho); z may be in a lower
+// precision than r (preconditioner output) and S (per-DOF scale) may be null.
+template <typename T, typename ZT>
+__global__ __launch_bounds__(kVecBlock) void rho_kernel(long long n, const T* __restrict__ r, const ZT* __restrict__ z,
+                                                        const double* __restrict__ scale, double* partials,
+                                                        const int* skip) {
   if (skip && *skip) return;
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    acc += double(r[i]) * double(z[i]);
+    acc += double(r[i]) * double(z[i]) * (scale ? scale[i] : 1.0);
   __shared__ double sm[kVecBlock / 64];
   const double bs = block_sum(acc, sm);
   double v[1] = {bs};
-  if (grid_reduce_last<1>(v, partials, counter, sc + kScRho) && threadIdx.x == 0) {
-    const double rho = sc[kScRho];
-    sc[kScBeta] = first ? 0.0 : (sc[kScRhoPrev] != 0.0 ? rho / sc[kScRhoPrev] : 0.0);
-    sc[kScRhoPrev] = rho;
+  store_block_partials<1>(v, partials);
+}
+
+// p = S z + beta p  (p = S z on the first iteration)
+template <typename T, typename ZT>
+__global__ void update_p_kernel(long long n, const ZT* __restrict__ z, const double* __restrict__ scale,
+                                T* __restrict__ p, const double* sc, int first, const int* skip) {
+  if (skip && *skip) return;
+  const T beta = T(sc[kScBeta]);
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const T zi = scale ? T(scale[i] * double(z[i])) : T(z[i]);
+    p[i] = first ? zi : zi + beta * p[i];
   }
 }
 
-// p = z + beta p
-template <typename T>
-__global__ void update_p_kernel(long long n, const T* __restrict__ z, T* __restrict__ p, const double* sc,
-                                const int* skip) {
-  if (skip && *skip) return;
-  const T beta = T(sc[kScBeta]);
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    p[i] = z[i] + beta * p[i];
-}
-
-// x += alpha p; r -= alpha q; rr = r . r.  Finalise: count the iteration and
-// raise the done flag once rr <= sc[kScTol2] (set by the driver to rtol^2 rr0).
-template <typename T>
+// x += alpha p; r -= alpha q; partials of rr = r . r (finalised with kFinRR);
+// optionally rp = S r in the preconditioner's precision.
+template <typename T, typename RT>
 __global__ __launch_bounds__(kVecBlock) void update_xr_kernel(long long n, T* __restrict__ x, T* __restrict__ r,
                                                               const T* __restrict__ p, const T* __restrict__ q,
-                                                              double* partials, unsigned int* counter, double* sc,
-                                                              int* flags) {
+                                                              RT* __restrict__ rp, const double* __restrict__ scale,
+                                                              double* partials, const double* sc, const int* flags) {
   if (flags && flags[0]) return;
   const T alpha = T(sc[kScAlpha]);
   double acc = 0.0;
@@ -101,28 +94,26 @@ __global__ __launch_bounds__(kVecBlock) void update_xr_kernel(long long n, T* __
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
+    if (rp) rp[i] = RT(scale ? scale[i] * double(ri) : double(ri));
     acc += double(ri) * double(ri);
   }
   __shared__ double sm[kVecBlock / 64];
   const double bs = block_sum(acc, sm);
   double v[1] = {bs};
-  if (grid_reduce_last<1>(v, partials, counter, sc + kScRR) && threadIdx.x == 0 && flags) {
-    flags[1] += 1;
-    if (sc[kScRR] <= sc[kScTol2] || !(sc[kScRR] == sc[kScRR])) flags[0] = 1;
-  }
+  store_block_partials<1>(v, partials);
 }
 
 // generic dot into out[0]
 template <typename T>
 __global__ __launch_bounds__(kVecBlock) void dot_kernel(long long n, const T* __restrict__ a, const T* __restrict__ b,
-                                                        double* partials, unsigned int* counter, double* out) {
+                                                        double* partials) {
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     acc += double(a[i]) * double(b[i]);
   __shared__ double sm[kVecBlock / 64];
   const double bs = block_sum(acc, sm);
   double v[1] = {bs};
-  grid_reduce_last<1>(v, partials, counter, out);
+  store_block_partials<1>(v, partials);
 }
 
 // u += f * s .* d   (FP64 refinement update; d in working precision)
@@ -141,6 +132,55 @@ __global__ void normalise_kernel(long long n, const T* __restrict__ x, T* __rest
   const double f = g > 0.0 ? 1.0 / sqrt(g) : 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     y[i] = T(double(x[i]) * f);
+}
+
+// One-block finalisation of a two-launch reduction (see common.hpp).
+__global__ __launch_bounds__(256) void dot_finalize_kernel(const double* __restrict__ partials, int nb, int K, int mode,
+                                                           double* out, double* sc, int* flags, int first,
+                                                           const int* skip) {
+  if (skip && *skip) return;
+  __shared__ double red[2][4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partials[(size_t)k * nb + i];
+    s = wave_sum(s);
+    if (lane == 0) red[k][wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double v[2] = {0.0, 0.0};
+  for (int k = 0; k < K; ++k)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v[k] += red[k][w];
+  switch (mode) {
+    case kFinRho: {
+      sc[kScRho] = v[0];
+      const double prev = sc[kScRhoPrev];
+      sc[kScBeta] = first ? 0.0 : (prev != 0.0 ? v[0] / prev : 0.0);
+      sc[kScRhoPrev] = v[0];
+      break;
+    }
+    case kFinGamma:
+      sc[kScGamma] = v[0];
+      sc[kScAlpha] = v[0] != 0.0 ? sc[kScRho] / v[0] : 0.0;
+      break;
+    case kFinRR:
+      sc[kScRR] = v[0];
+      if (flags) {
+        flags[1] += 1;
+        if (v[0] <= sc[kScTol2] || !(v[0] == v[0])) flags[0] = 1;
+      }
+      break;
+    default:
+      for (int k = 0; k < K; ++k) out[k] = v[k];
+  }
+}
+
+int launch_finalize(const double* partials, int nb, int K, int mode, double* out, double* sc, int* flags, int first,
+                    const int* skip, hipStream_t s) {
+  hipLaunchKernelGGL(dot_finalize_kernel, dim3(1), dim3(256), 0, s, partials, nb, K, mode, out, sc, flags, first,
+                     skip);
+  return static_cast<int>(hipGetLastError());
 }
 
 // ---------------------------------------------------------------- launchers
@@ -171,28 +211,40 @@ template int launch_cast_scale<float, float>(long long, const float*, const doub
 template <typename T>
 int launch_jacobi_dot(long long n, const T* r, const T* dinv, T* z, Workspace& w, int first, const int* skip,
                       hipStream_t s) {
-  hipLaunchKernelGGL(jacobi_dot_kernel<T>, WN_GRID(n), n, r, dinv, z, w.partials, w.counter, w.sc, first, skip);
+  const int g = vec_grid(n);
+  hipLaunchKernelGGL(jacobi_dot_kernel<T>, dim3(g), dim3(kVecBlock), 0, s, n, r, dinv, z, w.partials, skip);
+  int e = static_cast<int>(hipGetLastError());
+  return e ? e : launch_finalize(w.partials, g, 1, kFinRho, nullptr, w.sc, nullptr, first, skip, s);
+}
+template <typename T, typename ZT>
+int launch_rho(long long n, const T* r, const ZT* z, const double* scale, Workspace& w, int first, const int* skip,
+               hipStream_t s) {
+  const int g = vec_grid(n);
+  hipLaunchKernelGGL((rho_kernel<T, ZT>), dim3(g), dim3(kVecBlock), 0, s, n, r, z, scale, w.partials, skip);
+  int e = static_cast<int>(hipGetLastError());
+  return e ? e : launch_finalize(w.partials, g, 1, kFinRho, nullptr, w.sc, nullptr, first, skip, s);
+}
+template <typename T, typename ZT>
+int launch_update_p(long long n, const ZT* z, const double* scale, T* p, Workspace& w, int first, const int* skip,
+                    hipStream_t s) {
+  hipLaunchKernelGGL((update_p_kernel<T, ZT>), WN_GRID(n), n, z, scale, p, w.sc, first, skip);
   return static_cast<int>(hipGetLastError());
 }
-template <typename T>
-int launch_rho(long long n, const T* r, const T* z, Workspace& w, int first, const int* skip, hipStream_t s) {
-  hipLaunchKernelGGL(rho_kernel<T>, WN_GRID(n), n, r, z, w.partials, w.counter, w.sc, first, skip);
-  return static_cast<int>(hipGetLastError());
-}
-template <typename T>
-int launch_update_p(long long n, const T* z, T* p, Workspace& w, const int* skip, hipStream_t s) {
-  hipLaunchKernelGGL(update_p_kernel<T>, WN_GRID(n), n, z, p, w.sc, skip);
-  return static_cast<int>(hipGetLastError());
-}
-template <typename T>
-int launch_update_xr(long long n, T* x, T* r, const T* p, const T* q, Workspace& w, int* flags, hipStream_t s) {
-  hipLaunchKernelGGL(update_xr_kernel<T>, WN_GRID(n), n, x, r, p, q, w.partials, w.counter, w.sc, flags);
-  return static_cast<int>(hipGetLastError());
+template <typename T, typename RT>
+int launch_update_xr(long long n, T* x, T* r, const T* p, const T* q, RT* rp, const double* scale, Workspace& w,
+                     int* flags, hipStream_t s) {
+  const int g = vec_grid(n);
+  hipLaunchKernelGGL((update_xr_kernel<T, RT>), dim3(g), dim3(kVecBlock), 0, s, n, x, r, p, q, rp, scale, w.partials,
+                     w.sc, flags);
+  int e = static_cast<int>(hipGetLastError());
+  return e ? e : launch_finalize(w.partials, g, 1, kFinRR, nullptr, w.sc, flags, 0, flags, s);
 }
 template <typename T>
 int launch_dot(long long n, const T* a, const T* b, Workspace& w, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(dot_kernel<T>, WN_GRID(n), n, a, b, w.partials, w.counter, out);
-  return static_cast<int>(hipGetLastError());
+  const int g = vec_grid(n);
+  hipLaunchKernelGGL(dot_kernel<T>, dim3(g), dim3(kVecBlock), 0, s, n, a, b, w.partials);
+  int e = static_cast<int>(hipGetLastError());
+  return e ? e : launch_finalize(w.partials, g, 1, kFinStore, out, w.sc, nullptr, 0, nullptr, s);
 }
 template <typename T>
 int launch_ir_update(long long n, double* u, const T* d, const double* sc_vec, const double* nrm2, hipStream_t s) {
@@ -207,13 +259,21 @@ int launch_normalise(long long n, const T* x, T* y, const double* nrm2, hipStrea
 
 #define WN_INST(T)                                                                                   \
   template int launch_jacobi_dot<T>(long long, const T*, const T*, T*, Workspace&, int, const int*, hipStream_t); \
-  template int launch_rho<T>(long long, const T*, const T*, Workspace&, int, const int*, hipStream_t);            \
-  template int launch_update_p<T>(long long, const T*, T*, Workspace&, const int*, hipStream_t);                  \
-  template int launch_update_xr<T>(long long, T*, T*, const T*, const T*, Workspace&, int*, hipStream_t);         \
   template int launch_dot<T>(long long, const T*, const T*, Workspace&, double*, hipStream_t);        \
   template int launch_ir_update<T>(long long, double*, const T*, const double*, const double*, hipStream_t); \
   template int launch_normalise<T>(long long, const T*, T*, const double*, hipStream_t);
 WN_INST(float)
 WN_INST(double)
+
+#define WN_INST2(T, U)                                                                                           \
+  template int launch_rho<T, U>(long long, const T*, const U*, const double*, Workspace&, int, const int*,      \
+                                hipStream_t);                                                                  \
+  template int launch_update_p<T, U>(long long, const U*, const double*, T*, Workspace&, int, const int*,      \
+                                     hipStream_t);                                                             \
+  template int launch_update_xr<T, U>(long long, T*, T*, const T*, const T*, U*, const double*, Workspace&,    \
+                                      int*, hipStream_t);
+WN_INST2(double, double)
+WN_INST2(double, float)
+WN_INST2(float, float)
 
 }  // namespace wn

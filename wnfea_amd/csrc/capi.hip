@@ -27,25 +27,26 @@ int sa_tentative(int n_nodes, int b, int k, const int* agg, int n_agg, const dou
 struct Amg;
 struct AmgLevel;
 struct Solver;
-template <typename T> int amg_apply(Amg& g, const T* r, T* z, const int* skip, hipStream_t s);
+int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s);
 template <typename T>
-int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, const T* b, T* x, double rtol,
-            int max_iter, int check_every, const SpmvTune& tune0, hipStream_t s, int* iters, double* rel);
-int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float* dinv32, Amg* amg, const double* f,
-           double* u, double tol, double inner_rtol, int max_outer, int max_inner, int check_every,
+int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, int amg_vt, const double* scale,
+            const T* b, T* x, double rtol, int max_iter, int check_every, const SpmvTune& tune0, hipStream_t s,
+            int* iters, double* rel);
+int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float* dinv32, Amg* amg, int amg_vt,
+           const double* f, double* u, double tol, double inner_rtol, int max_outer, int max_inner, int check_every,
            const SpmvTune& tune, hipStream_t s, int* outer_out, int* inner_out, double* rel_out, int zero_init);
 Solver* solver_new();
 void solver_delete(Solver*);
 Workspace& solver_ws(Solver&);
 Amg* amg_new();
 void amg_delete(Amg*);
-int amg_resize(Amg&, int n_levels);
+int amg_resize(Amg&, int n_levels, int store);
 int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const int* a_rowptr, const int* a_colidx,
                   const double* a_vals, const double* dinv, double lmax, double ratio, int degree, int bc_coarse,
                   int n_coarse_rows, const int* p_rowptr, const int* p_colidx, const double* p_vals,
                   const int* r_rowptr, const int* r_colidx, const double* r_vals, int sigma);
 int amg_set_coarse(Amg& g, int n, const double* inv);
-int amg_finalize(Amg& g, int prec_mask, const SpmvTune& t);
+int amg_finalize(Amg& g, const SpmvTune& t);
 }  // namespace wn
 
 using namespace wn;
@@ -161,9 +162,10 @@ int wnfea_sa_tentative(int n_nodes, int b, int k, const int* agg, int n_agg, con
 }
 
 // ------------------------------------------------------------ block matrices
-void* wnfea_bsr_create(int br, int bc, int n_rows, int n_cols, const int* rowptr, const int* colidx, int sigma) {
+void* wnfea_bsr_create(int br, int bc, int n_rows, int n_cols, const int* rowptr, const int* colidx, int sigma,
+                       int lanes) {
   auto* h = new WnBsr();
-  if (h->m.build(br, bc, n_rows, n_cols, rowptr, colidx, sigma) != kOk) {
+  if (h->m.build(br, bc, n_rows, n_cols, rowptr, colidx, sigma, lanes) != kOk) {
     h->m.release();
     delete h;
     return nullptr;
@@ -179,7 +181,7 @@ void wnfea_bsr_destroy(void* h) {
   delete b;
 }
 
-// out: n_slices, slots, nnzb, sigma, has_v64, has_v32, has_v64e
+// out: n_slices, slots, nnzb, sigma, has_v64, has_v32, has_v64e, lanes per row
 int wnfea_bsr_info(void* h, long long* out) {
   if (!h || !out) return kBadArg;
   const SellMatrix& m = static_cast<WnBsr*>(h)->m;
@@ -190,14 +192,16 @@ int wnfea_bsr_info(void* h, long long* out) {
   out[4] = m.d_v64 != nullptr;
   out[5] = m.d_v32 != nullptr;
   out[6] = m.d_v64e != nullptr;
+  out[7] = m.lanes;
   return kOk;
 }
 
-int wnfea_bsr_tune(void* h, int waves, int xcd_remap) {
-  if (!h || (waves != 1 && waves != 2 && waves != 4)) return kBadArg;
+int wnfea_bsr_tune(void* h, int waves, int xcd_remap, int unroll) {
+  if (!h || (waves != 1 && waves != 2 && waves != 4) || (unroll != 1 && unroll != 2 && unroll != 4)) return kBadArg;
   auto* b = static_cast<WnBsr*>(h);
   b->tune.waves = waves;
   b->tune.xcd_remap = xcd_remap ? 1 : 0;
+  b->tune.unroll = unroll;
   return kOk;
 }
 
@@ -226,9 +230,9 @@ int wnfea_bsr_diag(void* h, const long long* diag_block, double* diag, void* str
 }
 
 int wnfea_bsr_equilibrate(void* h, const int* rowidx, const int* colidx, const double* scale, int want64e,
-                          void* stream) {
+                          int want16, void* stream) {
   if (!h || !rowidx || !colidx || !scale) return kBadArg;
-  return equilibrate(static_cast<WnBsr*>(h)->m, rowidx, colidx, scale, want64e, S(stream));
+  return equilibrate(static_cast<WnBsr*>(h)->m, rowidx, colidx, scale, want64e, S(stream), want16);
 }
 
 // prec: 0 = FP64 primary values, 1 = FP32 values, 2 = FP64 equilibrated values
@@ -272,10 +276,11 @@ int wnfea_bsr_spmv_dot(void* h, int prec, const void* x, void* y, double* out, v
 }
 
 // ------------------------------------------------------------ AMG
-void* wnfea_amg_create(int n_levels) {
-  if (n_levels <= 0 || n_levels > 32) return nullptr;
+// store: bit0 FP64 values, bit1 FP32 values, bit2 FP16 values (with FP32 arithmetic)
+void* wnfea_amg_create(int n_levels, int store) {
+  if (n_levels <= 0 || n_levels > 32 || !(store & 7)) return nullptr;
   Amg* g = amg_new();
-  if (amg_resize(*g, n_levels) != kOk) {
+  if (amg_resize(*g, n_levels, store) != kOk) {
     amg_delete(g);
     return nullptr;
   }
@@ -303,22 +308,20 @@ int wnfea_amg_set_coarse(void* h, int n, const double* inv) {
   return amg_set_coarse(*static_cast<Amg*>(h), n, inv);
 }
 
-// prec_mask bit0: allocate FP64 cycle vectors, bit1: FP32
-int wnfea_amg_finalize(void* h, int prec_mask, int waves, int xcd_remap) {
-  if (!h || !(prec_mask & 3) || (waves != 1 && waves != 2 && waves != 4)) return kBadArg;
+// allocate the cycle vectors of the stored precisions; set the SpMV launch shape
+int wnfea_amg_finalize(void* h, int waves, int xcd_remap, int unroll) {
+  if (!h || (waves != 1 && waves != 2 && waves != 4) || (unroll != 1 && unroll != 2 && unroll != 4)) return kBadArg;
   SpmvTune t;
   t.waves = waves;
   t.xcd_remap = xcd_remap ? 1 : 0;
-  return amg_finalize(*static_cast<Amg*>(h), prec_mask, t);
+  t.unroll = unroll;
+  return amg_finalize(*static_cast<Amg*>(h), t);
 }
 
-// z = M r (one V-cycle), prec 0 = FP64, 1 = FP32
-int wnfea_amg_vcycle(void* h, int prec, const void* r, void* z, void* stream) {
+// z = M r (one V-cycle); vt 0 = FP64, 1 = FP32, 2 = FP16 values (vt >= 1: FP32 vectors)
+int wnfea_amg_vcycle(void* h, int vt, const void* r, void* z, void* stream) {
   if (!h || !r || !z) return kBadArg;
-  Amg& g = *static_cast<Amg*>(h);
-  if (prec == 0) return amg_apply<double>(g, static_cast<const double*>(r), static_cast<double*>(z), nullptr, S(stream));
-  if (prec == 1) return amg_apply<float>(g, static_cast<const float*>(r), static_cast<float*>(z), nullptr, S(stream));
-  return kBadMode;
+  return amg_apply(*static_cast<Amg*>(h), vt, r, z, nullptr, S(stream));
 }
 
 // ------------------------------------------------------------ solvers
@@ -327,37 +330,41 @@ void wnfea_solver_destroy(void* h) {
   if (h) solver_delete(static_cast<Solver*>(h));
 }
 
-// PCG on A x = b from x = 0 until ||r|| <= rtol ||b|| (recursive residual).
+// PCG from x = 0 until ||r|| <= rtol ||b|| (recursive residual).
 // prec: 0 FP64 primary values, 1 FP32 values (FP32 vectors, FP64 dots),
 // 2 FP64 equilibrated values.  pkind: 0 none, 1 Jacobi (dinv, same precision
-// as the vectors), 2 AMG (amg handle).
-int wnfea_pcg(void* solver, void* bsr, int prec, int pkind, const void* dinv, void* amg, const void* b, void* x,
-              double rtol, int max_iter, int check_every, int* iters, double* rel, void* stream) {
-  if (!solver || !bsr || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2) return kBadArg;
+// as the vectors), 2 AMG (amg handle; amg_vt 0 FP64 / 1 FP32 / 2 FP16 cycle;
+// scale, when given, makes the preconditioner z = S M(S r)).
+int wnfea_pcg(void* solver, void* bsr, int prec, int pkind, const void* dinv, void* amg, int amg_vt,
+              const double* scale, const void* b, void* x, double rtol, int max_iter, int check_every, int* iters,
+              double* rel, void* stream) {
+  if (!solver || !bsr || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2 || amg_vt < 0 || amg_vt > 2)
+    return kBadArg;
   if ((pkind == 1 && !dinv) || (pkind == 2 && !amg)) return kBadArg;
   auto* B = static_cast<WnBsr*>(bsr);
   Solver& Sv = *static_cast<Solver*>(solver);
   Amg* g = static_cast<Amg*>(amg);
   if (prec == 1)
-    return pcg_run<float>(Sv, B->m, pkind, static_cast<const float*>(dinv), g, static_cast<const float*>(b),
-                          static_cast<float*>(x), rtol, max_iter, check_every, B->tune, S(stream), iters, rel);
+    return pcg_run<float>(Sv, B->m, pkind, static_cast<const float*>(dinv), g, amg_vt, scale,
+                          static_cast<const float*>(b), static_cast<float*>(x), rtol, max_iter, check_every, B->tune,
+                          S(stream), iters, rel);
   if (prec == 0 || prec == 2) {
     if (prec == 2 && !B->m.d_v64e) return kNotReady;
     return pcg_run<double>(Sv, prec == 2 ? B->m.equilibrated() : B->m, pkind, static_cast<const double*>(dinv), g,
-                           static_cast<const double*>(b), static_cast<double*>(x), rtol, max_iter, check_every,
-                           B->tune, S(stream), iters, rel);
+                           amg_vt, scale, static_cast<const double*>(b), static_cast<double*>(x), rtol, max_iter,
+                           check_every, B->tune, S(stream), iters, rel);
   }
   return kBadMode;
 }
 
-// FP64 iterative refinement with an FP32 inner PCG (see solver.hip).
-int wnfea_ir(void* solver, void* bsr, const double* scale, int pkind, const float* dinv32, void* amg,
+// FP64 iterative refinement with an FP32 inner PCG (see solver.hip); amg_vt 1 or 2.
+int wnfea_ir(void* solver, void* bsr, const double* scale, int pkind, const float* dinv32, void* amg, int amg_vt,
              const double* f, double* u, double tol, double inner_rtol, int max_outer, int max_inner,
              int check_every, int zero_init, int* outer, int* inner, double* rel, void* stream) {
   if (!solver || !bsr || !scale || !f || !u || pkind < 0 || pkind > 2) return kBadArg;
-  if ((pkind == 1 && !dinv32) || (pkind == 2 && !amg)) return kBadArg;
+  if ((pkind == 1 && !dinv32) || (pkind == 2 && (!amg || amg_vt < 1 || amg_vt > 2))) return kBadArg;
   auto* B = static_cast<WnBsr*>(bsr);
-  return ir_run(*static_cast<Solver*>(solver), B->m, scale, pkind, dinv32, static_cast<Amg*>(amg), f, u, tol,
+  return ir_run(*static_cast<Solver*>(solver), B->m, scale, pkind, dinv32, static_cast<Amg*>(amg), amg_vt, f, u, tol,
                 inner_rtol, max_outer, max_inner, check_every, B->tune, S(stream), outer, inner, rel, zero_init);
 }
 
@@ -371,7 +378,6 @@ int wnfea_residual(void* solver, void* bsr, const double* u, const double* f, do
   if (e) return e;
   const long long n = (long long)B->m.n_rows * B->m.br;
   SpmvTune t = B->tune;
-  t.skip = nullptr;
   if ((e = spmv_residual64(B->m, u, f, r, nullptr, nullptr, w, w.sc + kScTrue, t, S(stream)))) return e;
   if ((e = launch_dot<double>(n, f, f, w, w.sc + kScF2, S(stream)))) return e;
   double h[kScCount];

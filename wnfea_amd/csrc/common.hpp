@@ -25,6 +25,27 @@ enum Status : int {
     if (_e != hipSuccess) return static_cast<int>(_e); \
   } while (0)
 
+// Device scalar slots used by the Krylov kernels.  Slots written together by
+// one two-dot reduction are consecutive (kScTrue/kScTrueS, kScTrue2/kScTrue2S).
+enum ScIdx : int {
+  kScRho = 0,
+  kScRhoPrev = 1,
+  kScBeta = 2,
+  kScAlpha = 3,
+  kScGamma = 4,
+  kScRR = 5,       // ||r||^2 of the (inner) recursion
+  kScRR0 = 6,
+  kScTrue = 7,     // ||f - K u||^2 (FP64 true residual) ...
+  kScTrueS = 8,    // ... and ||S (f - K u)||^2
+  kScF2 = 9,       // ||f||^2
+  kScInnerN = 10,  // ||r_hat||^2 of the current inner right-hand side
+  kScTol2 = 11,    // convergence threshold on kScRR
+  kScTrue2 = 15,   // ||f - K u||^2 of the restart refinement loop ...
+  kScTrue2S = 16,  // ... and its scaled companion
+  kScTmp = 17,
+  kScCount = 24,
+};
+
 constexpr int kWave = 64;
 
 inline int ceil_div(long a, long b) { return static_cast<int>((a + b - 1) / b); }
@@ -52,52 +73,26 @@ __device__ __forceinline__ T block_sum(T v, T* smem) {
   return t;
 }
 
-// Deterministic grid reduction of K values.  Each block stores its partials
-// at partials[k * gridDim.x + blockIdx.x]; the block that arrives last (one
-// counter increment per block) sums every partial in index order and writes
-// out[k].  The counter is left at zero for the next launch.  Returns true in
-// the finalising block (after out[] is written and visible to that block).
+// Deterministic grid reductions, in two launches.  A reducing kernel stores
+// one partial per block (thread 0 of block b writes partials[k*nb + b]); a
+// one-block finalise kernel (dot_finalize, blas.hip) then sums the partials
+// in index order and applies the solver bookkeeping.  No device-scope fences
+// in the reducing kernel: on a multi-XCD GPU each such fence writes back the
+// non-coherent L2, which costs far more than one extra small launch.
 template <int K>
-__device__ bool grid_reduce_last(const double (&vals)[K], double* partials, unsigned int* counter,
-                                 double* out) {
-  __shared__ bool s_last;
-  __shared__ double s_red[K][16];
+__device__ __forceinline__ void store_block_partials(const double (&vals)[K], double* partials) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = vals[k];
-    __threadfence();
-    const unsigned int t = atomicAdd(counter, 1u);
-    s_last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  const int nb = gridDim.x;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-    // strided partial sums in fixed order, then fixed-order combine
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-      s += __hip_atomic_load(&partials[(size_t)k * nb + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    s = wave_sum(s);
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    if (lane == 0) s_red[k][wid] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = blockDim.x / kWave;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double t = 0.0;
-      for (int w = 0; w < nw; ++w) t += s_red[k][w];
-      out[k] = t;
-    }
-    *counter = 0u;
-    __threadfence();
-  }
-  __syncthreads();
-  return true;
 }
+
+// Bookkeeping applied by the finalise kernel to the reduced sums.
+enum FinMode : int {
+  kFinStore = 0,  // out[k] = sum_k
+  kFinRho = 1,    // rho: sc[Rho] = sum; sc[Beta] = first ? 0 : rho / rho_prev; sc[RhoPrev] = rho
+  kFinGamma = 2,  // gamma: sc[Gamma] = sum; sc[Alpha] = rho / gamma
+  kFinRR = 3,     // ||r||^2: sc[RR] = sum; count the iteration; done flag when sc[RR] <= sc[Tol2]
+};
 
 }  // namespace wn

@@ -11,19 +11,15 @@ int Workspace::alloc(int need) {
   release();
   max_blocks = std::max(need, 4096);
   WN_HIP(hipMalloc(&partials, sizeof(double) * 4 * (size_t)max_blocks));
-  WN_HIP(hipMalloc(&counter, sizeof(unsigned int) * 16));
   WN_HIP(hipMalloc(&sc, sizeof(double) * kScCount));
-  WN_HIP(hipMemset(counter, 0, sizeof(unsigned int) * 16));
   WN_HIP(hipMemset(sc, 0, sizeof(double) * kScCount));
   return kOk;
 }
 
 void Workspace::release() {
   if (partials) (void)hipFree(partials);
-  if (counter) (void)hipFree(counter);
   if (sc) (void)hipFree(sc);
   partials = nullptr;
-  counter = nullptr;
   sc = nullptr;
   max_blocks = 0;
 }
@@ -34,8 +30,9 @@ void SellMatrix::release() {
     eq_alias = nullptr;
   }
   for (void* p : {(void*)d_off, (void*)d_len, (void*)d_row, (void*)d_col, (void*)d_slot, (void*)d_v64, (void*)d_v64e,
-                  (void*)d_v32})
+                  (void*)d_v32, (void*)d_v16})
     if (p) (void)hipFree(p);
+  d_v16 = nullptr;
   d_off = nullptr;
   d_len = d_row = d_col = nullptr;
   d_slot = nullptr;
@@ -49,16 +46,29 @@ SellMatrix& SellMatrix::equilibrated() {
   if (!eq_alias) eq_alias = new SellMatrix();
   SellMatrix& a = *eq_alias;
   a.br = br; a.bc = bc; a.n_rows = n_rows; a.n_cols = n_cols; a.n_slices = n_slices; a.sigma = sigma;
+  a.lanes = lanes;
   a.nnzb = nnzb; a.slots = slots; a.d_off = d_off; a.d_len = d_len; a.d_row = d_row; a.d_col = d_col;
-  a.d_slot = d_slot; a.d_v64 = d_v64e; a.d_v64e = nullptr; a.d_v32 = d_v32;
+  a.d_slot = d_slot; a.d_v64 = d_v64e; a.d_v64e = nullptr; a.d_v32 = d_v32; a.d_v16 = d_v16;
   return a;
 }
 
+// Lanes per block row: enough lanes that short matrices with long rows still
+// spread over many wavefronts (about 2048 x 64 lane-rows), but no more than
+// an eighth of the average row length, so that every lane has work.
+static int auto_lanes(int n_rows, long long nnzb) {
+  const double avg = n_rows > 0 ? double(nnzb) / n_rows : 0.0;
+  int L = 1;
+  while (L < 64 && (long long)n_rows * L < 64LL * 2048 && L * 8 < avg) L *= 2;
+  return L;
+}
+
 // Host plan: stable sort of rows by length (descending) inside windows of
-// `sigma` rows, slices of 64 consecutive sorted rows, column-major slots.
-int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* rowptr, const int* colidx, int sigma_) {
+// `sigma` rows, slices of 64/L consecutive sorted rows, column-major slots.
+int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* rowptr, const int* colidx, int sigma_,
+                      int lanes_) {
   if (br_ <= 0 || bc_ <= 0 || n_rows_ < 0 || n_cols_ < 0 || !rowptr || (!colidx && rowptr[n_rows_] > 0))
     return kBadArg;
+  if (lanes_ > 64 || (lanes_ > 0 && (lanes_ & (lanes_ - 1)))) return kBadArg;
   release();
   br = br_;
   bc = bc_;
@@ -67,9 +77,11 @@ int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* row
   sigma = sigma_ <= 0 ? 4096 : sigma_;
   sigma = ((sigma + 63) / 64) * 64;
   nnzb = rowptr[n_rows];
+  lanes = lanes_ > 0 ? lanes_ : auto_lanes(n_rows, nnzb);
+  const int rows_per_slice = 64 / lanes;
   h_rowptr.assign(rowptr, rowptr + n_rows + 1);
   h_colidx.assign(colidx, colidx + nnzb);
-  n_slices = (n_rows + 63) / 64;
+  n_slices = (n_rows + rows_per_slice - 1) / rows_per_slice;
   std::vector<int> order(n_rows);
   std::iota(order.begin(), order.end(), 0);
   for (int w0 = 0; w0 < n_rows; w0 += sigma) {
@@ -79,35 +91,38 @@ int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* row
     });
   }
   std::vector<long long> off(n_slices + 1, 0);
-  std::vector<int> len(n_slices * 64, 0), row(n_slices * 64, -1);
+  std::vector<int> len((size_t)n_slices * 64, 0), row((size_t)n_slices * 64, -1);
   for (int s = 0; s < n_slices; ++s) {
-    int mx = 0;
-    for (int l = 0; l < 64; ++l) {
-      const int k = s * 64 + l;
-      if (k < n_rows) {
-        const int r = order[k];
-        row[k] = r;
-        len[k] = rowptr[r + 1] - rowptr[r];
-        mx = std::max(mx, len[k]);
+    int steps = 0;
+    for (int r = 0; r < rows_per_slice; ++r) {
+      const long long k = (long long)s * rows_per_slice + r;
+      if (k >= n_rows) break;
+      const int g = order[k];
+      const int rl = rowptr[g + 1] - rowptr[g];
+      for (int q = 0; q < lanes; ++q) {
+        const int ln = s * 64 + r * lanes + q;
+        row[ln] = g;
+        len[ln] = rl > q ? (rl - q + lanes - 1) / lanes : 0;
+        steps = std::max(steps, len[ln]);
       }
     }
-    off[s + 1] = off[s] + 64LL * mx;
+    off[s + 1] = off[s] + 64LL * steps;
   }
   slots = off[n_slices];
   std::vector<int> col(std::max<long long>(slots, 1), 0);
   std::vector<long long> slot(std::max<long long>(nnzb, 1), 0);
   for (int s = 0; s < n_slices; ++s)
-    for (int l = 0; l < 64; ++l) {
-      const int k = s * 64 + l;
-      if (row[k] < 0) continue;
-      const int r = row[k];
-      for (int j = 0; j < len[k]; ++j) {
-        const long long sl = off[s] + 64LL * j + l;
-        col[sl] = colidx[rowptr[r] + j];
-        slot[rowptr[r] + j] = sl;
+    for (int r = 0; r < rows_per_slice; ++r) {
+      const long long k = (long long)s * rows_per_slice + r;
+      if (k >= n_rows) break;
+      const int g = order[k];
+      for (int j = 0; j < rowptr[g + 1] - rowptr[g]; ++j) {
+        const long long sl = off[s] + 64LL * (j / lanes) + r * lanes + (j % lanes);
+        col[sl] = colidx[rowptr[g] + j];
+        slot[rowptr[g] + j] = sl;
       }
       // padding slots keep column 0 and value 0; lanes stop at their own
-      // length, so they are never read.
+      // step count, so they are never read.
     }
   WN_HIP(hipMalloc(&d_off, sizeof(long long) * off.size()));
   WN_HIP(hipMalloc(&d_len, sizeof(int) * len.size()));
@@ -122,8 +137,12 @@ int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* row
   return kOk;
 }
 
-int SellMatrix::ensure_values(int want64, int want32) {
+int SellMatrix::ensure_values(int want64, int want32, int want16) {
   const size_t nv = (size_t)std::max<long long>(slots, 1) * br * bc;
+  if (want16 && !d_v16) {
+    WN_HIP(hipMalloc(&d_v16, sizeof(_Float16) * nv));
+    WN_HIP(hipMemset(d_v16, 0, sizeof(_Float16) * nv));
+  }
   if (want64 && !d_v64) {
     WN_HIP(hipMalloc(&d_v64, sizeof(double) * nv));
     WN_HIP(hipMemset(d_v64, 0, sizeof(double) * nv));
@@ -140,7 +159,8 @@ __device__ __forceinline__ long long value_base(long long slot, int bb) {
 }
 
 __global__ void std_to_sell_kernel(long long nnzb, int bb, const long long* __restrict__ slot,
-                                   const double* __restrict__ vs, double* __restrict__ v64, float* __restrict__ v32) {
+                                   const double* __restrict__ vs, double* __restrict__ v64, float* __restrict__ v32,
+                                   _Float16* __restrict__ v16) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= nnzb * bb) return;
   const long long k = t / bb;
@@ -149,6 +169,7 @@ __global__ void std_to_sell_kernel(long long nnzb, int bb, const long long* __re
   const double v = vs[t];
   if (v64) v64[p] = v;
   if (v32) v32[p] = float(v);
+  if (v16) v16[p] = _Float16(float(v));
 }
 
 __global__ void sell_to_std_kernel(long long nnzb, int bb, const long long* __restrict__ slot,
@@ -160,13 +181,13 @@ __global__ void sell_to_std_kernel(long long nnzb, int bb, const long long* __re
   vs[t] = v64[value_base(slot[k], bb) + 64LL * e];
 }
 
-int sell_from_std(SellMatrix& A, const double* vals_std, int to64, int to32, hipStream_t s) {
-  int r = A.ensure_values(to64, to32);
+int sell_from_std(SellMatrix& A, const double* vals_std, int to64, int to32, hipStream_t s, int to16) {
+  int r = A.ensure_values(to64, to32, to16);
   if (r) return r;
   const long long n = A.nnzb * A.br * A.bc;
   if (n == 0) return kOk;
   hipLaunchKernelGGL(std_to_sell_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, A.nnzb, A.br * A.bc, A.d_slot,
-                     vals_std, to64 ? A.d_v64 : nullptr, to32 ? A.d_v32 : nullptr);
+                     vals_std, to64 ? A.d_v64 : nullptr, to32 ? A.d_v32 : nullptr, to16 ? A.d_v16 : nullptr);
   return static_cast<int>(hipGetLastError());
 }
 
@@ -251,7 +272,7 @@ int extract_diag(const SellMatrix& A, const long long* diag_block, double* diag,
 __global__ void equilibrate_kernel(long long nnzb, int b, const int* __restrict__ rowidx,
                                    const int* __restrict__ colidx, const long long* __restrict__ slot,
                                    const double* __restrict__ scale, const double* __restrict__ v64,
-                                   float* __restrict__ v32, double* __restrict__ v64e) {
+                                   float* __restrict__ v32, double* __restrict__ v64e, _Float16* __restrict__ v16) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const int bb = b * b;
   if (t >= nnzb * bb) return;
@@ -261,12 +282,13 @@ __global__ void equilibrate_kernel(long long nnzb, int b, const int* __restrict_
   const double v = scale[(long long)rowidx[k] * b + r] * v64[p] * scale[(long long)colidx[k] * b + c];
   v32[p] = float(v);
   if (v64e) v64e[p] = v;
+  if (v16) v16[p] = _Float16(float(v));
 }
 
 int equilibrate(SellMatrix& A, const int* d_rowidx, const int* d_colidx, const double* scale, int want64e,
-                hipStream_t s) {
+                hipStream_t s, int want16) {
   if (A.br != A.bc || !A.d_v64) return kBadArg;
-  int r = A.ensure_values(0, 1);
+  int r = A.ensure_values(0, 1, want16);
   if (r) return r;
   const size_t nv = (size_t)std::max<long long>(A.slots, 1) * A.br * A.bc;
   if (want64e && !A.d_v64e) {
@@ -276,27 +298,48 @@ int equilibrate(SellMatrix& A, const int* d_rowidx, const int* d_colidx, const d
   const long long n = A.nnzb * A.br * A.bc;
   if (n == 0) return kOk;
   hipLaunchKernelGGL(equilibrate_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, A.nnzb, A.br, d_rowidx, d_colidx,
-                     A.d_slot, scale, A.d_v64, A.d_v32, want64e ? A.d_v64e : nullptr);
+                     A.d_slot, scale, A.d_v64, A.d_v32, want64e ? A.d_v64e : nullptr, A.d_v16);
   return static_cast<int>(hipGetLastError());
 }
 
 // ---------------------------------------------------------------- SpMV dispatch
+template <int BR, int BC, typename VT, typename XT, int W, class Op>
+static void launch_w(const SellView<VT>& v, const XT* x, const Op& op, int grid, int unroll, int xcd, const int* flags,
+                     int guard, hipStream_t s) {
+  // keep the unrolled value registers within budget: U*BB values of VT
+  constexpr int kMaxU = (BR * BC * (int)sizeof(VT) <= 48) ? 4 : ((BR * BC * (int)sizeof(VT) <= 96) ? 2 : 1);
+  const int u = std::min(unroll, kMaxU);
+  if (u >= 4 && kMaxU >= 4)
+    hipLaunchKernelGGL((sell_spmv_kernel<BR, BC, VT, XT, W, (kMaxU >= 4 ? 4 : 1), Op>), dim3(grid), dim3(64 * W), 0,
+                       s, v, x, op, xcd, flags, guard);
+  else if (u >= 2 && kMaxU >= 2)
+    hipLaunchKernelGGL((sell_spmv_kernel<BR, BC, VT, XT, W, (kMaxU >= 2 ? 2 : 1), Op>), dim3(grid), dim3(64 * W), 0,
+                       s, v, x, op, xcd, flags, guard);
+  else
+    hipLaunchKernelGGL((sell_spmv_kernel<BR, BC, VT, XT, W, 1, Op>), dim3(grid), dim3(64 * W), 0, s, v, x, op, xcd,
+                       flags, guard);
+}
+
+static int sell_grid(const SellMatrix& A, const SpmvTune& t) {
+  const int per = (A.n_slices + 7) / 8;
+  return t.xcd_remap ? per * 8 : A.n_slices;
+}
+
 template <int BR, int BC, typename VT, typename XT, class Op>
 static int launch_sell(const SellMatrix& A, const XT* x, const Op& op, const SpmvTune& t, hipStream_t s) {
   if (A.n_slices == 0) return kOk;
   const SellView<VT> v = A.view<VT>();
   if (!v.val) return kNotReady;
-  const int per = (A.n_slices + 7) / 8;
-  const int grid = t.xcd_remap ? per * 8 : A.n_slices;
+  const int grid = sell_grid(A, t);
   switch (t.waves) {
     case 2:
-      hipLaunchKernelGGL((sell_spmv_kernel<BR, BC, VT, XT, 2, Op>), dim3(grid), dim3(128), 0, s, v, x, op, t.xcd_remap, t.skip);
+      launch_w<BR, BC, VT, XT, 2, Op>(v, x, op, grid, t.unroll, t.xcd_remap, t.flags, t.guard, s);
       break;
     case 4:
-      hipLaunchKernelGGL((sell_spmv_kernel<BR, BC, VT, XT, 4, Op>), dim3(grid), dim3(256), 0, s, v, x, op, t.xcd_remap, t.skip);
+      launch_w<BR, BC, VT, XT, 4, Op>(v, x, op, grid, t.unroll, t.xcd_remap, t.flags, t.guard, s);
       break;
     default:
-      hipLaunchKernelGGL((sell_spmv_kernel<BR, BC, VT, XT, 1, Op>), dim3(grid), dim3(64), 0, s, v, x, op, t.xcd_remap, t.skip);
+      launch_w<BR, BC, VT, XT, 1, Op>(v, x, op, grid, t.unroll, t.xcd_remap, t.flags, t.guard, s);
   }
   return static_cast<int>(hipGetLastError());
 }
@@ -333,14 +376,16 @@ int spmv_dot(const SellMatrix& A, const VT* p, VT* y, Workspace& w, int with_alp
   if (A.br != A.bc) return kBadArg;
   int r = w.alloc(A.n_slices + 8);
   if (r) return r;
-#define CALL(R, C)                                                                                          \
-  {                                                                                                         \
-    OpSpmvDot<R, VT, AT> op{y, p, w.partials, w.counter, w.sc + kScGamma, w.sc + kScRho,                    \
-                            with_alpha ? w.sc + kScAlpha : nullptr};                                        \
-    return launch_sell<R, C, VT, VT>(A, p, op, t, s);                                                       \
+#define CALL(R, C)                                       \
+  {                                                      \
+    OpSpmvDot<R, VT, AT> op{y, p, w.partials};           \
+    r = launch_sell<R, C, VT, VT>(A, p, op, t, s);       \
   }
   if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
 #undef CALL
+  if (r || A.n_slices == 0) return r;
+  return launch_finalize(w.partials, sell_grid(A, t), 1, with_alpha ? kFinGamma : kFinStore, w.sc + kScGamma, w.sc,
+                         nullptr, 0, t.guard ? t.flags : nullptr, s);
 }
 template int spmv_dot<double>(const SellMatrix&, const double*, double*, Workspace&, int, const SpmvTune&,
                               hipStream_t);
@@ -351,58 +396,74 @@ int spmv_residual64(const SellMatrix& A, const double* x, const double* b, doubl
   if (A.br != A.bc) return kBadArg;
   int e = w.alloc(A.n_slices + 8);
   if (e) return e;
-#define CALL(R, C)                                                         \
-  {                                                                        \
-    OpResidual<R, double> op{b, r, rs, scale, w.partials, w.counter, out}; \
-    return launch_sell<R, C, double, double>(A, x, op, t, s);              \
+#define CALL(R, C)                                                  \
+  {                                                                 \
+    OpResidual<R, double> op{b, r, rs, scale, w.partials};          \
+    e = launch_sell<R, C, double, double>(A, x, op, t, s);          \
   }
   if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
 #undef CALL
+  if (e || A.n_slices == 0) return e;
+  return launch_finalize(w.partials, sell_grid(A, t), 2, kFinStore, out, w.sc, nullptr, 0,
+                         t.guard ? t.flags : nullptr, s);
 }
 
-template <typename T>
+template <typename VT, typename T>
 int spmv_cheb(const SellMatrix& A, const T* x_in, T* x_out, const T* b, const T* dinv, T* d, double c1, double c2,
               const SpmvTune& t, hipStream_t s) {
   if (A.br != A.bc) return kBadArg;
 #define CALL(R, C)                                                  \
   {                                                                 \
     OpCheb<R, T> op{b, dinv, x_in, x_out, d, T(c1), T(c2)};         \
-    return launch_sell<R, C, T, T>(A, x_in, op, t, s);              \
+    return launch_sell<R, C, VT, T>(A, x_in, op, t, s);             \
   }
   if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
 #undef CALL
 }
-template int spmv_cheb<float>(const SellMatrix&, const float*, float*, const float*, const float*, float*, double,
-                              double, const SpmvTune&, hipStream_t);
-template int spmv_cheb<double>(const SellMatrix&, const double*, double*, const double*, const double*, double*,
-                               double, double, const SpmvTune&, hipStream_t);
 
-template <typename T>
+template <typename VT, typename T>
 int spmv_defect(const SellMatrix& A, const T* x, const T* b, T* res, const SpmvTune& t, hipStream_t s) {
   if (A.br != A.bc) return kBadArg;
 #define CALL(R, C)                                     \
   {                                                    \
     OpDefect<R, T> op{b, res};                         \
-    return launch_sell<R, C, T, T>(A, x, op, t, s);    \
+    return launch_sell<R, C, VT, T>(A, x, op, t, s);   \
   }
   if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
 #undef CALL
 }
-template int spmv_defect<float>(const SellMatrix&, const float*, const float*, float*, const SpmvTune&, hipStream_t);
-template int spmv_defect<double>(const SellMatrix&, const double*, const double*, double*, const SpmvTune&,
-                                 hipStream_t);
 
-template <typename T>
+template <typename VT, typename T>
 int spmv_addto(const SellMatrix& A, const T* e, T* x, const SpmvTune& t, hipStream_t s) {
-#define CALL(R, C)                                  \
-  {                                                 \
-    OpAddTo<R, T> op{x};                            \
-    return launch_sell<R, C, T, T>(A, e, op, t, s); \
+#define CALL(R, C)                                   \
+  {                                                  \
+    OpAddTo<R, T> op{x};                             \
+    return launch_sell<R, C, VT, T>(A, e, op, t, s); \
   }
   WN_BLOCK_DISPATCH(A, CALL);
 #undef CALL
 }
-template int spmv_addto<float>(const SellMatrix&, const float*, float*, const SpmvTune&, hipStream_t);
-template int spmv_addto<double>(const SellMatrix&, const double*, double*, const SpmvTune&, hipStream_t);
+
+template <typename VT, typename T>
+int spmv_restrict(const SellMatrix& A, const T* x, T* y, const SpmvTune& t, hipStream_t s) {
+  using AT = typename AccT<VT>::type;
+#define CALL(R, C)                                   \
+  {                                                  \
+    OpAxpby<R, T, AT> op{y, AT(1), AT(0)};           \
+    return launch_sell<R, C, VT, T>(A, x, op, t, s); \
+  }
+  WN_BLOCK_DISPATCH(A, CALL);
+#undef CALL
+}
+
+#define WN_VINST(VT, T)                                                                                          \
+  template int spmv_cheb<VT, T>(const SellMatrix&, const T*, T*, const T*, const T*, T*, double, double,       \
+                                const SpmvTune&, hipStream_t);                                                 \
+  template int spmv_defect<VT, T>(const SellMatrix&, const T*, const T*, T*, const SpmvTune&, hipStream_t);   \
+  template int spmv_addto<VT, T>(const SellMatrix&, const T*, T*, const SpmvTune&, hipStream_t);              \
+  template int spmv_restrict<VT, T>(const SellMatrix&, const T*, T*, const SpmvTune&, hipStream_t);
+WN_VINST(double, double)
+WN_VINST(float, float)
+WN_VINST(_Float16, float)
 
 }  // namespace wn

@@ -1,15 +1,18 @@
 // Device layout of block-sparse matrices: sliced ELL over block rows.
 //
-// Block rows are grouped into slices of 64 (one wavefront lane per block
-// row).  Rows are first sorted by block count inside windows of `sigma`
-// consecutive rows (stable, longest first), so the rows sharing a slice have
-// nearly equal lengths; the slice is then padded to its longest row.  Within
-// a slice, block j of the row in lane l sits at slot  off[s] + j*64 + l,  and
-// entry k (row-major inside the BRxBC block) of that block sits at value
-// index  off[s]*BB + (j*BB + k)*64 + l.  A wavefront therefore reads every
-// block entry of 64 rows as one contiguous 256 B (FP32) / 512 B (FP64) load,
-// and each lane accumulates whole block rows in registers: no cross-lane
-// reduction and no atomics.  Padding lanes have length 0 and row -1.
+// Each block row is owned by L consecutive lanes of a wavefront (L = 1 for
+// long matrices with short rows such as the fine 3x3 operator; L up to 64 for
+// short matrices with long rows such as restriction and coarse operators), so
+// a slice holds 64/L rows.  Rows are first sorted by block count inside
+// windows of `sigma` consecutive rows (stable, longest first), so the rows
+// sharing a slice have nearly equal lengths.  Block j of the row in slice
+// position r goes to lane l = r*L + j%L at step t = j/L; its slot is
+// off[s] + t*64 + l, and entry k (row-major inside the BRxBC block) sits at
+// value index off[s]*BB + (t*BB + k)*64 + l.  A wavefront therefore reads
+// every block entry of a step as one contiguous 256 B (FP32) / 512 B (FP64)
+// load; each lane accumulates whole block rows in registers and the L lanes
+// of a row meet in a shuffle reduction.  No atomics.  Padding lanes have
+// zero steps and row -1.
 #pragma once
 #include "common.hpp"
 
@@ -19,6 +22,7 @@ template <typename T>
 struct SellView {
   int n_rows;              // block rows
   int n_slices;
+  int lanes;               // L: lanes per block row (power of two, <= 64)
   const long long* off;    // [n_slices+1] slot offset of each slice
   const int* lane_len;     // [n_slices*64]
   const int* lane_row;     // [n_slices*64]
@@ -27,6 +31,7 @@ struct SellView {
 };
 
 template <typename T> struct AccT { using type = T; };
+template <> struct AccT<_Float16> { using type = float; };  // FP16 storage, FP32 arithmetic
 
 // Slice handled by this block.  With xcd_remap the 8 XCDs (blocks are issued
 // round-robin over XCDs) each get one contiguous range of slices, so rows that
@@ -43,26 +48,62 @@ __device__ __forceinline__ int slice_of_block(int n_slices, int xcd_remap) {
 // block columns round-robin; their partial sums meet in LDS.  `Op` receives
 // (block row, accumulated BR values) for every real row and may return a
 // contribution to one fused dot product, reduced deterministically.
-template <int BR, int BC, typename VT, typename XT, int W, class Op>
+// Launch guard read from device flags (uniform over the launch):
+//   mode 0: always run; 1: skip once flags[0] (done) is set.
+__device__ __forceinline__ bool guard_skip(const int* flags, int mode) {
+  return flags && mode != 0 && flags[0] != 0;
+}
+
+// Generic sliced-ELL block SpMV.  W wavefronts share one slice and split its
+// steps round-robin; their partial sums meet in LDS.  Each lane keeps U blocks
+// in flight per loop trip (U column loads, then U*BB value loads and U x
+// gathers, then the FMAs) to cover memory latency.  `Op` receives (block row,
+// accumulated BR values) for every real row and may add to up to two fused
+// dot products, reduced deterministically.
+template <int BR, int BC, typename VT, typename XT, int W, int U, class Op>
 __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const XT* __restrict__ x, Op op,
-                                                           int xcd_remap, const int* skip) {
-  if (skip && *skip) return;  // uniform: every block of the launch sees the same flag
+                                                           int xcd_remap, const int* flags, int guard) {
+  if (guard_skip(flags, guard)) return;
   using AT = typename AccT<VT>::type;
   constexpr int BB = BR * BC;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int s = slice_of_block(A.n_slices, xcd_remap);  // uniform over the block
-  double dot = 0.0;
+  double dot[2] = {0.0, 0.0};
   if (s < A.n_slices) {
     const int slot_lane = s * 64 + lane;
-    const int len = A.lane_len[slot_lane];
+    const int len = A.lane_len[slot_lane];  // steps of this lane
     const long long base = A.off[s];
     AT acc[BR];
 #pragma unroll
     for (int r = 0; r < BR; ++r) acc[r] = AT(0);
     const int* __restrict__ colp = A.col + base + lane;
     const VT* __restrict__ valp = A.val + base * BB + lane;
-    for (int j = wave; j < len; j += W) {
+    int j = wave;
+    if constexpr (U > 1) {
+      for (; j + (U - 1) * W < len; j += U * W) {
+        int c[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[u] = colp[(long long)(j + u * W) * 64];
+        VT v[U][BB];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int e = 0; e < BB; ++e) v[u][e] = valp[((long long)(j + u * W) * BB + e) * 64];
+        XT xv[U][BC];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int q = 0; q < BC; ++q) xv[u][q] = x[(long long)c[u] * BC + q];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int r = 0; r < BR; ++r)
+#pragma unroll
+            for (int q = 0; q < BC; ++q) acc[r] = fma(AT(v[u][r * BC + q]), AT(xv[u][q]), acc[r]);
+      }
+    }
+    for (; j < len; j += W) {
       const int c = colp[(long long)j * 64];
       const VT* v = valp + (long long)j * BB * 64;
       XT xv[BC];
@@ -87,13 +128,19 @@ __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const
           for (int r = 0; r < BR; ++r) acc[r] += red[w][r][lane];
       }
     }
-    if (wave == 0 && len > 0) dot = op.row(A.lane_row[slot_lane], acc);
+    // the L lanes of a row hold partial sums: butterfly over the low bits
+    for (int off = A.lanes >> 1; off > 0; off >>= 1)
+#pragma unroll
+      for (int r = 0; r < BR; ++r) acc[r] += __shfl_xor(acc[r], off, 64);
+    const int row = A.lane_row[slot_lane];
+    if (wave == 0 && row >= 0 && (lane & (A.lanes - 1)) == 0) op.row(row, acc, dot);
   }
-  if constexpr (Op::kDot) {
+  if constexpr (Op::kDots > 0) {
     __shared__ double sred[W];
-    const double bs = block_sum(dot, sred);
-    double v[1] = {bs};
-    if (grid_reduce_last<1>(v, op.partials, op.counter, op.dot_out)) op.finalize();
+    double v[Op::kDots];
+#pragma unroll
+    for (int k = 0; k < Op::kDots; ++k) v[k] = block_sum(dot[k], sred);
+    store_block_partials<Op::kDots>(v, op.partials);
   }
 }
 
@@ -101,13 +148,11 @@ __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const
 // y = alpha * A x + beta * y
 template <int BR, typename YT, typename AT>
 struct OpAxpby {
-  static constexpr bool kDot = false;
+  static constexpr int kDots = 0;
   YT* y;
   AT alpha, beta;
   double* partials = nullptr;
-  unsigned int* counter = nullptr;
-  double* dot_out = nullptr;
-  __device__ __forceinline__ double row(int i, const AT (&acc)[BR]) const {
+  __device__ __forceinline__ void row(int i, const AT (&acc)[BR], double (&)[2]) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
       const long long k = (long long)i * BR + r;
@@ -115,72 +160,55 @@ struct OpAxpby {
       if (beta != AT(0)) v += beta * AT(y[k]);
       y[k] = YT(v);
     }
-    return 0.0;
   }
-  __device__ void finalize() const {}
 };
 
-// y = A p and gamma = p . (A p); finalize: alpha = rho / gamma (PCG step length).
+// y = A p and gamma = p . (A p) (block partials; the finaliser forms alpha = rho / gamma).
 template <int BR, typename YT, typename AT>
 struct OpSpmvDot {
-  static constexpr bool kDot = true;
+  static constexpr int kDots = 1;
   YT* y;
   const YT* p;        // same vector as the gathered x (square matrix)
   double* partials;
-  unsigned int* counter;
-  double* dot_out;    // gamma
-  const double* rho;  // may be null (plain fused dot)
-  double* alpha;      // may be null
-  __device__ __forceinline__ double row(int i, const AT (&acc)[BR]) const {
-    double d = 0.0;
+  __device__ __forceinline__ void row(int i, const AT (&acc)[BR], double (&d)[2]) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
       const long long k = (long long)i * BR + r;
       y[k] = YT(acc[r]);
-      d += double(p[k]) * double(acc[r]);
-    }
-    return d;
-  }
-  __device__ void finalize() const {
-    if (threadIdx.x == 0 && alpha) {
-      const double g = *dot_out;
-      *alpha = (g != 0.0) ? (*rho / g) : 0.0;
+      d[0] += double(p[k]) * double(acc[r]);
     }
   }
 };
 
-// r = b - A x (FP64 true residual) with ||r||^2, and optionally the scaled
-// single-precision copy rs = scale_vec .* r used as the inner right-hand side.
+// r = b - A x (FP64 true residual): dots ||r||^2 and ||scale .* r||^2; writes
+// r (optional) and the single-precision rs = scale .* r (optional).
 template <int BR, typename AT>
 struct OpResidual {
-  static constexpr bool kDot = true;
+  static constexpr int kDots = 2;
   const double* b;
-  double* r;           // may be null
-  float* rs;           // may be null
-  const double* scale; // per-DOF scale for rs (may be null = 1)
-  double* partials;
-  unsigned int* counter;
-  double* dot_out;     // ||r||^2
-  __device__ __forceinline__ double row(int i, const AT (&acc)[BR]) const {
-    double d = 0.0;
+  double* r;            // may be null
+  float* rs;            // may be null
+  const double* scale;  // per-DOF scale (may be null = 1)
+  double* partials;     // [0] ||r||^2, [1] ||scale r||^2 partials
+  __device__ __forceinline__ void row(int i, const AT (&acc)[BR], double (&d)[2]) const {
 #pragma unroll
     for (int r0 = 0; r0 < BR; ++r0) {
       const long long k = (long long)i * BR + r0;
       const double v = b[k] - double(acc[r0]);
+      const double vs = scale ? scale[k] * v : v;
       if (r) r[k] = v;
-      if (rs) rs[k] = float(scale ? scale[k] * v : v);
-      d += v * v;
+      if (rs) rs[k] = float(vs);
+      d[0] += v * v;
+      d[1] += vs * vs;
     }
-    return d;
   }
-  __device__ void finalize() const {}
 };
 
 // One Chebyshev smoothing step on D^-1 A (D^-1 given as `dinv`):
 //   res = b - A x_in;  d = c1 * d + c2 * dinv .* res;  x_out = x_in + d
 template <int BR, typename T>
 struct OpCheb {
-  static constexpr bool kDot = false;
+  static constexpr int kDots = 0;
   const T* b;
   const T* dinv;
   const T* x_in;
@@ -188,9 +216,7 @@ struct OpCheb {
   T* d;
   T c1, c2;
   double* partials = nullptr;
-  unsigned int* counter = nullptr;
-  double* dot_out = nullptr;
-  __device__ __forceinline__ double row(int i, const T (&acc)[BR]) const {
+  __device__ __forceinline__ void row(int i, const T (&acc)[BR], double (&)[2]) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
       const long long k = (long long)i * BR + r;
@@ -199,45 +225,35 @@ struct OpCheb {
       d[k] = dn;
       x_out[k] = x_in[k] + dn;
     }
-    return 0.0;
   }
-  __device__ void finalize() const {}
 };
 
 // res = b - A x (working precision), written to `res`.
 template <int BR, typename T>
 struct OpDefect {
-  static constexpr bool kDot = false;
+  static constexpr int kDots = 0;
   const T* b;
   T* res;
   double* partials = nullptr;
-  unsigned int* counter = nullptr;
-  double* dot_out = nullptr;
-  __device__ __forceinline__ double row(int i, const T (&acc)[BR]) const {
+  __device__ __forceinline__ void row(int i, const T (&acc)[BR], double (&)[2]) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
       const long long k = (long long)i * BR + r;
       res[k] = b[k] - acc[r];
     }
-    return 0.0;
   }
-  __device__ void finalize() const {}
 };
 
 // x += A e (prolongation: the gathered vector is the coarse correction).
 template <int BR, typename T>
 struct OpAddTo {
-  static constexpr bool kDot = false;
+  static constexpr int kDots = 0;
   T* x;
   double* partials = nullptr;
-  unsigned int* counter = nullptr;
-  double* dot_out = nullptr;
-  __device__ __forceinline__ double row(int i, const T (&acc)[BR]) const {
+  __device__ __forceinline__ void row(int i, const T (&acc)[BR], double (&)[2]) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r) x[(long long)i * BR + r] += acc[r];
-    return 0.0;
   }
-  __device__ void finalize() const {}
 };
 
 }  // namespace wn

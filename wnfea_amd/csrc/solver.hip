@@ -1,6 +1,8 @@
 // Smoothed-aggregation AMG V-cycle and the PCG / iterative-refinement drivers.
 //
-// V-cycle (per level l < coarsest), working precision T:
+// V-cycle (per level l < coarsest), stored values VT (double, float or
+// _Float16) with vectors and arithmetic T (double for VT = double, float
+// otherwise):
 //   pre-smooth  : Chebyshev(deg) on D^-1 A from x = 0   (1 elementwise + deg-1 SpMV kernels)
 //   defect      : res = b - A x                        (1 SpMV)
 //   restrict    : b_{l+1} = R res                      (1 SpMV with R = P^T stored explicitly)
@@ -13,9 +15,13 @@
 //
 // PCG keeps rho/alpha/beta/||r||^2 on the device; a done flag raised by the
 // x/r update makes every later kernel of the loop return at once, so the host
-// enqueues iterations in chunks and checks the flag only between chunks.
+// enqueues iterations in chunks and checks the flag only between chunks.  The
+// preconditioner may run in a lower precision than the Krylov vectors and on
+// a diagonally scaled operator: z = S M(S r); the precision conversions and
+// the scaling are fused into the rho, p-update and x/r-update kernels.
 #include <cmath>
 #include <memory>
+#include <type_traits>
 #include <utility>
 #include "wn.hpp"
 
@@ -31,13 +37,14 @@ struct AmgLevel {
   float* dinv32 = nullptr;
   double lmax = 2.0, ratio = 30.0;
   int degree = 2;
-  // level vectors (b, x, tmp, d, res) per precision
+  // level vectors (b, x, tmp, d, res) per arithmetic precision
   double* v64[5] = {};
   float* v32[5] = {};
 };
 
 struct Amg {
   std::vector<std::unique_ptr<AmgLevel>> L;
+  int store = 3;  // bit0 FP64, bit1 FP32, bit2 FP16 values
   int n_coarse = 0;
   double* inv = nullptr;  // n_coarse^2, row-major
   SpmvTune tune;
@@ -75,7 +82,7 @@ int amg_alloc_vectors(Amg& g, int prec_mask) {
     const size_t n = std::max(l.n, 1);
     for (int k = 0; k < 5; ++k) {
       if ((prec_mask & 1) && !l.v64[k]) WN_HIP(hipMalloc(&l.v64[k], sizeof(double) * n));
-      if ((prec_mask & 2) && !l.v32[k]) WN_HIP(hipMalloc(&l.v32[k], sizeof(float) * n));
+      if ((prec_mask & 6) && !l.v32[k]) WN_HIP(hipMalloc(&l.v32[k], sizeof(float) * n));
     }
   }
   return kOk;
@@ -127,11 +134,12 @@ static ChebCoef cheb_coefs(double lmax, double ratio, int deg) {
   return c;
 }
 
-template <typename T>
+template <typename VT, typename T>
 static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t s) {
   AmgLevel& L = *g.L[l];
   SpmvTune t = g.tune;
-  t.skip = skip;
+  t.flags = skip;
+  t.guard = skip ? 1 : 0;
   const long long n = L.n;
   if (l == (int)g.L.size() - 1) {
     if (L.n != g.n_coarse || !g.inv) return kNotReady;
@@ -158,31 +166,42 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   hipLaunchKernelGGL(cheb_first_kernel<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, b, dinv, d, cur, T(c.c2[0]), skip);
   if ((r = static_cast<int>(hipGetLastError()))) return r;
   for (int k = 1; k < deg; ++k) {
-    if ((r = spmv_cheb<T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
+    if ((r = spmv_cheb<VT, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
     std::swap(cur, oth);
   }
   // defect and restriction
-  if ((r = spmv_defect<T>(A, cur, b, res, t, s))) return r;
+  if ((r = spmv_defect<VT, T>(A, cur, b, res, t, s))) return r;
   AmgLevel& N = *g.L[l + 1];
   T** nv = level_vecs<T>(N);
-  if ((r = spmv_axpby<T, T>(L.R, res, nv[0], 1.0, 0.0, t, s))) return r;
-  if ((r = vcycle<T>(g, l + 1, nv[0], nv[1], skip, s))) return r;
-  if ((r = spmv_addto<T>(L.P, nv[1], cur, t, s))) return r;  // cur == post here
+  if ((r = spmv_restrict<VT, T>(L.R, res, nv[0], t, s))) return r;
+  if ((r = vcycle<VT, T>(g, l + 1, nv[0], nv[1], skip, s))) return r;
+  if ((r = spmv_addto<VT, T>(L.P, nv[1], cur, t, s))) return r;  // cur == post here
   oth = (cur == x) ? tmp : x;
   for (int k = 0; k < deg; ++k) {
-    if ((r = spmv_cheb<T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
+    if ((r = spmv_cheb<VT, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
     std::swap(cur, oth);
   }
   return kOk;
 }
 
-template <typename T>
-int amg_apply(Amg& g, const T* r, T* z, const int* skip, hipStream_t s) {
+// One V-cycle, z = M r.  vt: 0 FP64 values and vectors, 1 FP32 values, 2 FP16
+// values (both with FP32 vectors and arithmetic).
+int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s) {
   if (g.L.empty()) return kNotReady;
-  return vcycle<T>(g, 0, r, z, skip, s);
+  switch (vt) {
+    case 0:
+      if (!(g.store & 1)) return kNotReady;
+      return vcycle<double, double>(g, 0, static_cast<const double*>(r), static_cast<double*>(z), skip, s);
+    case 1:
+      if (!(g.store & 2)) return kNotReady;
+      return vcycle<float, float>(g, 0, static_cast<const float*>(r), static_cast<float*>(z), skip, s);
+    case 2:
+      if (!(g.store & 4)) return kNotReady;
+      return vcycle<_Float16, float>(g, 0, static_cast<const float*>(r), static_cast<float*>(z), skip, s);
+    default:
+      return kBadMode;
+  }
 }
-template int amg_apply<float>(Amg&, const float*, float*, const int*, hipStream_t);
-template int amg_apply<double>(Amg&, const double*, double*, const int*, hipStream_t);
 
 // ------------------------------------------------------------------ PCG
 template <typename T>
@@ -215,6 +234,10 @@ struct Solver {
   double* h_sc = nullptr;
   KVec<double> k64;
   KVec<float> k32;
+  // preconditioner input/output when it runs in FP32 under an FP64 Krylov loop
+  float* rp32 = nullptr;
+  float* zp32 = nullptr;
+  long long n_p = 0;
   double* r64 = nullptr;  // FP64 residual of the refinement loop
   float* rhs32 = nullptr;
   float* d32 = nullptr;
@@ -227,6 +250,15 @@ struct Solver {
     WN_HIP(hipHostMalloc(&h_sc, sizeof(double) * kScCount, 0));
     return ws.alloc(4096);
   }
+  int ensure_p(long long n) {
+    if (n <= n_p && rp32) return kOk;
+    if (rp32) (void)hipFree(rp32);
+    if (zp32) (void)hipFree(zp32);
+    WN_HIP(hipMalloc(&rp32, sizeof(float) * std::max<long long>(n, 1)));
+    WN_HIP(hipMalloc(&zp32, sizeof(float) * std::max<long long>(n, 1)));
+    n_p = n;
+    return kOk;
+  }
   ~Solver() {
     ws.release();
     k64.release();
@@ -234,7 +266,7 @@ struct Solver {
     if (flags) (void)hipFree(flags);
     if (h_flags) (void)hipHostFree(h_flags);
     if (h_sc) (void)hipHostFree(h_sc);
-    for (void* p : {(void*)r64, (void*)rhs32, (void*)d32})
+    for (void* p : {(void*)r64, (void*)rhs32, (void*)d32, (void*)rp32, (void*)zp32})
       if (p) (void)hipFree(p);
   }
 };
@@ -248,10 +280,13 @@ __global__ void pcg_start_kernel(double* sc, int* flags, double rtol2) {
   }
 }
 
-// Preconditioner kinds: 0 none, 1 Jacobi (dinv), 2 AMG.
+// PCG on A x = b from x = 0, vectors in T.  Preconditioner kinds: 0 none,
+// 1 Jacobi (dinv in T), 2 AMG (amg_vt as amg_apply; scale S optional: the
+// preconditioner is z = S M(S r)).
 template <typename T>
-int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, const T* b, T* x, double rtol,
-            int max_iter, int check_every, const SpmvTune& tune0, hipStream_t s, int* iters, double* rel) {
+int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, int amg_vt, const double* scale,
+            const T* b, T* x, double rtol, int max_iter, int check_every, const SpmvTune& tune0, hipStream_t s,
+            int* iters, double* rel) {
   KVec<T>& k = *reinterpret_cast<KVec<T>*>(std::is_same<T, double>::value ? (void*)&S.k64 : (void*)&S.k32);
   const long long n = (long long)A.n_rows * A.br;
   int r;
@@ -259,13 +294,24 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
   if ((r = k.ensure(n))) return r;
   if ((r = S.ws.alloc(A.n_slices + 8))) return r;
   SpmvTune t = tune0;
-  t.skip = S.flags;
+  t.flags = S.flags;
+  t.guard = 1;
   if (check_every <= 0) check_every = 8;
+  // precision of the preconditioner vectors; a separate FP32 pair is needed
+  // when the V-cycle runs in FP32 under FP64 vectors, or when S is applied
+  const bool amg32 = (pkind == 2) && amg_vt != 0;
+  if (pkind == 2 && !amg32 && !std::is_same<T, double>::value) return kBadMode;
+  const bool sep32 = amg32 && (!std::is_same<T, float>::value || scale != nullptr);
+  const bool sep64 = (pkind == 2) && !amg32 && scale != nullptr;
+  if (sep32 && (r = S.ensure_p(n))) return r;
+  float* rp32 = sep32 ? S.rp32 : nullptr;
   // x = 0, r = b, rr = b.b
   if ((r = launch_fill<T>(n, x, T(0), s))) return r;
   if ((r = launch_cast_scale<T, T>(n, b, nullptr, 1.0, k.r, s))) return r;
   if ((r = launch_dot<T>(n, k.r, k.r, S.ws, S.ws.sc + kScRR, s))) return r;
   hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(64), 0, s, S.ws.sc, S.flags, rtol * rtol);
+  if (sep32 && (r = launch_cast_scale<T, float>(n, k.r, scale, 1.0, rp32, s))) return r;
+  if (sep64 && (r = launch_cast_scale<T, T>(n, k.r, scale, 1.0, k.z, s))) return r;
   int done_iters = 0;
   for (int it = 0; it < max_iter;) {
     const int chunk = std::min(check_every, max_iter - it);
@@ -273,15 +319,32 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
       const int first = (it == 0);
       if (pkind == 1) {
         if ((r = launch_jacobi_dot<T>(n, k.r, dinv, k.z, S.ws, first, S.flags, s))) return r;
+        if ((r = launch_update_p<T, T>(n, k.z, nullptr, k.p, S.ws, first, S.flags, s))) return r;
+      } else if (pkind == 2 && amg32) {
+        const float* vin = sep32 ? rp32 : reinterpret_cast<const float*>(k.r);
+        float* vout = sep32 ? S.zp32 : reinterpret_cast<float*>(k.z);
+        if ((r = amg_apply(*amg, amg_vt, vin, vout, S.flags, s))) return r;
+        if ((r = launch_rho<T, float>(n, k.r, vout, scale, S.ws, first, S.flags, s))) return r;
+        if ((r = launch_update_p<T, float>(n, vout, scale, k.p, S.ws, first, S.flags, s))) return r;
       } else if (pkind == 2) {
-        if ((r = amg_apply<T>(*amg, k.r, k.z, S.flags, s))) return r;
-        if ((r = launch_rho<T>(n, k.r, k.z, S.ws, first, S.flags, s))) return r;
+        // FP64 cycle: in-place scaled input in k.z when S is given
+        const T* vin = sep64 ? k.z : k.r;
+        T* vout = k.q;  // q is free until the SpMV below
+        if ((r = amg_apply(*amg, 0, vin, vout, S.flags, s))) return r;
+        if ((r = launch_rho<T, T>(n, k.r, vout, scale, S.ws, first, S.flags, s))) return r;
+        if ((r = launch_update_p<T, T>(n, vout, scale, k.p, S.ws, first, S.flags, s))) return r;
       } else {
-        if ((r = launch_rho<T>(n, k.r, k.r, S.ws, first, S.flags, s))) return r;
+        if ((r = launch_rho<T, T>(n, k.r, k.r, nullptr, S.ws, first, S.flags, s))) return r;
+        if ((r = launch_update_p<T, T>(n, k.r, nullptr, k.p, S.ws, first, S.flags, s))) return r;
       }
-      if ((r = launch_update_p<T>(n, pkind ? k.z : k.r, k.p, S.ws, S.flags, s))) return r;
       if ((r = spmv_dot<T>(A, k.p, k.q, S.ws, 1, t, s))) return r;
-      if ((r = launch_update_xr<T>(n, x, k.r, k.p, k.q, S.ws, S.flags, s))) return r;
+      if (sep32) {
+        if ((r = launch_update_xr<T, float>(n, x, k.r, k.p, k.q, rp32, scale, S.ws, S.flags, s))) return r;
+      } else if (sep64) {
+        if ((r = launch_update_xr<T, T>(n, x, k.r, k.p, k.q, k.z, scale, S.ws, S.flags, s))) return r;
+      } else {
+        if ((r = launch_update_xr<T, T>(n, x, k.r, k.p, k.q, nullptr, nullptr, S.ws, S.flags, s))) return r;
+      }
     }
     WN_HIP(hipMemcpyAsync(S.h_flags, S.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
     WN_HIP(hipStreamSynchronize(s));
@@ -294,18 +357,18 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
   if (rel) *rel = S.h_sc[kScRR0] > 0.0 ? std::sqrt(S.h_sc[kScRR] / S.h_sc[kScRR0]) : 0.0;
   return kOk;
 }
-template int pcg_run<double>(Solver&, const SellMatrix&, int, const double*, Amg*, const double*, double*, double, int,
-                             int, const SpmvTune&, hipStream_t, int*, double*);
-template int pcg_run<float>(Solver&, const SellMatrix&, int, const float*, Amg*, const float*, float*, double, int, int,
-                            const SpmvTune&, hipStream_t, int*, double*);
+template int pcg_run<double>(Solver&, const SellMatrix&, int, const double*, Amg*, int, const double*, const double*,
+                             double*, double, int, int, const SpmvTune&, hipStream_t, int*, double*);
+template int pcg_run<float>(Solver&, const SellMatrix&, int, const float*, Amg*, int, const double*, const float*,
+                            float*, double, int, int, const SpmvTune&, hipStream_t, int*, double*);
 
 // ------------------------------------------------------------------ refinement
 // FP64 iterative refinement around an FP32 (equilibrated) inner PCG:
 //   r = f - K u (FP64, K = A.d_v64);  rhat = s .* r;  solve Ahat dhat = rhat/|rhat|
 //   in FP32 (A.d_v32 = S K S) to relative inner_rtol;  u += |rhat| s .* dhat.
 // Stops when ||f - K u|| <= tol ||f||.  Returns total inner iterations.
-int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float* dinv32, Amg* amg, const double* f,
-           double* u, double tol, double inner_rtol, int max_outer, int max_inner, int check_every,
+int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float* dinv32, Amg* amg, int amg_vt,
+           const double* f, double* u, double tol, double inner_rtol, int max_outer, int max_inner, int check_every,
            const SpmvTune& tune, hipStream_t s, int* outer_out, int* inner_out, double* rel_out, int zero_init) {
   const long long n = (long long)A.n_rows * A.br;
   int r;
@@ -325,21 +388,22 @@ int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float
   int total_inner = 0, outer = 0;
   double rel = 0.0;
   SpmvTune t = tune;
-  t.skip = nullptr;
+  t.flags = nullptr;
+  t.guard = 0;
   for (;; ++outer) {
-    if ((r = spmv_residual64(A, u, f, S.r64, S.rhs32, scale, S.ws, S.ws.sc + kScTrue, t, s))) return r;
+    if ((r = spmv_residual64(A, u, f, S.r64, S.rhs32, scale, S.ws, S.ws.sc + kScTrue2, t, s))) return r;
     WN_HIP(hipMemcpyAsync(S.h_sc, S.ws.sc, sizeof(double) * kScCount, hipMemcpyDeviceToHost, s));
     WN_HIP(hipStreamSynchronize(s));
     const double f2 = S.h_sc[kScF2];
-    rel = f2 > 0.0 ? std::sqrt(S.h_sc[kScTrue] / f2) : 0.0;
+    rel = f2 > 0.0 ? std::sqrt(S.h_sc[kScTrue2] / f2) : 0.0;
     if (rel <= tol || outer >= max_outer || !(rel == rel)) break;
     // normalised FP32 right-hand side
     if ((r = launch_dot<float>(n, S.rhs32, S.rhs32, S.ws, S.ws.sc + kScInnerN, s))) return r;
     if ((r = launch_normalise<float>(n, S.rhs32, S.rhs32, S.ws.sc + kScInnerN, s))) return r;
     int it = 0;
     double irel = 0.0;
-    if ((r = pcg_run<float>(S, A, pkind, dinv32, amg, S.rhs32, S.d32, inner_rtol, max_inner, check_every, tune, s,
-                            &it, &irel)))
+    if ((r = pcg_run<float>(S, A, pkind, dinv32, amg, amg_vt, nullptr, S.rhs32, S.d32, inner_rtol, max_inner,
+                            check_every, tune, s, &it, &irel)))
       return r;
     total_inner += it;
     if ((r = launch_ir_update<float>(n, u, S.d32, scale, S.ws.sc + kScInnerN, s))) return r;
@@ -363,8 +427,9 @@ Workspace& solver_ws(Solver& s) {
 
 Amg* amg_new() { return new Amg(); }
 void amg_delete(Amg* g) { delete g; }
-int amg_resize(Amg& g, int n_levels) {
+int amg_resize(Amg& g, int n_levels, int store) {
   g.release();
+  g.store = store;
   for (int i = 0; i < n_levels; ++i) g.L.emplace_back(new AmgLevel());
   return kOk;
 }
@@ -383,8 +448,10 @@ int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const i
     return kBadArg;
   AmgLevel& L = *g.L[lvl];
   int r;
+  const int to64 = g.store & 1, to32 = (g.store & 6) ? 1 : 0, to16 = (g.store & 4) ? 1 : 0;
   if (fine) {
     if (fine->br != br || fine->n_rows != n_rows || !fine->d_v32) return kBadArg;
+    if ((to64 && !fine->d_v64e) || (to16 && !fine->d_v16)) return kNotReady;
     L.A = &fine->equilibrated();
     L.own_A = false;
   } else {
@@ -396,7 +463,7 @@ int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const i
     const size_t nv = (size_t)L.A->nnzb * br * br;
     WN_HIP(hipMalloc(&dv, sizeof(double) * std::max<size_t>(nv, 1)));
     WN_HIP(hipMemcpy(dv, a_vals, sizeof(double) * nv, hipMemcpyHostToDevice));
-    r = sell_from_std(*L.A, dv, 1, 1, nullptr);
+    r = sell_from_std(*L.A, dv, to64, to32, nullptr, to16);
     (void)hipDeviceSynchronize();
     (void)hipFree(dv);
     if (r) return r;
@@ -422,7 +489,7 @@ int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const i
       const size_t nv = (size_t)M.nnzb * rb * cb;
       WN_HIP(hipMalloc(&dv, sizeof(double) * std::max<size_t>(nv, 1)));
       WN_HIP(hipMemcpy(dv, vals, sizeof(double) * nv, hipMemcpyHostToDevice));
-      e = sell_from_std(M, dv, 1, 1, nullptr);
+      e = sell_from_std(M, dv, to64, to32, nullptr, to16);
       (void)hipDeviceSynchronize();
       (void)hipFree(dv);
       return e;
@@ -443,12 +510,12 @@ int amg_set_coarse(Amg& g, int n, const double* inv) {
   return kOk;
 }
 
-int amg_finalize(Amg& g, int prec_mask, const SpmvTune& t) {
+int amg_finalize(Amg& g, const SpmvTune& t) {
   g.tune = t;
   for (size_t l = 0; l + 1 < g.L.size(); ++l)
     if (!g.L[l]->A || !g.L[l]->has_PR) return kNotReady;
   if (g.L.empty() || !g.L.back()->A || g.n_coarse != g.L.back()->n) return kNotReady;
-  return amg_alloc_vectors(g, prec_mask);
+  return amg_alloc_vectors(g, g.store);
 }
 
 }  // namespace wn

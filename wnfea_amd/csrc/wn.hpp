@@ -6,28 +6,10 @@
 
 namespace wn {
 
-// Device scalar slots used by the Krylov kernels.
-enum ScIdx : int {
-  kScRho = 0,
-  kScRhoPrev = 1,
-  kScBeta = 2,
-  kScAlpha = 3,
-  kScGamma = 4,
-  kScRR = 5,      // ||r||^2 of the (inner) recursion
-  kScRR0 = 6,
-  kScTrue = 7,    // ||f - K u||^2 (FP64 true residual)
-  kScF2 = 8,      // ||f||^2
-  kScInnerN = 9,  // ||r_hat||^2 of the current inner right-hand side
-  kScTol2 = 10,   // convergence threshold on kScRR
-  kScTmp = 11,
-  kScCount = 16,
-};
-
-// Reduction scratch of one stream: per-block partials, the arrival counter of
-// the last-block finaliser, and the scalar slots.
+// Reduction scratch of one stream: per-block partials (see store_block_partials)
+// and the device scalar slots.
 struct Workspace {
   double* partials = nullptr;
-  unsigned int* counter = nullptr;
   double* sc = nullptr;
   int max_blocks = 0;
   int alloc(int max_blocks_needed);
@@ -37,7 +19,7 @@ struct Workspace {
 // Sliced-ELL block matrix owned by the library (device buffers).
 struct SellMatrix {
   int br = 3, bc = 3;
-  int n_rows = 0, n_cols = 0, n_slices = 0, sigma = 0;
+  int n_rows = 0, n_cols = 0, n_slices = 0, sigma = 0, lanes = 1;
   long long nnzb = 0, slots = 0;
   long long* d_off = nullptr;
   int* d_len = nullptr;
@@ -47,11 +29,14 @@ struct SellMatrix {
   double* d_v64 = nullptr;   // primary FP64 values (the assembled K)
   double* d_v64e = nullptr;  // optional FP64 equilibrated values S K S
   float* d_v32 = nullptr;    // FP32 values (equilibrated for the fine level)
+  _Float16* d_v16 = nullptr; // optional FP16 copy of the FP32 values (preconditioner storage)
   // host copies of the standard structure (for exports and AMG setup)
   std::vector<int> h_rowptr, h_colidx;
 
-  int build(int br_, int bc_, int n_rows_, int n_cols_, const int* rowptr, const int* colidx, int sigma_);
-  int ensure_values(int want64, int want32);
+  // lanes_ <= 0 picks the lanes per row from the shape (see sell.hip)
+  int build(int br_, int bc_, int n_rows_, int n_cols_, const int* rowptr, const int* colidx, int sigma_,
+            int lanes_ = 0);
+  int ensure_values(int want64, int want32, int want16 = 0);
   void release();
   // Non-owning alias whose FP64 values are the equilibrated copy (when present).
   SellMatrix* eq_alias = nullptr;
@@ -60,17 +45,22 @@ struct SellMatrix {
 };
 
 template <> inline SellView<double> SellMatrix::view<double>() const {
-  return SellView<double>{n_rows, n_slices, d_off, d_len, d_row, d_col, d_v64};
+  return SellView<double>{n_rows, n_slices, lanes, d_off, d_len, d_row, d_col, d_v64};
 }
 template <> inline SellView<float> SellMatrix::view<float>() const {
-  return SellView<float>{n_rows, n_slices, d_off, d_len, d_row, d_col, d_v32};
+  return SellView<float>{n_rows, n_slices, lanes, d_off, d_len, d_row, d_col, d_v32};
+}
+template <> inline SellView<_Float16> SellMatrix::view<_Float16>() const {
+  return SellView<_Float16>{n_rows, n_slices, lanes, d_off, d_len, d_row, d_col, d_v16};
 }
 
 // SpMV launch knobs (picked by the autotuner; defaults are sane).
 struct SpmvTune {
-  int waves = 1;              // wavefronts per slice (1, 2, 4)
-  int xcd_remap = 1;          // contiguous slice ranges per XCD
-  const int* skip = nullptr;  // device flag: when nonzero the launch does nothing
+  int waves = 1;               // wavefronts per slice (1, 2, 4)
+  int unroll = 1;              // blocks in flight per lane (1, 2, 4; capped by register use)
+  int xcd_remap = 1;           // contiguous slice ranges per XCD
+  const int* flags = nullptr;  // device loop flags (see guard_skip in sell.hpp)
+  int guard = 0;               // 0 always run, 1 skip once the done flag is set
 };
 
 // -------- element kernels (element.hip)
@@ -90,17 +80,27 @@ int launch_adjoint_dot(int M, const int* conn, const double* E, const double* fe
 
 // -------- vector kernels (blas.hip)
 int vec_grid(long long n);
+// one-block finalisation of block partials (FinMode in common.hpp)
+int launch_finalize(const double* partials, int nb, int K, int mode, double* out, double* sc, int* flags, int first,
+                    const int* skip, hipStream_t s);
 template <typename T> int launch_fill(long long n, T* x, T v, hipStream_t s);
 template <typename A, typename B>
 int launch_cast_scale(long long n, const A* x, const double* sc, double f, B* y, hipStream_t s);
 template <typename T>
 int launch_jacobi_dot(long long n, const T* r, const T* dinv, T* z, Workspace& w, int first, const int* skip,
                       hipStream_t s);
-template <typename T>
-int launch_rho(long long n, const T* r, const T* z, Workspace& w, int first, const int* skip, hipStream_t s);
-template <typename T> int launch_update_p(long long n, const T* z, T* p, Workspace& w, const int* skip, hipStream_t s);
-template <typename T>
-int launch_update_xr(long long n, T* x, T* r, const T* p, const T* q, Workspace& w, int* flags, hipStream_t s);
+// rho = r . (S z) (S may be null); finalise beta = rho / rho_prev
+template <typename T, typename ZT>
+int launch_rho(long long n, const T* r, const ZT* z, const double* scale, Workspace& w, int first, const int* skip,
+               hipStream_t s);
+// p = S z + beta p (p = S z when first)
+template <typename T, typename ZT>
+int launch_update_p(long long n, const ZT* z, const double* scale, T* p, Workspace& w, int first, const int* skip,
+                    hipStream_t s);
+// x += alpha p; r -= alpha q; rr; optional rp = S r; raises flags[0] (done) when rr <= sc[kScTol2]
+template <typename T, typename RT>
+int launch_update_xr(long long n, T* x, T* r, const T* p, const T* q, RT* rp, const double* scale, Workspace& w,
+                     int* flags, hipStream_t s);
 template <typename T> int launch_dot(long long n, const T* a, const T* b, Workspace& w, double* out, hipStream_t s);
 template <typename T>
 int launch_ir_update(long long n, double* u, const T* d, const double* sc_vec, const double* nrm2, hipStream_t s);
@@ -114,22 +114,29 @@ int spmv_axpby(const SellMatrix& A, const XT* x, XT* y, double alpha, double bet
 template <typename VT>
 int spmv_dot(const SellMatrix& A, const VT* p, VT* y, Workspace& w, int with_alpha, const SpmvTune& t, hipStream_t s);
 // r = b - A x in FP64, ||r||^2 -> out; optional rs = scale .* r (FP32)
+// r = b - A x in FP64; dots ||r||^2 -> out[0], ||scale r||^2 -> out[1]; optional
+// rs = scale .* r (FP32).
 int spmv_residual64(const SellMatrix& A, const double* x, const double* b, double* r, float* rs, const double* scale,
                     Workspace& w, double* out, const SpmvTune& t, hipStream_t s);
-template <typename T>
+// V-cycle kernels: VT is the stored value type (double, float or _Float16),
+// T the vector/arithmetic type (double for VT = double, float otherwise).
+template <typename VT, typename T>
 int spmv_cheb(const SellMatrix& A, const T* x_in, T* x_out, const T* b, const T* dinv, T* d, double c1, double c2,
               const SpmvTune& t, hipStream_t s);
-template <typename T>
+template <typename VT, typename T>
 int spmv_defect(const SellMatrix& A, const T* x, const T* b, T* res, const SpmvTune& t, hipStream_t s);
-template <typename T> int spmv_addto(const SellMatrix& A, const T* e, T* x, const SpmvTune& t, hipStream_t s);
+template <typename VT, typename T> int spmv_addto(const SellMatrix& A, const T* e, T* x, const SpmvTune& t,
+                                                  hipStream_t s);
+template <typename VT, typename T>
+int spmv_restrict(const SellMatrix& A, const T* x, T* y, const SpmvTune& t, hipStream_t s);
 
 // standard block layout <-> sliced layout, and assembly
-int sell_from_std(SellMatrix& A, const double* vals_std, int to64, int to32, hipStream_t s);
+int sell_from_std(SellMatrix& A, const double* vals_std, int to64, int to32, hipStream_t s, int to16 = 0);
 int sell_to_std(const SellMatrix& A, double* vals_std, hipStream_t s);
 int assemble_gather(SellMatrix& A, const int* d_rowidx, const int* d_colidx, const long long* cptr, const int* codes,
                     const double* Ke, const unsigned char* node_fix, double* vals_std, hipStream_t s);
 int extract_diag(const SellMatrix& A, const long long* diag_block, double* diag, hipStream_t s);
 int equilibrate(SellMatrix& A, const int* d_rowidx, const int* d_colidx, const double* scale, int want64e,
-                hipStream_t s);
+                hipStream_t s, int want16 = 0);
 
 }  // namespace wn

@@ -103,7 +103,12 @@ def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None
     A = sp.bsr_matrix(A)
     A.sort_indices()
     b = A.blocksize[0]
-    B = rigid_body_modes(coords, b)
+    if coords is not None:
+        B = rigid_body_modes(coords, b)
+    else:  # no geometry: the b translation modes only (constant per DOF component)
+        B = np.zeros((A.shape[0], b))
+        for c in range(b):
+            B[c::b, c] = 1.0
     if dof_scale is not None:
         B *= np.asarray(dof_scale, dtype=np.float64)[:, None]
     if fixed_mask is not None:
@@ -165,10 +170,14 @@ class SmoothedAggregationAMG:
     its FP64 equilibrated values, are the level-0 operator, shared with PCG).
     """
 
+    STORAGE_BITS = {"fp64": 1, "fp32": 2, "fp16": 4}
+    VT_CODE = {"fp64": 0, "fp32": 1, "fp16": 2}
+
     def __init__(self, fine, A_hat: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None = None,
                  dof_scale: np.ndarray | None = None, theta: float = 0.08, theta_decay: float = 1.0,
                  degree: int = 2, ratio: float = 30.0, coarse_size: int = 600, max_levels: int = 10,
-                 precisions: int = 3, sigma: int = 4096, waves: int = 1, xcd_remap: bool = True):
+                 storage: tuple = ("fp64", "fp32", "fp16"), sigma: int = 4096, waves: int = 1,
+                 xcd_remap: bool = True, unroll: int = 1):
         t0 = time.perf_counter()
         self.lib = _lib.load()
         self.levels, inv = sa_setup(A_hat, coords, fixed_mask, theta=theta, theta_decay=theta_decay,
@@ -177,7 +186,11 @@ class SmoothedAggregationAMG:
         self.degree, self.ratio = degree, ratio
         self.fine = fine
         nl = len(self.levels)
-        self.handle = self.lib.wnfea_amg_create(nl)
+        self.storage = tuple(storage)
+        mask = 0
+        for name in self.storage:
+            mask |= self.STORAGE_BITS[name]
+        self.handle = self.lib.wnfea_amg_create(nl, mask)
         if not self.handle:
             raise _lib.HipError("wnfea_amg_create failed")
         self._keep = []
@@ -213,7 +226,7 @@ class SmoothedAggregationAMG:
             _lib.check(rc, f"wnfea_amg_set_level({l})")
         inv = np.ascontiguousarray(inv, dtype=np.float64)
         _lib.check(self.lib.wnfea_amg_set_coarse(self.handle, inv.shape[0], _lib.hptr(inv)), "wnfea_amg_set_coarse")
-        _lib.check(self.lib.wnfea_amg_finalize(self.handle, int(precisions), int(waves), int(bool(xcd_remap))),
+        _lib.check(self.lib.wnfea_amg_finalize(self.handle, int(waves), int(bool(xcd_remap)), int(unroll)),
                    "wnfea_amg_finalize")
         self._keep = None
         self.setup_sec = time.perf_counter() - t0
@@ -228,14 +241,18 @@ class SmoothedAggregationAMG:
         return {"levels": out, "operator_complexity": sum(o["nnz"] for o in out) / max(nnz0, 1),
                 "setup_sec": self.setup_sec, "setup_host_sec": self.setup_host_sec}
 
-    def vcycle(self, r, z=None):
-        """z = M r on the device (FP32 or FP64 by r's dtype)."""
+    def vcycle(self, r, z=None, storage: str | None = None):
+        """z = M r on the device.  FP64 vectors run the FP64 cycle; FP32 vectors
+        the FP32 cycle, or the FP16-stored one with storage="fp16"."""
         import torch
-        prec = 1 if r.dtype == torch.float32 else 0
+        if storage is None:
+            storage = "fp32" if r.dtype == torch.float32 else "fp64"
+        if (storage == "fp64") != (r.dtype == torch.float64):
+            raise ValueError("FP64 cycles take FP64 vectors, FP32/FP16 cycles FP32 vectors")
         if z is None:
             z = torch.empty_like(r)
-        _lib.check(self.lib.wnfea_amg_vcycle(self.handle, prec, _lib.dptr(r), _lib.dptr(z), _lib.stream_of()),
-                   "wnfea_amg_vcycle")
+        _lib.check(self.lib.wnfea_amg_vcycle(self.handle, self.VT_CODE[storage], _lib.dptr(r), _lib.dptr(z),
+                                             _lib.stream_of()), "wnfea_amg_vcycle")
         return z
 
     def close(self):

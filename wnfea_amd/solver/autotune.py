@@ -1,0 +1,89 @@
+"""SpMV launch-shape autotuner with an on-disk cache.
+
+The sliced-ELL SpMV has three launch knobs (csrc/sell.hpp): wavefronts per
+slice (1, 2, 4), blocks in flight per lane (1, 2, 4) and the XCD-contiguous
+slice mapping.  ``autotune_spmv`` times every combination on the actual
+matrix with HIP events on the launch stream, keeps the fastest, applies it to
+the matrix handle and caches the choice per (device, block shape, size,
+precision).  The reference only has fixed launch heuristics
+(``wnfea/solver/fast_kernels.py:243-260``).
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+
+_DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "wnfea_amd", "autotune.json")
+
+
+def _cache_path() -> str:
+    return os.environ.get("WNFEA_AUTOTUNE_CACHE", _DEFAULT_CACHE)
+
+
+def _load_cache() -> dict:
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(cache: dict) -> None:
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only costs a re-tune next time
+
+
+def _key(K, precision: str) -> str:
+    import torch
+    name = torch.cuda.get_device_name(K.device)
+    info = K.info()
+    return f"{name}|{K.br}x{K.bc}|rows={K.n_rows}|nnzb={K.nnzb}|lanes={info['lanes']}|{precision}"
+
+
+def time_config(K, precision: str, waves: int, unroll: int, xcd: bool, reps: int = 20) -> float:
+    """Average seconds per SpMV launch for one launch shape."""
+    import torch
+    K.tune(waves, xcd, unroll)
+    dt = torch.float32 if precision == "fp32" else torch.float64
+    x = torch.ones(K.shape[1], dtype=dt, device=K.device)
+    y = torch.empty(K.shape[0], dtype=dt, device=K.device)
+    for _ in range(2):
+        K.spmv(x, y, precision=precision)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        K.spmv(x, y, precision=precision)
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / 1e3 / reps
+
+
+def autotune_spmv(K, precision: str = "fp32", reps: int = 20, use_cache: bool = True) -> dict:
+    """Pick and apply the fastest launch shape for K; returns the choice and timings."""
+    key = _key(K, precision)
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        c = cache[key]
+        K.tune(c["waves"], c["xcd_remap"], c["unroll"])
+        return dict(c, cached=True)
+    table = []
+    for waves, unroll, xcd in itertools.product((1, 2, 4), (1, 2, 4), (True, False)):
+        table.append({"waves": waves, "unroll": unroll, "xcd_remap": xcd,
+                      "seconds": time_config(K, precision, waves, unroll, xcd, reps)})
+    best = min(table, key=lambda r: r["seconds"])
+    K.tune(best["waves"], best["xcd_remap"], best["unroll"])
+    choice = {"waves": best["waves"], "unroll": best["unroll"], "xcd_remap": best["xcd_remap"],
+              "seconds": best["seconds"]}
+    if use_cache:
+        cache[key] = choice
+        _store_cache(cache)
+    return dict(choice, cached=False, table=table)

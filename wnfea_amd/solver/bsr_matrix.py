@@ -100,7 +100,7 @@ class DeviceBSR:
     """Library-owned sliced-ELL block matrix on the current device."""
 
     def __init__(self, br: int, bc: int, n_rows: int, n_cols: int, rowptr: np.ndarray, colidx: np.ndarray,
-                 sigma: int = 4096):
+                 sigma: int = 4096, lanes: int = 0):
         import torch
         self.device = _lib.require_gpu()
         self.lib = _lib.load()
@@ -114,7 +114,8 @@ class DeviceBSR:
             raise ValueError("block column index out of range")
         self.nnzb = int(self.rowptr[-1])
         self.handle = self.lib.wnfea_bsr_create(self.br, self.bc, self.n_rows, self.n_cols,
-                                                _lib.hptr(self.rowptr), _lib.hptr(self.colidx), int(sigma))
+                                                _lib.hptr(self.rowptr), _lib.hptr(self.colidx), int(sigma),
+                                                int(lanes))
         if not self.handle:
             raise _lib.HipError("wnfea_bsr_create failed")
         rows = np.repeat(np.arange(self.n_rows, dtype=np.int32), np.diff(self.rowptr))
@@ -122,9 +123,9 @@ class DeviceBSR:
         self.d_colidx = torch.as_tensor(self.colidx, device=self.device)
 
     @classmethod
-    def from_host(cls, K: BSRMatrix, which: int = 3, sigma: int = 4096) -> "DeviceBSR":
+    def from_host(cls, K: BSRMatrix, which: int = 3, sigma: int = 4096, lanes: int = 0) -> "DeviceBSR":
         import torch
-        m = cls(K.block_size, K.block_size, K.n_nodes, K.n_nodes, K.block_row_ptr, K.block_col_idx, sigma)
+        m = cls(K.block_size, K.block_size, K.n_nodes, K.n_nodes, K.block_row_ptr, K.block_col_idx, sigma, lanes)
         vals = torch.as_tensor(np.ascontiguousarray(K.values, dtype=np.float64), device=m.device)
         m.set_values(vals, which)
         return m
@@ -148,10 +149,12 @@ class DeviceBSR:
         out = (_lib._ll * 8)()
         _lib.check(self.lib.wnfea_bsr_info(self.handle, out), "wnfea_bsr_info")
         return {"n_slices": out[0], "slots": out[1], "nnzb": out[2], "sigma": out[3], "has_fp64": bool(out[4]),
-                "has_fp32": bool(out[5]), "has_fp64_equilibrated": bool(out[6])}
+                "has_fp32": bool(out[5]), "has_fp64_equilibrated": bool(out[6]), "lanes": out[7]}
 
-    def tune(self, waves: int = 1, xcd_remap: bool = True):
-        _lib.check(self.lib.wnfea_bsr_tune(self.handle, int(waves), int(bool(xcd_remap))), "wnfea_bsr_tune")
+    def tune(self, waves: int = 1, xcd_remap: bool = True, unroll: int = 1):
+        _lib.check(self.lib.wnfea_bsr_tune(self.handle, int(waves), int(bool(xcd_remap)), int(unroll)),
+                   "wnfea_bsr_tune")
+        self.tuning = {"waves": int(waves), "xcd_remap": bool(xcd_remap), "unroll": int(unroll)}
 
     def set_values(self, vals_std, which: int = 3):
         """Values in standard block layout (nnzb, br*bc) FP64 device tensor; which: 1 FP64, 2 FP32, 3 both."""

@@ -90,7 +90,7 @@ def _finish_system(K: DeviceBSR, F_host: np.ndarray, fixed_mask: np.ndarray, fix
     scale = torch.where(ad > 1e-12, 1.0 / torch.sqrt(torch.where(ad > 1e-12, ad, torch.ones_like(ad))),
                         torch.ones_like(ad)).contiguous()
     _lib.check(L.wnfea_bsr_equilibrate(K.handle, _lib.dptr(K.d_rowidx), _lib.dptr(K.d_colidx), _lib.dptr(scale),
-                                       int(want_fp64_equilibrated), stream), "wnfea_bsr_equilibrate")
+                                       int(want_fp64_equilibrated), 0, stream), "wnfea_bsr_equilibrate")
     F = torch.as_tensor(np.ascontiguousarray(F_host, dtype=np.float64), device=dev)
     return SolidSystem(K=K, F=F, fixed_mask=fixed_mask, fixed_dofs=fixed_dofs, diag=diag, scale=scale,
                        n_nodes=n_nodes, coords=coords, mesh=mesh or {}, timings=timings or {})
@@ -224,10 +224,18 @@ def _as_system(K, F, want_fp64_equilibrated=False) -> SolidSystem:
     return system_from_csr(K, F, want_fp64_equilibrated=want_fp64_equilibrated)
 
 
+_VT = {"fp64": 0, "fp32": 1, "fp16": 2}
+
+
 def pcg_device(system: SolidSystem, b=None, rtol: float = 1e-8, max_iter: int = 2000, preconditioner: str = "jacobi",
-               precision: str = "fp64", amg=None, check_every: int = 16):
-    """One device PCG on the system.  precision: fp64 (K, FP64 vectors),
-    fp64_equilibrated (S K S), fp32 (S K S in FP32, FP32 vectors, FP64 dots).
+               precision: str = "fp64", amg=None, amg_precision: str = "fp64", check_every: int = 16):
+    """One device PCG on the system.
+
+    precision (the Krylov operator and vectors): "fp64" (K, FP64 vectors),
+    "fp64_equilibrated" (S K S), "fp32" (S K S in FP32, FP32 vectors, FP64
+    dots).  With preconditioner="amg" on "fp64", the V-cycle runs on S K S as
+    z = S M(S r) in ``amg_precision`` ("fp64", "fp32" or "fp16" storage with
+    FP32 arithmetic): FP64 CG with a lower-precision AMG preconditioner.
     Returns (x (device), iterations, recursive relative residual, seconds)."""
     import torch
     L = _lib.load()
@@ -244,15 +252,19 @@ def pcg_device(system: SolidSystem, b=None, rtol: float = 1e-8, max_iter: int = 
         dinv = (1.0 / system.diag).to(dt) if prec == 0 else torch.ones(n, dtype=dt, device=dev)
     if pkind == 2 and amg is None:
         raise ValueError("preconditioner='amg' needs an amg handle")
+    vt = _VT[amg_precision] if pkind == 2 else 0
+    if pkind == 2 and prec == 1 and vt == 0:
+        raise ValueError("an FP32 Krylov loop takes an FP32 or FP16 V-cycle")
+    scale = system.scale if (pkind == 2 and prec == 0) else None
     x = torch.empty(n, dtype=dt, device=dev)
     it = _lib.ctypes.c_int(0)
     rel = _lib.ctypes.c_double(0.0)
     _sync()
     t0 = time.perf_counter()
     _lib.check(L.wnfea_pcg(_Solver.get(), system.K.handle, prec, pkind, _lib.dptr(dinv) if dinv is not None else None,
-                           amg.handle if amg is not None else None, _lib.dptr(b), _lib.dptr(x), float(rtol),
-                           int(max_iter), int(check_every), _lib.ctypes.byref(it), _lib.ctypes.byref(rel),
-                           _lib.stream_of()), "wnfea_pcg")
+                           amg.handle if amg is not None else None, vt, _lib.dptr(scale) if scale is not None else None,
+                           _lib.dptr(b), _lib.dptr(x), float(rtol), int(max_iter), int(check_every),
+                           _lib.ctypes.byref(it), _lib.ctypes.byref(rel), _lib.stream_of()), "wnfea_pcg")
     _sync()
     dt_s = time.perf_counter() - t0
     if prec != 0:
@@ -260,10 +272,45 @@ def pcg_device(system: SolidSystem, b=None, rtol: float = 1e-8, max_iter: int = 
     return x, it.value, rel.value, dt_s
 
 
-def ir_device(system: SolidSystem, tol: float = 1e-8, inner_rtol: float = 1e-3, preconditioner: str = "amg",
-              amg=None, max_outer: int = 30, max_inner: int = 500, check_every: int = 16, u=None):
-    """FP64 iterative refinement with an FP32 inner PCG.  Returns
-    (u (device FP64), outer steps, total inner iterations, ||f-Ku||/||f||, seconds)."""
+def solve_mixed(system: SolidSystem, amg, tol: float = 1e-8, amg_precision: str = "fp16", max_iter: int = 2000,
+                check_every: int = 16, margin: float = 0.5, max_restarts: int = 3):
+    """Mixed-precision solve of the resident system to ||F - K u|| <= tol ||F||.
+
+    FP64 CG on K (FP64 SpMV, FP64 alpha/beta, FP64 solution update) with the
+    SA-AMG V-cycle in ``amg_precision`` on S K S.  The stopping test of CG
+    uses its recursive residual, so the run targets margin*tol and the FP64
+    true residual is verified afterwards; if it is above tol (recursion drift)
+    CG is restarted on the true residual (a refinement step).
+    Returns (u, iterations, refinement steps, ||F-Ku||/||F||, seconds)."""
+    t0 = time.perf_counter()
+    u = None
+    iters = 0
+    steps = 0
+    b = system.F
+    rel = float("inf")
+    for steps in range(1, max_restarts + 2):
+        x, it, _, _ = pcg_device(system, b=b, rtol=margin * tol * (1.0 if u is None else
+                                                                    float(system.F.norm() / b.norm())),
+                                 max_iter=max_iter, preconditioner="amg", precision="fp64", amg=amg,
+                                 amg_precision=amg_precision, check_every=check_every)
+        iters += it
+        u = x if u is None else u + x
+        rel = true_residual(system, u)
+        if rel <= tol or steps > max_restarts:
+            break
+        b = system.F - system.K.spmv(u)
+    _sync()
+    return u, iters, steps, rel, time.perf_counter() - t0
+
+
+def ir_device(system: SolidSystem, tol: float = 1e-8, inner_rtol: float = 0.05, preconditioner: str = "amg",
+              amg=None, amg_precision: str = "fp32", max_outer: int = 30, max_inner: int = 500,
+              check_every: int = 16, u=None):
+    """FP64 iterative refinement with an FP32 inner PCG on S K S (the
+    reference's mixed_fp32 scheme, solid_solver.py:271-300, with an AMG inner
+    preconditioner).  Converges when eps_fp32 * kappa_eff of the system is well
+    below one.  Returns (u (device FP64), outer steps, total inner iterations,
+    ||F-Ku||/||F||, seconds)."""
     import torch
     L = _lib.load()
     dev = system.K.device
@@ -279,9 +326,10 @@ def ir_device(system: SolidSystem, tol: float = 1e-8, inner_rtol: float = 1e-3, 
     t0 = time.perf_counter()
     _lib.check(L.wnfea_ir(_Solver.get(), system.K.handle, _lib.dptr(system.scale), pkind,
                           _lib.dptr(dinv) if dinv is not None else None, amg.handle if amg is not None else None,
-                          _lib.dptr(system.F), _lib.dptr(u), float(tol), float(inner_rtol), int(max_outer),
-                          int(max_inner), int(check_every), int(zero), _lib.ctypes.byref(o), _lib.ctypes.byref(i),
-                          _lib.ctypes.byref(rel), _lib.stream_of()), "wnfea_ir")
+                          _VT[amg_precision] if pkind == 2 else 1, _lib.dptr(system.F), _lib.dptr(u), float(tol),
+                          float(inner_rtol), int(max_outer), int(max_inner), int(check_every), int(zero),
+                          _lib.ctypes.byref(o), _lib.ctypes.byref(i), _lib.ctypes.byref(rel), _lib.stream_of()),
+               "wnfea_ir")
     _sync()
     return u, o.value, i.value, rel.value, time.perf_counter() - t0
 
@@ -295,19 +343,24 @@ def true_residual(system: SolidSystem, u) -> float:
     return out[0] / out[1] if out[1] > 0 else 0.0
 
 
-def build_amg(system: SolidSystem, precisions: int = 3, **kw):
+def build_amg(system: SolidSystem, storage: tuple = ("fp64", "fp32", "fp16"), **kw):
+    """SA-AMG of the system (rigid-body near-null-space from the node
+    coordinates; translations only when the system carries none).  ``storage``
+    lists the value precisions the V-cycle may run with."""
     from .amg_preconditioner import SmoothedAggregationAMG
-    if system.coords is None:
-        raise ValueError("AMG needs node coordinates (rigid-body modes)")
-    if (precisions & 1) and not system.K.info()["has_fp64_equilibrated"]:
+    info = system.K.info()
+    need64 = "fp64" in storage and not info["has_fp64_equilibrated"]
+    need16 = "fp16" in storage
+    if need64 or need16:
         L = _lib.load()
         _lib.check(L.wnfea_bsr_equilibrate(system.K.handle, _lib.dptr(system.K.d_rowidx),
-                                           _lib.dptr(system.K.d_colidx), _lib.dptr(system.scale), 1,
-                                           _lib.stream_of()), "wnfea_bsr_equilibrate")
+                                           _lib.dptr(system.K.d_colidx), _lib.dptr(system.scale),
+                                           int("fp64" in storage), int(need16), _lib.stream_of()),
+                   "wnfea_bsr_equilibrate")
     A_hat = system.equilibrated_host()
     dof_scale = (1.0 / system.scale).cpu().numpy()
     amg = SmoothedAggregationAMG(system.K, A_hat, system.coords, system.fixed_mask, dof_scale=dof_scale,
-                                 precisions=precisions, **kw)
+                                 storage=storage, **kw)
     del A_hat
     return amg
 
@@ -342,13 +395,8 @@ def solve_c3d10_pcg(K, F, rtol: float = 1e-6, max_iter: int = 2000, precondition
 
 
 def _pcg_amg64(system: SolidSystem, rtol, max_iter, amg):
-    if not system.K.info()["has_fp64_equilibrated"]:
-        L = _lib.load()
-        _lib.check(L.wnfea_bsr_equilibrate(system.K.handle, _lib.dptr(system.K.d_rowidx),
-                                           _lib.dptr(system.K.d_colidx), _lib.dptr(system.scale), 1,
-                                           _lib.stream_of()), "wnfea_bsr_equilibrate")
-    return pcg_device(system, rtol=rtol, max_iter=max_iter, preconditioner="amg", precision="fp64_equilibrated",
-                      amg=amg)
+    return pcg_device(system, rtol=rtol, max_iter=max_iter, preconditioner="amg", precision="fp64", amg=amg,
+                      amg_precision="fp64")
 
 
 def solve_c3d10_system(K, F, method: str = "auto", rtol: float = 1e-6, max_iter: int = 2000,
