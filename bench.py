@@ -5,7 +5,9 @@ Workload (default): the C3D10 I-beam, 3 m long, 0.2 x 0.02 flanges, 0.012 web,
 DOF, 208 800 elements), root clamped, -50 kN shear + 5 kN m torsion at the
 tip.  One step = one solve of the resident system from u = 0 to
 ||f - K u|| / ||f|| <= 1e-8 (FP64-verified) with the mixed-precision solver:
-FP64 iterative refinement around an FP32 smoothed-aggregation AMG-PCG.
+FP64 conjugate gradients on K preconditioned by a smoothed-aggregation AMG
+V-cycle that runs in FP32 on the equilibrated operator S K S (chunks of
+iterations replayed as HIP graphs).
 
 The printed JSON line carries value = seconds per solve (lower is better),
 the same solve with the all-FP64 AMG-PCG and the speedup, the FP32 / FP64 BSR
@@ -91,7 +93,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
-    ap.add_argument("--amg-precision", choices=["fp16", "fp32"], default="fp16")
+    ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine"], default="fp32")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
     ap.add_argument("--ir", action="store_true", help="also time the restart-IR (FP32 Krylov) scheme")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
@@ -103,11 +106,12 @@ def main() -> int:
     from wnfea_amd.mesh.rcm import apply_rcm_to_model
     from wnfea_amd.mesh.solid_generator import generate_c3d10_ibeam, generate_c3d10_structured_block
     from wnfea_amd.solver.autotune import autotune_spmv
-    from wnfea_amd.solver.solid_solver import (assemble_c3d10_bsr, build_amg, ir_device, pcg_device, solve_mixed,
-                                               true_residual)
+    from wnfea_amd.solver.solid_solver import (assemble_c3d10_bsr, build_amg, ir_device, pcg_device, set_graphs,
+                                               solve_mixed, true_residual)
 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     torch.cuda.set_device(0)
+    set_graphs(not args.no_graphs)
     t0 = time.perf_counter()
     if args.config == "ibeam":
         model, meta = generate_c3d10_ibeam(n_layers=args.layers, length=3.0 * args.layers / 300)
@@ -168,14 +172,30 @@ def main() -> int:
     rel64 = true_residual(system, x64)
     log(f"[bench] fp64 AMG-PCG: {t_fp64 * 1e3:.1f} ms/solve iters {it64} rel {rel64:.2e}")
     err_u = float(torch.linalg.norm(u - x64) / torch.linalg.norm(x64))
-    other = "fp32" if args.amg_precision == "fp16" else "fp16"
-    solve_mixed(system, amg, tol=1e-8, amg_precision=other)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    uo, ito, _, relo, _ = solve_mixed(system, amg, tol=1e-8, amg_precision=other)
-    torch.cuda.synchronize()
-    t_other = time.perf_counter() - t0
-    log(f"[bench] mixed with {other} V-cycle: {t_other * 1e3:.1f} ms iters {ito} rel {relo:.2e}")
+    others = {}
+    for other in ("fp32", "fp16", "fp16_fine"):
+        if other == args.amg_precision:
+            continue
+        solve_mixed(system, amg, tol=1e-8, amg_precision=other)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        uo, ito, _, relo, _ = solve_mixed(system, amg, tol=1e-8, amg_precision=other)
+        torch.cuda.synchronize()
+        t_other = time.perf_counter() - t0
+        others[f"mixed_{other}_vcycle"] = {"seconds": round(t_other, 5), "iterations": ito, "rel_res": relo}
+        log(f"[bench] mixed with {other} V-cycle: {t_other * 1e3:.1f} ms iters {ito} rel {relo:.2e}")
+    # the same two solves with every kernel launched from the host
+    set_graphs(args.no_graphs)
+    eager = {}
+    for name, fn in (("mixed", mixed_solve), ("fp64", fp64_solve)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        eager[name] = round(time.perf_counter() - t0, 5)
+    set_graphs(not args.no_graphs)
+    log(f"[bench] {'graphs' if args.no_graphs else 'host launches'}: {eager}")
     ir = None
     if args.ir:
         _, ir_outer, ir_inner, ir_rel, ir_t = ir_device(system, tol=1e-8, amg=amg, max_outer=30)
@@ -227,7 +247,9 @@ def main() -> int:
         "speedup_mixed_vs_fp64_amg_pcg": round(t_fp64 / t_mixed, 3),
         "fp64_jacobi_pcg": jac,
         "restart_ir_fp32_krylov": ir,
-        f"mixed_{other}_vcycle": {"seconds": round(t_other, 5), "iterations": ito, "rel_res": relo},
+        **others,
+        ("graph_replay_seconds" if args.no_graphs else "host_launch_seconds"): eager,
+        "graphs": not args.no_graphs,
         "speedup_mixed_vs_fp64_jacobi_pcg": round(jac["seconds"] / t_mixed, 2) if jac else None,
         "displacement_rel_err_mixed_vs_fp64": err_u,
         "spmv_gbs": {"fp32": round(gbs32, 1), "fp64": round(gbs64, 1), "fp32_us": round(t32 * 1e6, 2),

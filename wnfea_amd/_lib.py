@@ -56,6 +56,7 @@ SIGNATURES = {
     "wnfea_amg_vcycle": (_i, [_p, _i, _p, _p, _p]),
     "wnfea_solver_create": (_p, []),
     "wnfea_solver_destroy": (None, [_p]),
+    "wnfea_solver_set_graphs": (_i, [_p, _i]),
     "wnfea_pcg": (_i, [_p, _p, _i, _i, _p, _p, _i, _p, _p, _p, _d, _i, _i, _pi, _pd, _p]),
     "wnfea_ir": (_i, [_p, _p, _p, _i, _p, _p, _i, _p, _p, _d, _d, _i, _i, _i, _i, _pi, _pi, _pd, _p]),
     "wnfea_residual": (_i, [_p, _p, _p, _p, _p, _pd, _p]),

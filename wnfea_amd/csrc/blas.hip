@@ -68,15 +68,18 @@ __global__ __launch_bounds__(kVecBlock) void rho_kernel(long long n, const T* __
   store_block_partials<1>(v, partials);
 }
 
-// p = S z + beta p  (p = S z on the first iteration)
+// p = S z + beta p  (p = S z on the first iteration).  first < 0: the first
+// iteration is the one where the loop's iteration counter skip[1] is still 0,
+// so a captured iteration can be replayed unchanged.
 template <typename T, typename ZT>
 __global__ void update_p_kernel(long long n, const ZT* __restrict__ z, const double* __restrict__ scale,
                                 T* __restrict__ p, const double* sc, int first, const int* skip) {
   if (skip && *skip) return;
   const T beta = T(sc[kScBeta]);
+  const bool f = first >= 0 ? first != 0 : skip[1] == 0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const T zi = scale ? T(scale[i] * double(z[i])) : T(z[i]);
-    p[i] = first ? zi : zi + beta * p[i];
+    p[i] = f ? zi : zi + beta * p[i];
   }
 }
 
@@ -156,7 +159,8 @@ __global__ __launch_bounds__(256) void dot_finalize_kernel(const double* __restr
     case kFinRho: {
       sc[kScRho] = v[0];
       const double prev = sc[kScRhoPrev];
-      sc[kScBeta] = first ? 0.0 : (prev != 0.0 ? v[0] / prev : 0.0);
+      const bool f = first >= 0 ? first != 0 : skip[1] == 0;
+      sc[kScBeta] = f ? 0.0 : (prev != 0.0 ? v[0] / prev : 0.0);
       sc[kScRhoPrev] = v[0];
       break;
     }

@@ -38,6 +38,7 @@ int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float
 Solver* solver_new();
 void solver_delete(Solver*);
 Workspace& solver_ws(Solver&);
+void solver_set_graphs(Solver&, int enable);
 Amg* amg_new();
 void amg_delete(Amg*);
 int amg_resize(Amg&, int n_levels, int store);
@@ -318,7 +319,8 @@ int wnfea_amg_finalize(void* h, int waves, int xcd_remap, int unroll) {
   return amg_finalize(*static_cast<Amg*>(h), t);
 }
 
-// z = M r (one V-cycle); vt 0 = FP64, 1 = FP32, 2 = FP16 values (vt >= 1: FP32 vectors)
+// z = M r (one V-cycle); vt 0 = FP64, 1 = FP32, 2 = FP16 values, 3 = FP16 on the
+// finest level and FP32 below (vt >= 1: FP32 vectors)
 int wnfea_amg_vcycle(void* h, int vt, const void* r, void* z, void* stream) {
   if (!h || !r || !z) return kBadArg;
   return amg_apply(*static_cast<Amg*>(h), vt, r, z, nullptr, S(stream));
@@ -330,15 +332,23 @@ void wnfea_solver_destroy(void* h) {
   if (h) solver_delete(static_cast<Solver*>(h));
 }
 
+// enable != 0: run full chunks of PCG iterations as a replayed HIP graph
+int wnfea_solver_set_graphs(void* h, int enable) {
+  if (!h) return kBadArg;
+  solver_set_graphs(*static_cast<Solver*>(h), enable);
+  return kOk;
+}
+
 // PCG from x = 0 until ||r|| <= rtol ||b|| (recursive residual).
 // prec: 0 FP64 primary values, 1 FP32 values (FP32 vectors, FP64 dots),
 // 2 FP64 equilibrated values.  pkind: 0 none, 1 Jacobi (dinv, same precision
-// as the vectors), 2 AMG (amg handle; amg_vt 0 FP64 / 1 FP32 / 2 FP16 cycle;
+// as the vectors), 2 AMG (amg handle; amg_vt 0 FP64 / 1 FP32 / 2 FP16 /
+// 3 FP16-finest-level cycle;
 // scale, when given, makes the preconditioner z = S M(S r)).
 int wnfea_pcg(void* solver, void* bsr, int prec, int pkind, const void* dinv, void* amg, int amg_vt,
               const double* scale, const void* b, void* x, double rtol, int max_iter, int check_every, int* iters,
               double* rel, void* stream) {
-  if (!solver || !bsr || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2 || amg_vt < 0 || amg_vt > 2)
+  if (!solver || !bsr || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2 || amg_vt < 0 || amg_vt > 3)
     return kBadArg;
   if ((pkind == 1 && !dinv) || (pkind == 2 && !amg)) return kBadArg;
   auto* B = static_cast<WnBsr*>(bsr);
@@ -362,7 +372,7 @@ int wnfea_ir(void* solver, void* bsr, const double* scale, int pkind, const floa
              const double* f, double* u, double tol, double inner_rtol, int max_outer, int max_inner,
              int check_every, int zero_init, int* outer, int* inner, double* rel, void* stream) {
   if (!solver || !bsr || !scale || !f || !u || pkind < 0 || pkind > 2) return kBadArg;
-  if ((pkind == 1 && !dinv32) || (pkind == 2 && (!amg || amg_vt < 1 || amg_vt > 2))) return kBadArg;
+  if ((pkind == 1 && !dinv32) || (pkind == 2 && (!amg || amg_vt < 1 || amg_vt > 3))) return kBadArg;
   auto* B = static_cast<WnBsr*>(bsr);
   return ir_run(*static_cast<Solver*>(solver), B->m, scale, pkind, dinv32, static_cast<Amg*>(amg), amg_vt, f, u, tol,
                 inner_rtol, max_outer, max_inner, check_every, B->tune, S(stream), outer, inner, rel, zero_init);

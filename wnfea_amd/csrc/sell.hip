@@ -154,10 +154,6 @@ int SellMatrix::ensure_values(int want64, int want32, int want16) {
   return kOk;
 }
 
-__device__ __forceinline__ long long value_base(long long slot, int bb) {
-  return slot * bb - (slot & 63) * (bb - 1);
-}
-
 __global__ void std_to_sell_kernel(long long nnzb, int bb, const long long* __restrict__ slot,
                                    const double* __restrict__ vs, double* __restrict__ v64, float* __restrict__ v32,
                                    _Float16* __restrict__ v16) {
@@ -165,11 +161,11 @@ __global__ void std_to_sell_kernel(long long nnzb, int bb, const long long* __re
   if (t >= nnzb * bb) return;
   const long long k = t / bb;
   const int e = (int)(t - k * bb);
-  const long long p = value_base(slot[k], bb) + 64LL * e;
+  const long long sl = slot[k];
   const double v = vs[t];
-  if (v64) v64[p] = v;
-  if (v32) v32[p] = float(v);
-  if (v16) v16[p] = _Float16(float(v));
+  if (v64) v64[val_index<double>(sl, e, bb)] = v;
+  if (v32) v32[val_index<float>(sl, e, bb)] = float(v);
+  if (v16) v16[val_index<_Float16>(sl, e, bb)] = _Float16(float(v));
 }
 
 __global__ void sell_to_std_kernel(long long nnzb, int bb, const long long* __restrict__ slot,
@@ -178,7 +174,7 @@ __global__ void sell_to_std_kernel(long long nnzb, int bb, const long long* __re
   if (t >= nnzb * bb) return;
   const long long k = t / bb;
   const int e = (int)(t - k * bb);
-  vs[t] = v64[value_base(slot[k], bb) + 64LL * e];
+  vs[t] = v64[val_index<double>(slot[k], e, bb)];
 }
 
 int sell_from_std(SellMatrix& A, const double* vals_std, int to64, int to32, hipStream_t s, int to16) {
@@ -225,14 +221,14 @@ __global__ void assemble_gather_kernel(long long nnzb, const int* __restrict__ r
   }
   const int i = rowidx[k], j = colidx[k];
   const unsigned fi = node_fix ? node_fix[i] : 0u, fj = node_fix ? node_fix[j] : 0u;
-  const long long base = value_base(slot[k], 9);
+  const long long sl = slot[k];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       double v = acc[r * 3 + c];
       if (((fi >> r) & 1u) || ((fj >> c) & 1u)) v = (i == j && r == c) ? 1.0 : 0.0;
-      v64[base + 64LL * (r * 3 + c)] = v;
+      v64[val_index<double>(sl, r * 3 + c, 9)] = v;
       if (vstd) vstd[k * 9 + r * 3 + c] = v;
     }
 }
@@ -256,7 +252,7 @@ __global__ void extract_diag_kernel(int n_rows, int b, const long long* __restri
   const long long i = t / b;
   const int r = (int)(t - i * b);
   const long long k = diag_block[i];
-  diag[t] = (k >= 0) ? v64[value_base(slot[k], b * b) + 64LL * (r * b + r)] : 0.0;
+  diag[t] = (k >= 0) ? v64[val_index<double>(slot[k], r * b + r, b * b)] : 0.0;
 }
 
 int extract_diag(const SellMatrix& A, const long long* diag_block, double* diag, hipStream_t s) {
@@ -278,11 +274,12 @@ __global__ void equilibrate_kernel(long long nnzb, int b, const int* __restrict_
   if (t >= nnzb * bb) return;
   const long long k = t / bb;
   const int e = (int)(t - k * bb), r = e / b, c = e - r * b;
-  const long long p = value_base(slot[k], bb) + 64LL * e;
+  const long long sl = slot[k];
+  const long long p = val_index<double>(sl, e, bb);
   const double v = scale[(long long)rowidx[k] * b + r] * v64[p] * scale[(long long)colidx[k] * b + c];
-  v32[p] = float(v);
+  v32[val_index<float>(sl, e, bb)] = float(v);
   if (v64e) v64e[p] = v;
-  if (v16) v16[p] = _Float16(float(v));
+  if (v16) v16[val_index<_Float16>(sl, e, bb)] = _Float16(float(v));
 }
 
 int equilibrate(SellMatrix& A, const int* d_rowidx, const int* d_colidx, const double* scale, int want64e,

@@ -7,12 +7,16 @@
 // windows of `sigma` consecutive rows (stable, longest first), so the rows
 // sharing a slice have nearly equal lengths.  Block j of the row in slice
 // position r goes to lane l = r*L + j%L at step t = j/L; its slot is
-// off[s] + t*64 + l, and entry k (row-major inside the BRxBC block) sits at
-// value index off[s]*BB + (t*BB + k)*64 + l.  A wavefront therefore reads
-// every block entry of a step as one contiguous 256 B (FP32) / 512 B (FP64)
-// load; each lane accumulates whole block rows in registers and the L lanes
-// of a row meet in a shuffle reduction.  No atomics.  Padding lanes have
-// zero steps and row -1.
+// off[s] + t*64 + l.  The BB entries of the 64 blocks of one step form a
+// region of 64*BB values starting at (off[s] + t*64)*BB, cut into chunks of
+// V = 16 bytes / sizeof(value) entries: chunk g of lane l holds entries
+// g*V .. g*V+V-1 (row-major inside the block) at region + g*64*V + l*V, and
+// the BB mod V entries left over sit at region + (BB/V)*64*V + l*(BB mod V).
+// A lane thus reads a block with 16-byte loads (three for an FP32 3x3
+// block), and each load instruction of a wavefront covers 1 KiB of
+// contiguous memory.  Each lane accumulates whole block rows in registers and
+// the L lanes of a row meet in a shuffle reduction.  No atomics.  Padding
+// lanes have zero steps and row -1.
 #pragma once
 #include "common.hpp"
 
@@ -29,6 +33,41 @@ struct SellView {
   const int* col;          // [slots] block column
   const T* val;            // [slots*BB]
 };
+
+// Index of entry e (row-major) of the block in `slot` in a VT value array.
+template <typename VT>
+__host__ __device__ __forceinline__ long long val_index(long long slot, int e, int bb) {
+  constexpr int V = 16 / (int)sizeof(VT);
+  const long long region = (slot & ~63LL) * bb;
+  const int l = (int)(slot & 63);
+  const int g = e / V, full = bb / V;
+  if (g < full) return region + (long long)g * 64 * V + l * V + (e - g * V);
+  return region + (long long)full * 64 * V + l * (bb - full * V) + (e - full * V);
+}
+
+template <typename VT, int N>
+struct alignas(N * sizeof(VT)) VecN {
+  VT e[N];
+};
+
+// The BB entries of this lane's block in the region of one step.
+template <typename VT, int BB>
+__device__ __forceinline__ void load_block(const VT* __restrict__ region, int lane, VT (&v)[BB]) {
+  constexpr int V = 16 / (int)sizeof(VT);
+  constexpr int FULL = BB / V, REM = BB - FULL * V;
+  static_assert((REM & (REM - 1)) == 0, "leftover entries must form a power-of-two load");
+#pragma unroll
+  for (int g = 0; g < FULL; ++g) {
+    const VecN<VT, V> c = *reinterpret_cast<const VecN<VT, V>*>(region + g * 64 * V + lane * V);
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[g * V + i] = c.e[i];
+  }
+  if constexpr (REM > 0) {
+    const VecN<VT, REM> c = *reinterpret_cast<const VecN<VT, REM>*>(region + FULL * 64 * V + lane * REM);
+#pragma unroll
+    for (int i = 0; i < REM; ++i) v[FULL * V + i] = c.e[i];
+  }
+}
 
 template <typename T> struct AccT { using type = T; };
 template <> struct AccT<_Float16> { using type = float; };  // FP16 storage, FP32 arithmetic
@@ -78,7 +117,7 @@ __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const
 #pragma unroll
     for (int r = 0; r < BR; ++r) acc[r] = AT(0);
     const int* __restrict__ colp = A.col + base + lane;
-    const VT* __restrict__ valp = A.val + base * BB + lane;
+    const VT* __restrict__ valp = A.val + base * BB;  // region of step 0
     int j = wave;
     if constexpr (U > 1) {
       for (; j + (U - 1) * W < len; j += U * W) {
@@ -87,9 +126,7 @@ __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const
         for (int u = 0; u < U; ++u) c[u] = colp[(long long)(j + u * W) * 64];
         VT v[U][BB];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-          for (int e = 0; e < BB; ++e) v[u][e] = valp[((long long)(j + u * W) * BB + e) * 64];
+        for (int u = 0; u < U; ++u) load_block<VT, BB>(valp + (long long)(j + u * W) * 64 * BB, lane, v[u]);
         XT xv[U][BC];
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -105,14 +142,15 @@ __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const
     }
     for (; j < len; j += W) {
       const int c = colp[(long long)j * 64];
-      const VT* v = valp + (long long)j * BB * 64;
+      VT v[BB];
+      load_block<VT, BB>(valp + (long long)j * 64 * BB, lane, v);
       XT xv[BC];
 #pragma unroll
       for (int q = 0; q < BC; ++q) xv[q] = x[(long long)c * BC + q];
 #pragma unroll
       for (int r = 0; r < BR; ++r)
 #pragma unroll
-        for (int q = 0; q < BC; ++q) acc[r] = fma(AT(v[(r * BC + q) * 64]), AT(xv[q]), acc[r]);
+        for (int q = 0; q < BC; ++q) acc[r] = fma(AT(v[r * BC + q]), AT(xv[q]), acc[r]);
     }
     if constexpr (W > 1) {
       __shared__ AT red[W - 1][BR][64];

@@ -20,6 +20,7 @@
 // a diagonally scaled operator: z = S M(S r); the precision conversions and
 // the scaling are fused into the rho, p-update and x/r-update kernels.
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <type_traits>
 #include <utility>
@@ -134,7 +135,8 @@ static ChebCoef cheb_coefs(double lmax, double ratio, int deg) {
   return c;
 }
 
-template <typename VT, typename T>
+// VT: values of this level; VTC: values of the levels below it.
+template <typename VT, typename VTC, typename T>
 static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t s) {
   AmgLevel& L = *g.L[l];
   SpmvTune t = g.tune;
@@ -174,7 +176,7 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   AmgLevel& N = *g.L[l + 1];
   T** nv = level_vecs<T>(N);
   if ((r = spmv_restrict<VT, T>(L.R, res, nv[0], t, s))) return r;
-  if ((r = vcycle<VT, T>(g, l + 1, nv[0], nv[1], skip, s))) return r;
+  if ((r = vcycle<VTC, VTC, T>(g, l + 1, nv[0], nv[1], skip, s))) return r;
   if ((r = spmv_addto<VT, T>(L.P, nv[1], cur, t, s))) return r;  // cur == post here
   oth = (cur == x) ? tmp : x;
   for (int k = 0; k < deg; ++k) {
@@ -185,19 +187,25 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
 }
 
 // One V-cycle, z = M r.  vt: 0 FP64 values and vectors, 1 FP32 values, 2 FP16
-// values (both with FP32 vectors and arithmetic).
+// values on every level, 3 FP16 values on the finest level and FP32 below it
+// (vt >= 1: FP32 vectors and arithmetic).
 int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s) {
   if (g.L.empty()) return kNotReady;
+  const float* r32 = static_cast<const float*>(r);
+  float* z32 = static_cast<float*>(z);
   switch (vt) {
     case 0:
       if (!(g.store & 1)) return kNotReady;
-      return vcycle<double, double>(g, 0, static_cast<const double*>(r), static_cast<double*>(z), skip, s);
+      return vcycle<double, double, double>(g, 0, static_cast<const double*>(r), static_cast<double*>(z), skip, s);
     case 1:
       if (!(g.store & 2)) return kNotReady;
-      return vcycle<float, float>(g, 0, static_cast<const float*>(r), static_cast<float*>(z), skip, s);
+      return vcycle<float, float, float>(g, 0, r32, z32, skip, s);
     case 2:
       if (!(g.store & 4)) return kNotReady;
-      return vcycle<_Float16, float>(g, 0, static_cast<const float*>(r), static_cast<float*>(z), skip, s);
+      return vcycle<_Float16, _Float16, float>(g, 0, r32, z32, skip, s);
+    case 3:
+      if ((g.store & 6) != 6) return kNotReady;
+      return vcycle<_Float16, float, float>(g, 0, r32, z32, skip, s);
     default:
       return kBadMode;
   }
@@ -242,6 +250,18 @@ struct Solver {
   float* rhs32 = nullptr;
   float* d32 = nullptr;
   long long n_ir = 0;
+  // Replayable chunk of PCG iterations (HIP graph) and the key of everything
+  // its kernel arguments were taken from: a solve with the same key replays
+  // it, any other solve captures a new one.
+  int use_graphs = 1;
+  hipStream_t cap = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gkey;
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+    gkey.clear();
+  }
   int init() {
     if (flags) return kOk;
     WN_HIP(hipMalloc(&flags, sizeof(int) * 4));
@@ -260,6 +280,8 @@ struct Solver {
     return kOk;
   }
   ~Solver() {
+    drop_graph();
+    if (cap) (void)hipStreamDestroy(cap);
     ws.release();
     k64.release();
     k32.release();
@@ -278,6 +300,75 @@ __global__ void pcg_start_kernel(double* sc, int* flags, double rtol2) {
     flags[0] = (sc[kScRR] == 0.0) ? 1 : 0;
     flags[1] = 0;
   }
+}
+
+// ------------------------------------------------------------------ graph keys
+template <typename P>
+static void key_push(std::vector<uint64_t>& k, P v) {
+  uint64_t u = 0;
+  static_assert(sizeof(P) <= sizeof(u), "key item too wide");
+  std::memcpy(&u, &v, sizeof(P));
+  k.push_back(u);
+}
+
+static void key_matrix(std::vector<uint64_t>& k, const SellMatrix& A) {
+  for (int v : {A.br, A.bc, A.n_rows, A.n_cols, A.n_slices, A.sigma, A.lanes}) key_push(k, v);
+  key_push(k, A.slots);
+  for (const void* p : {(const void*)A.d_off, (const void*)A.d_len, (const void*)A.d_row, (const void*)A.d_col,
+                        (const void*)A.d_v64, (const void*)A.d_v64e, (const void*)A.d_v32, (const void*)A.d_v16})
+    key_push(k, p);
+}
+
+static void key_tune(std::vector<uint64_t>& k, const SpmvTune& t) {
+  for (int v : {t.waves, t.unroll, t.xcd_remap, t.guard}) key_push(k, v);
+  key_push(k, t.flags);
+}
+
+static void key_amg(std::vector<uint64_t>& k, const Amg* g) {
+  key_push(k, g);
+  if (!g) return;
+  for (int v : {g->store, g->n_coarse, (int)g->L.size()}) key_push(k, v);
+  key_push(k, g->inv);
+  key_tune(k, g->tune);
+  for (const auto& lp : g->L) {
+    const AmgLevel& l = *lp;
+    for (int v : {l.n, l.degree, (int)l.has_PR}) key_push(k, v);
+    key_push(k, l.lmax);
+    key_push(k, l.ratio);
+    key_push(k, l.dinv64);
+    key_push(k, l.dinv32);
+    for (auto* p : l.v64) key_push(k, p);
+    for (auto* p : l.v32) key_push(k, p);
+    if (l.A) key_matrix(k, *l.A);
+    if (l.has_PR) {
+      key_matrix(k, l.P);
+      key_matrix(k, l.R);
+    }
+  }
+}
+
+// Capture `enqueue(stream)` into S.gexec unless a graph with this key exists.
+template <typename F>
+static int graph_for(Solver& S, const std::vector<uint64_t>& key, F&& enqueue) {
+  if (S.gexec && S.gkey == key) return kOk;
+  S.drop_graph();
+  if (!S.cap) WN_HIP(hipStreamCreateWithFlags(&S.cap, hipStreamNonBlocking));
+  WN_HIP(hipStreamBeginCapture(S.cap, hipStreamCaptureModeThreadLocal));
+  const int r = enqueue(S.cap);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(S.cap, &graph);
+  if (r || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r ? r : static_cast<int>(e);
+  }
+  const hipError_t ei = hipGraphInstantiate(&S.gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    S.gexec = nullptr;
+    return static_cast<int>(ei);
+  }
+  S.gkey = key;
+  return kOk;
 }
 
 // PCG on A x = b from x = 0, vectors in T.  Preconditioner kinds: 0 none,
@@ -312,40 +403,67 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
   hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(64), 0, s, S.ws.sc, S.flags, rtol * rtol);
   if (sep32 && (r = launch_cast_scale<T, float>(n, k.r, scale, 1.0, rp32, s))) return r;
   if (sep64 && (r = launch_cast_scale<T, T>(n, k.r, scale, 1.0, k.z, s))) return r;
+  // One iteration; the first-iteration switch is read on the device from the
+  // iteration counter, so every iteration enqueues the same kernels.
+  auto iteration = [&](hipStream_t st) -> int {
+    int e;
+    const int first = -1;
+    if (pkind == 1) {
+      if ((e = launch_jacobi_dot<T>(n, k.r, dinv, k.z, S.ws, first, S.flags, st))) return e;
+      if ((e = launch_update_p<T, T>(n, k.z, nullptr, k.p, S.ws, first, S.flags, st))) return e;
+    } else if (pkind == 2 && amg32) {
+      const float* vin = sep32 ? rp32 : reinterpret_cast<const float*>(k.r);
+      float* vout = sep32 ? S.zp32 : reinterpret_cast<float*>(k.z);
+      if ((e = amg_apply(*amg, amg_vt, vin, vout, S.flags, st))) return e;
+      if ((e = launch_rho<T, float>(n, k.r, vout, scale, S.ws, first, S.flags, st))) return e;
+      if ((e = launch_update_p<T, float>(n, vout, scale, k.p, S.ws, first, S.flags, st))) return e;
+    } else if (pkind == 2) {
+      // FP64 cycle: in-place scaled input in k.z when S is given
+      const T* vin = sep64 ? k.z : k.r;
+      T* vout = k.q;  // q is free until the SpMV below
+      if ((e = amg_apply(*amg, 0, vin, vout, S.flags, st))) return e;
+      if ((e = launch_rho<T, T>(n, k.r, vout, scale, S.ws, first, S.flags, st))) return e;
+      if ((e = launch_update_p<T, T>(n, vout, scale, k.p, S.ws, first, S.flags, st))) return e;
+    } else {
+      if ((e = launch_rho<T, T>(n, k.r, k.r, nullptr, S.ws, first, S.flags, st))) return e;
+      if ((e = launch_update_p<T, T>(n, k.r, nullptr, k.p, S.ws, first, S.flags, st))) return e;
+    }
+    if ((e = spmv_dot<T>(A, k.p, k.q, S.ws, 1, t, st))) return e;
+    if (sep32) return launch_update_xr<T, float>(n, x, k.r, k.p, k.q, rp32, scale, S.ws, S.flags, st);
+    if (sep64) return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, k.z, scale, S.ws, S.flags, st);
+    return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, nullptr, nullptr, S.ws, S.flags, st);
+  };
+  std::vector<uint64_t> key;
+  if (S.use_graphs) {
+    key_push(key, (int)sizeof(T));
+    for (int v : {pkind, amg_vt, check_every}) key_push(key, v);
+    for (const void* p : {(const void*)dinv, (const void*)scale, (const void*)x, (const void*)k.r, (const void*)k.z,
+                          (const void*)k.p, (const void*)k.q, (const void*)rp32, (const void*)S.zp32,
+                          (const void*)S.ws.partials, (const void*)S.ws.sc, (const void*)S.flags})
+      key_push(key, p);
+    key_push(key, n);
+    key_matrix(key, A);
+    key_tune(key, t);
+    key_amg(key, pkind == 2 ? amg : nullptr);
+  }
   int done_iters = 0;
   for (int it = 0; it < max_iter;) {
     const int chunk = std::min(check_every, max_iter - it);
-    for (int c = 0; c < chunk; ++c, ++it) {
-      const int first = (it == 0);
-      if (pkind == 1) {
-        if ((r = launch_jacobi_dot<T>(n, k.r, dinv, k.z, S.ws, first, S.flags, s))) return r;
-        if ((r = launch_update_p<T, T>(n, k.z, nullptr, k.p, S.ws, first, S.flags, s))) return r;
-      } else if (pkind == 2 && amg32) {
-        const float* vin = sep32 ? rp32 : reinterpret_cast<const float*>(k.r);
-        float* vout = sep32 ? S.zp32 : reinterpret_cast<float*>(k.z);
-        if ((r = amg_apply(*amg, amg_vt, vin, vout, S.flags, s))) return r;
-        if ((r = launch_rho<T, float>(n, k.r, vout, scale, S.ws, first, S.flags, s))) return r;
-        if ((r = launch_update_p<T, float>(n, vout, scale, k.p, S.ws, first, S.flags, s))) return r;
-      } else if (pkind == 2) {
-        // FP64 cycle: in-place scaled input in k.z when S is given
-        const T* vin = sep64 ? k.z : k.r;
-        T* vout = k.q;  // q is free until the SpMV below
-        if ((r = amg_apply(*amg, 0, vin, vout, S.flags, s))) return r;
-        if ((r = launch_rho<T, T>(n, k.r, vout, scale, S.ws, first, S.flags, s))) return r;
-        if ((r = launch_update_p<T, T>(n, vout, scale, k.p, S.ws, first, S.flags, s))) return r;
-      } else {
-        if ((r = launch_rho<T, T>(n, k.r, k.r, nullptr, S.ws, first, S.flags, s))) return r;
-        if ((r = launch_update_p<T, T>(n, k.r, nullptr, k.p, S.ws, first, S.flags, s))) return r;
-      }
-      if ((r = spmv_dot<T>(A, k.p, k.q, S.ws, 1, t, s))) return r;
-      if (sep32) {
-        if ((r = launch_update_xr<T, float>(n, x, k.r, k.p, k.q, rp32, scale, S.ws, S.flags, s))) return r;
-      } else if (sep64) {
-        if ((r = launch_update_xr<T, T>(n, x, k.r, k.p, k.q, k.z, scale, S.ws, S.flags, s))) return r;
-      } else {
-        if ((r = launch_update_xr<T, T>(n, x, k.r, k.p, k.q, nullptr, nullptr, S.ws, S.flags, s))) return r;
-      }
+    if (S.use_graphs && chunk == check_every) {
+      r = graph_for(S, key, [&](hipStream_t st) -> int {
+        for (int c = 0; c < chunk; ++c) {
+          const int e = iteration(st);
+          if (e) return e;
+        }
+        return kOk;
+      });
+      if (r) return r;
+      WN_HIP(hipGraphLaunch(S.gexec, s));
+    } else {
+      for (int c = 0; c < chunk; ++c)
+        if ((r = iteration(s))) return r;
     }
+    it += chunk;
     WN_HIP(hipMemcpyAsync(S.h_flags, S.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
     WN_HIP(hipStreamSynchronize(s));
     done_iters = S.h_flags[1];
@@ -419,6 +537,10 @@ int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float
 namespace wn {
 
 Solver* solver_new() { return new Solver(); }
+void solver_set_graphs(Solver& s, int enable) {
+  s.use_graphs = enable ? 1 : 0;
+  if (!enable) s.drop_graph();
+}
 void solver_delete(Solver* s) { delete s; }
 Workspace& solver_ws(Solver& s) {
   (void)s.init();
