@@ -153,11 +153,13 @@ int wnfea_bsr_assemble(void *h, const int *rowidx, const int *colidx, const long
 /* Scalar diagonal (device) given diag_block[i] = standard index of block (i,i). */
 int wnfea_bsr_diag(void *h, const long long *diag_block, double *diag, void *stream);
 
-/* FP32 (and optionally FP64) values of S K S with S = diag(scale) (device).
+/* FP32 values of S K S with S = diag(scale) (device), plus the FP64 copy
+ * (want64e) and an FP16 copy (want16) for lower-precision preconditioning.
  * Replaces the equilibration of BlockBeamAMGPreconditioner
- * (wnfea/solver/amg_preconditioner.py:180-185). */
+ * (wnfea/solver/amg_preconditioner.py:180-185) and its FP16 value copies
+ * (wnfea/solver/native/hip_kernels.cpp:1924-2060, precision_mode 2). */
 int wnfea_bsr_equilibrate(void *h, const int *rowidx, const int *colidx, const double *scale, int want64e,
-                          void *stream);
+                          int want16, void *stream);
 
 /* y = alpha A x + beta y; prec 0 FP64, 1 FP32 (FP32 vectors), 2 FP64
  * equilibrated.  Replaces hip_spmv_bsr6x6_fp32 / _fp64
@@ -174,7 +176,9 @@ int wnfea_bsr_spmv_dot(void *h, int prec, const void *x, void *y, double *out, v
 /* Replaces hip_amg_create / hip_amg_set_level / hip_amg_set_coarse_cholesky /
  * hip_amg_apply_vcycle / hip_amg_destroy
  * (wnfea/solver/native/hip_kernels.cpp:1891,1924,2061,2519,2541). */
-void *wnfea_amg_create(int n_levels);
+/* store: value precisions kept on every level, bit0 FP64, bit1 FP32, bit2
+ * FP16 (the reference's precision_mode 0/1/2). */
+void *wnfea_amg_create(int n_levels, int store);
 void wnfea_amg_destroy(void *h);
 
 /* Level data (host arrays).  Level 0 passes fine_bsr (its FP32 / FP64
@@ -191,29 +195,44 @@ int wnfea_amg_set_level(void *h, int lvl, void *fine_bsr, int br, int n_rows, co
 /* Coarsest-level dense inverse (host, n x n row-major FP64). */
 int wnfea_amg_set_coarse(void *h, int n, const double *inv);
 
-/* Allocate cycle vectors (prec_mask bit0 FP64, bit1 FP32) and set the SpMV launch shape. */
-int wnfea_amg_finalize(void *h, int prec_mask, int waves, int xcd_remap, int unroll);
+/* Allocate the cycle vectors of the stored precisions and set the SpMV
+ * launch shape (as wnfea_bsr_tune). */
+int wnfea_amg_finalize(void *h, int waves, int xcd_remap, int unroll);
 
-/* z = M r, one V-cycle (device vectors); prec 0 FP64, 1 FP32. */
-int wnfea_amg_vcycle(void *h, int prec, const void *r, void *z, void *stream);
+/* z = M r, one V-cycle (device vectors).  vt: 0 FP64 values and vectors;
+ * 1 FP32 values, 2 FP16 values on every level, 3 FP16 values on the finest
+ * level and FP32 below (vt >= 1: FP32 vectors and arithmetic). */
+int wnfea_amg_vcycle(void *h, int vt, const void *r, void *z, void *stream);
 
 /* ---------------------------------------------------------------- solvers */
 
+/* Solver workspace (Krylov vectors, reduction buffers, the replayable
+ * iteration graph).  Replaces the per-handle scratch of hip_amg_solve_pcg
+ * (wnfea/solver/native/hip_kernels.cpp:2384-2517). */
 void *wnfea_solver_create(void);
 void wnfea_solver_destroy(void *h);
 
-/* PCG from x = 0 until ||r|| <= rtol ||b||; prec as wnfea_bsr_spmv; pkind 0
- * none, 1 Jacobi (dinv in the vectors' precision), 2 AMG.  Iterations run in
- * device-scalar chunks of check_every between host checks.  Replaces
- * hip_amg_solve_pcg(_device) (wnfea/solver/native/hip_kernels.cpp:2384-2517)
- * and solve_c3d10_pcg's FP64 branch (wnfea/solver/solid_solver.py:302-308). */
-int wnfea_pcg(void *solver, void *bsr, int prec, int pkind, const void *dinv, void *amg, const void *b, void *x,
-              double rtol, int max_iter, int check_every, int *iters, double *rel, void *stream);
+/* enable != 0 (default): full chunks of check_every PCG iterations are
+ * captured once as a HIP graph and replayed (bitwise the same kernels as host
+ * launches).  No reference counterpart (its loop launches every kernel). */
+int wnfea_solver_set_graphs(void *h, int enable);
+
+/* PCG from x = 0 until ||r|| <= rtol ||b|| (recursive residual); prec as
+ * wnfea_bsr_spmv; pkind 0 none, 1 Jacobi (dinv in the vectors' precision),
+ * 2 AMG (amg_vt as wnfea_amg_vcycle; with scale = S the preconditioner is
+ * z = S M(S r), M the V-cycle of S K S, so an FP64 loop on K takes an FP32 or
+ * FP16 cycle).  Iterations run in device-scalar chunks of check_every between
+ * host checks.  Replaces hip_amg_solve_pcg(_device)
+ * (wnfea/solver/native/hip_kernels.cpp:2384-2517) and solve_c3d10_pcg's FP64
+ * branch (wnfea/solver/solid_solver.py:302-308). */
+int wnfea_pcg(void *solver, void *bsr, int prec, int pkind, const void *dinv, void *amg, int amg_vt,
+              const double *scale, const void *b, void *x, double rtol, int max_iter, int check_every, int *iters,
+              double *rel, void *stream);
 
 /* FP64 iterative refinement: r = f - K u (FP64), inner FP32 PCG on S K S to
  * inner_rtol, u += S d, until ||f - K u|| <= tol ||f||.  Replaces
  * solve_c3d10_pcg's mixed_fp32 branch (wnfea/solver/solid_solver.py:271-300). */
-int wnfea_ir(void *solver, void *bsr, const double *scale, int pkind, const float *dinv32, void *amg,
+int wnfea_ir(void *solver, void *bsr, const double *scale, int pkind, const float *dinv32, void *amg, int amg_vt,
              const double *f, double *u, double tol, double inner_rtol, int max_outer, int max_inner,
              int check_every, int zero_init, int *outer, int *inner, double *rel, void *stream);
 

@@ -1,0 +1,220 @@
+"""Host side of the library without a GPU: the C-ABI loads and exports every
+declared entry point, and the host parts of the path (mesh generators, node
+graph, RCM, block conversions, smoothed-aggregation setup) match the golden
+vectors, scipy and the reference's rules.  No device call is made here."""
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _golden_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(GOLDEN, "make_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+G = _golden_module()
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "wnfea_hip.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(wnfea_\w+)\s*\(", txt)))
+
+
+# ---------------------------------------------------------------- C-ABI
+def test_library_exports_every_declared_symbol():
+    from wnfea_amd import _lib
+    L = _lib.load()
+    declared = _declared()
+    assert len(declared) >= 30
+    for name in declared:
+        assert hasattr(L, name), name
+    # the ctypes signature table covers exactly the declared entry points
+    assert sorted(_lib.SIGNATURES) == declared
+    assert L.wnfea_abi_version() == 1
+
+
+def test_header_arity_matches_ctypes_table():
+    from wnfea_amd import _lib
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "wnfea_hip.h")).read(), flags=re.S)
+    for name, (_, args) in _lib.SIGNATURES.items():
+        m = re.search(r"\b%s\s*\(([^)]*)\)" % name, txt)
+        assert m, name
+        params = [p for p in m.group(1).split(",") if p.strip() and p.strip() != "void"]
+        assert len(params) == len(args), (name, len(params), len(args))
+
+
+# ---------------------------------------------------------------- meshes
+def test_c0_generator_matches_config():
+    model = G.c0_model()
+    assert len(model.mesh_nodes) == 6561 and len(model.solid_elements) == 3840
+    assert model.solid_elements.dtype == np.int32 and model.solid_elements.shape[1] == 10
+    X = model.mesh_nodes[model.solid_elements]
+    # midside nodes at the edge midpoints (Abaqus order m01 m12 m20 m03 m13 m23)
+    for m, (a, b) in enumerate(G.EDGES):
+        assert np.abs(X[:, 4 + m] - 0.5 * (X[:, a] + X[:, b])).max() < 1e-15
+    # every lattice point of the 81x9x9 half grid is a node exactly once
+    q = np.round(model.mesh_nodes / np.array([1.0 / 80, 0.1 / 8, 0.1 / 8])).astype(int)
+    assert len({tuple(r) for r in q}) == 6561
+
+
+def test_ibeam_loads_resultants():
+    """Tip shear splits equally; the torsion couple has zero resultant force
+    and moment T about the section centroid."""
+    from wnfea_amd.mesh.solid_generator import generate_c3d10_ibeam
+    model, meta = generate_c3d10_ibeam(n_layers=4, length=0.04, shear_total=-50e3, torque=5e3)
+    F = model.load_vector(3).reshape(-1, 3)
+    tip = meta["tip_nodes"]
+    assert abs(F[:, 2].sum() + 50e3) < 1e-6 and abs(F[:, 1].sum()) < 1e-6 and abs(F[:, 0].sum()) == 0.0
+    ry = model.mesh_nodes[tip, 1]
+    rz = model.mesh_nodes[tip, 2] - 0.15
+    Mx = np.sum(ry * F[tip, 2] - rz * F[tip, 1])
+    assert abs(Mx - 5e3) < 1e-6
+    assert meta["section_quads"] == 116
+    assert model.fixed_dof_mask(3).sum() == 3 * meta["root_nodes_fixed"]
+    X = model.mesh_nodes[model.solid_elements]
+    vol = np.einsum("mi,mi->m", np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]), X[:, 3] - X[:, 0]) / 6
+    assert np.all(vol > 0)
+
+
+def test_ibeam_full_size_counts():
+    """The 300-layer I-beam: 351 585 nodes, 208 800 tets, 1 054 755 DOF."""
+    from wnfea_amd.mesh.solid_generator import generate_c3d10_ibeam
+    _, meta = generate_c3d10_ibeam()
+    assert (meta["n_nodes"], meta["n_elements"], meta["n_dofs"]) == (351585, 208800, 1054755)
+
+
+# ---------------------------------------------------------------- graph and RCM
+def test_node_graph_matches_reference_construction():
+    from wnfea_amd.mesh.rcm import compute_node_adjacency_matrix
+    model = G.c0_model()
+    A = compute_node_adjacency_matrix(len(model.mesh_nodes), None, model.solid_elements)
+    R = G.node_graph(len(model.mesh_nodes), model.solid_elements)
+    R.sort_indices()
+    assert np.array_equal(A.indptr, R.indptr) and np.array_equal(A.indices, R.indices)
+    assert A.dtype == np.int8 and np.all(A.data == 1)
+
+
+def test_rcm_matches_golden_and_scipy():
+    from wnfea_amd.mesh.rcm import compute_node_adjacency_matrix, compute_rcm_order
+    model = G.c0_model()
+    A = compute_node_adjacency_matrix(len(model.mesh_nodes), None, model.solid_elements)
+    n2o, o2n = compute_rcm_order(A)
+    assert np.array_equal(n2o, np.load(os.path.join(GOLDEN, "c0_rcm_new_to_old.npy")))
+    assert np.array_equal(o2n[n2o], np.arange(len(n2o)))
+    rng = np.random.default_rng(11)
+    for t in range(40):
+        n = int(rng.integers(1, 150))
+        B = sp.random(n, n, density=float(rng.uniform(0.0, 0.1)), random_state=100 + t, format="csr")
+        B = sp.csr_matrix(((B + B.T + sp.identity(n)) > 0).astype(np.int8))
+        B.sort_indices()
+        assert np.array_equal(compute_rcm_order(B)[0], csg.reverse_cuthill_mckee(B, symmetric_mode=True)), t
+
+
+def test_apply_rcm_roundtrip_and_bandwidth():
+    """tests/test_bsr_rcm.py:61-91: bandwidth does not grow; the permuted
+    solution maps back exactly; supports and loads follow their nodes."""
+    from oracle import oracle as O
+    from wnfea_amd.mesh.rcm import apply_rcm_to_model, compute_matrix_bandwidth, permute_solution_to_original
+    model, _ = G.generate_c3d10_structured_block(0.5, 0.1, 0.1, 20, 3, 3, tets_per_hex=6, load_direction="z")
+    mr, n2o, o2n = apply_rcm_to_model(model)
+    bw0 = compute_matrix_bandwidth(O.node_pattern(len(model.mesh_nodes), model.solid_elements))
+    bw1 = compute_matrix_bandwidth(O.node_pattern(len(mr.mesh_nodes), mr.solid_elements))
+    assert bw1 <= bw0
+    for dpn in (3, 6):
+        u = np.linspace(1.0, 100.0, len(n2o) * dpn)
+        up = u[(n2o[:, None] * dpn + np.arange(dpn)).ravel()]
+        assert np.array_equal(permute_solution_to_original(up, n2o, dpn), u)
+    assert np.array_equal(mr.load_vector(3).reshape(-1, 3), model.load_vector(3).reshape(-1, 3)[n2o])
+    assert np.array_equal(mr.fixed_dof_mask(3).reshape(-1, 3), model.fixed_dof_mask(3).reshape(-1, 3)[n2o])
+
+
+# ---------------------------------------------------------------- block matrices
+@pytest.mark.parametrize("b", [3, 6])
+def test_csr_bsr_roundtrip_and_oracle_pattern(b):
+    """tests/test_bsr_rcm.py:94-130 (round trip < 1e-15) and the block pattern
+    of the oracle's restatement of bsr_matrix.py:49-122."""
+    from oracle import oracle as O
+    from wnfea_amd.solver.bsr_matrix import bsr_to_csr, csr_to_bsr
+    model, _ = G.generate_c3d10_structured_block(0.3, 0.1, 0.1, 6, 2, 2, tets_per_hex=6, load_direction="z")
+    E, nu = model.element_moduli()
+    K, _ = O.assemble_csr(model.mesh_nodes, model.solid_elements, E, nu, model.fixed_dof_mask(3),
+                          model.load_vector(3))
+    n = K.shape[0]
+    if n % b:
+        K = sp.block_diag([K, sp.identity(b - n % b)]).tocsr()
+    B = csr_to_bsr(K, b, dtype=np.float64)
+    rp, ci, vals = O.csr_to_bsr(K, b)
+    assert np.array_equal(np.asarray(B.block_row_ptr), rp) and np.array_equal(np.asarray(B.block_col_idx), ci)
+    assert np.array_equal(np.asarray(B.values).reshape(len(ci), -1), vals)
+    K2 = bsr_to_csr(B)
+    assert sp.linalg.norm(K2 - K) / sp.linalg.norm(K) < 1e-15
+    assert B.nnz_blocks == len(ci) and B.block_size == b and B.shape == K.shape
+
+
+# ---------------------------------------------------------------- smoothed aggregation (host setup)
+def test_sa_setup_invariants():
+    """Aggregation covers every free node; R = P^T; coarse operators are the
+    symmetric Galerkin products R A P; the coarsest is positive definite and
+    its stored inverse is its inverse; P reproduces the coarse rigid-body
+    modes on the tentative level."""
+    from oracle import oracle as O
+    from wnfea_amd.solver.amg_preconditioner import sa_setup
+    model, _ = G.generate_c3d10_structured_block(1.0, 0.1, 0.1, 24, 3, 3, tets_per_hex=6, load_direction="z")
+    E, nu = model.element_moduli()
+    fixed = model.fixed_dof_mask(3)
+    K, _ = O.assemble_csr(model.mesh_nodes, model.solid_elements, E, nu, fixed, model.load_vector(3))
+    d = K.diagonal()
+    s = 1.0 / np.sqrt(np.abs(d))
+    A = (sp.diags(s) @ K @ sp.diags(s)).tobsr(blocksize=(3, 3))
+    levels, inv = sa_setup(A, model.mesh_nodes, fixed, coarse_size=300, dof_scale=np.sqrt(np.abs(d)))
+    assert len(levels) >= 2
+    for lv, nxt in zip(levels[:-1], levels[1:]):
+        assert nxt.A.shape[0] < lv.A.shape[0]
+        assert abs(lv.R - lv.P.T).max() == 0.0
+        Ac = (lv.R @ lv.A @ lv.P).toarray()
+        assert np.abs(nxt.A.toarray() - 0.5 * (Ac + Ac.T)).max() <= 1e-12 * np.abs(Ac).max()
+        # the Chebyshev bound covers the spectrum of D^-1 A, without a large margin
+        dh = np.sqrt(lv.dinv)
+        lam = sp.linalg.eigsh(sp.diags(dh) @ lv.A.tocsr() @ sp.diags(dh), k=1, which="LA",
+                              return_eigenvectors=False)[0]
+        assert lam <= lv.lmax <= 1.25 * lam
+    Acd = levels[-1].A.toarray()
+    assert np.linalg.eigvalsh(Acd).min() > 0
+    assert np.abs(inv @ Acd - np.eye(len(Acd))).max() < 1e-8
+
+
+def test_vcycle_mirror_pcg_converges():
+    """The numpy restatement of the library's V-cycle (tests/vcycle_mirror.py)
+    is a symmetric positive preconditioner that makes CG converge fast; the
+    GPU test compares the device cycle with it."""
+    from oracle import oracle as O
+    from vcycle_mirror import pcg, vcycle
+    from wnfea_amd.solver.amg_preconditioner import sa_setup
+    model, _ = G.generate_c3d10_structured_block(1.0, 0.1, 0.1, 24, 3, 3, tets_per_hex=6, load_direction="z")
+    E, nu = model.element_moduli()
+    fixed = model.fixed_dof_mask(3)
+    K, F = O.assemble_csr(model.mesh_nodes, model.solid_elements, E, nu, fixed, model.load_vector(3))
+    d = np.abs(K.diagonal())
+    s = 1.0 / np.sqrt(d)
+    A = (sp.diags(s) @ K @ sp.diags(s)).tocsr()
+    levels, inv = sa_setup(A.tobsr(blocksize=(3, 3)), model.mesh_nodes, fixed, coarse_size=300,
+                           dof_scale=np.sqrt(d))
+    rng = np.random.default_rng(0)
+    x, y = rng.standard_normal(A.shape[0]), rng.standard_normal(A.shape[0])
+    Mx, My = vcycle(levels, inv, x), vcycle(levels, inv, y)
+    assert abs(x @ My - y @ Mx) <= 1e-10 * abs(x @ Mx) and x @ Mx > 0
+    u_hat, it = pcg(A, s * F, lambda r: vcycle(levels, inv, r), 1e-8)
+    u = s * u_hat
+    assert it < 60
+    assert np.linalg.norm(F - K @ u) / np.linalg.norm(F) < 1e-7

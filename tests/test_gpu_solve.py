@@ -129,6 +129,18 @@ def test_drop_in_solvers(c0_model, c0_oracle):
     assert _rel(u3, u_ref) < 1e-9
 
 
+def test_vcycle_matches_cpu_mirror(c0_amg):
+    """FP64 device V-cycle against the numpy restatement of the same cycle on
+    the same host hierarchy."""
+    import torch
+    from vcycle_mirror import vcycle
+    n = c0_amg.levels[0].A.shape[0]
+    r = np.random.default_rng(3).standard_normal(n)
+    z_ref = vcycle(c0_amg.levels, c0_amg.coarse_inv, r, deg=c0_amg.degree, ratio=c0_amg.ratio)
+    z = c0_amg.vcycle(torch.as_tensor(r, device="cuda")).cpu().numpy()
+    assert _rel(z, z_ref) <= 1e-12
+
+
 def test_stress_field_vs_oracle(c0_model, c0_oracle):
     from oracle import oracle as O
     from wnfea_amd.solver.solid_solver import compute_c3d10_stress_field

@@ -52,16 +52,39 @@ def rigid_body_modes(coords: np.ndarray, dofs_per_node: int = 3) -> np.ndarray:
 
 
 def _power_lmax(A: sp.spmatrix, dinv: np.ndarray, iters: int = 15, seed: int = 0) -> float:
-    x = np.random.default_rng(seed).standard_normal(A.shape[0])
-    lam = 1.0
-    for _ in range(iters):
-        y = dinv * (A @ x)
-        ny = np.linalg.norm(y)
-        if ny == 0.0:
-            return 1.0
-        lam = ny / np.linalg.norm(x)
-        x = y / ny
-    return float(lam)
+    """Upper estimate of the largest eigenvalue of D^-1 A (dinv > 0): Lanczos
+    on the similar symmetric matrix D^-1/2 A D^-1/2, largest Ritz value plus
+    its residual norm.  Lanczos reaches the top of the spectrum in a few
+    steps where plain power iteration stalls on clustered eigenvalues; the
+    Chebyshev smoother needs a value that is not below the true one."""
+    n = A.shape[0]
+    dh = np.sqrt(np.abs(dinv))
+    v = np.random.default_rng(seed).standard_normal(n)
+    v /= np.linalg.norm(v)
+    V = [v]
+    alpha, beta = [], []
+    steps = max(2, min(iters + 5, n))
+    for k in range(steps):
+        w = dh * (A @ (dh * V[-1]))
+        a = float(V[-1] @ w)
+        w -= a * V[-1]
+        if k:
+            w -= beta[-1] * V[-2]
+        for q in V:  # full reorthogonalisation (few vectors)
+            w -= (q @ w) * q
+        alpha.append(a)
+        b = float(np.linalg.norm(w))
+        if b <= 1e-12 * max(abs(a), 1.0) or k == steps - 1:
+            beta.append(b)
+            break
+        beta.append(b)
+        V.append(w / b)
+    m = len(alpha)
+    T = np.diag(alpha) + np.diag(beta[:m - 1], 1) + np.diag(beta[:m - 1], -1)
+    ev, evec = np.linalg.eigh(T)
+    theta = float(ev[-1])
+    resid = abs(beta[m - 1] * evec[-1, -1])
+    return max(theta + resid, 1e-300)
 
 
 def _strength_graph(A: sp.bsr_matrix, theta: float) -> sp.csr_matrix:
@@ -91,7 +114,7 @@ class AMGLevelData:
 
 def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None = None, theta: float = 0.08,
              theta_decay: float = 1.0, omega: float = 4.0 / 3.0, coarse_size: int = 600, max_levels: int = 10,
-             power_iters: int = 15, lmax_safety: float = 1.1,
+             power_iters: int = 15, lmax_safety: float = 1.05,
              dof_scale: np.ndarray | None = None) -> tuple[list[AMGLevelData], np.ndarray]:
     """Host SA hierarchy of a block matrix.
 
@@ -226,6 +249,7 @@ class SmoothedAggregationAMG:
                 _lib.hptr(Ra[2]), int(sigma))
             _lib.check(rc, f"wnfea_amg_set_level({l})")
         inv = np.ascontiguousarray(inv, dtype=np.float64)
+        self.coarse_inv = inv
         _lib.check(self.lib.wnfea_amg_set_coarse(self.handle, inv.shape[0], _lib.hptr(inv)), "wnfea_amg_set_coarse")
         _lib.check(self.lib.wnfea_amg_finalize(self.handle, int(waves), int(bool(xcd_remap)), int(unroll)),
                    "wnfea_amg_finalize")
