@@ -83,6 +83,28 @@ def cpu_baseline(model, gpu_jacobi_iters: int | None, sample_iters: int = 60) ->
     return out
 
 
+def spmv_only(system, info, reps: int, log) -> int:
+    """SpMV launches alone: `reps` FP32 then `reps` FP64 SpMVs on the resident
+    I-beam matrix, then 5 copies of a 256 MiB FP32 buffer (known HBM bytes:
+    256 MiB read + 256 MiB written per copy) to calibrate the HBM counters."""
+    import torch
+    t32 = time_spmv(system.K, "fp32", reps)
+    t64 = time_spmv(system.K, "fp64", reps)
+    n = 64 * 2 ** 20
+    a = torch.ones(n, dtype=torch.float32, device=system.K.device)
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        c.copy_(a)
+    torch.cuda.synchronize()
+    nnzb, nb = info["nnzb"], system.n_nodes
+    print(json.dumps({"spmv_fp32_us": t32 * 1e6, "spmv_fp64_us": t64 * 1e6,
+                      "bytes_fp32": spmv_bytes(nnzb, nb, 3, 4, 4), "bytes_fp64": spmv_bytes(nnzb, nb, 3, 8, 8),
+                      "copy_bytes_each_way": 4 * n}), flush=True)
+    log("[bench] spmv-only done")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +117,8 @@ def main() -> int:
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine"], default="fp32")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the SpMV launches (and a 256 MiB copy for counter calibration), for rocprofv3 --pmc")
     ap.add_argument("--ir", action="store_true", help="also time the restart-IR (FP32 Krylov) scheme")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
@@ -137,6 +161,8 @@ def main() -> int:
         tune = autotune_spmv(system.K, "fp32", use_cache=False)
         tune.pop("table", None)
         log(f"[bench] autotune {tune} ({time.perf_counter() - t0:.1f}s)")
+    if args.spmv_only:
+        return spmv_only(system, info, args.spmv_reps, log)
     t0 = time.perf_counter()
     amg = build_amg(system, waves=tune["waves"], unroll=tune["unroll"], xcd_remap=tune["xcd_remap"])
     t_amg = time.perf_counter() - t0

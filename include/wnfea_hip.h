@@ -127,7 +127,8 @@ void *wnfea_bsr_create(int br, int bc, int n_rows, int n_cols, const int *rowptr
                        int lanes);
 void wnfea_bsr_destroy(void *h);
 
-/* out[0..7]: n_slices, slots, nnzb, sigma, has FP64, has FP32, has FP64 equilibrated, lanes per row. */
+/* out[0..8]: n_slices, slots, nnzb, sigma, has FP64, has FP32, has FP64 equilibrated, lanes per row,
+ * has FP16. */
 int wnfea_bsr_info(void *h, long long *out);
 
 /* SpMV launch shape: waves per slice (1, 2, 4), XCD-contiguous slice mapping,

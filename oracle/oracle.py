@@ -275,3 +275,25 @@ def stress_field(nodes, conn, E, nu, u3):
     if bad:
         raise ValueError(f"Non-positive Jacobian determinant in element {bad - 1}")
     return cs, cv, nv
+
+
+# ---------------------------------------------------------------- sensitivities
+def adjoint_grad(nodes, conn, E, nu, u3, lam3, chunk=20000):
+    """g_e = -lam_e^T (K_e / E_e) u_e with K_e from the oracle element routine
+    (K is linear in E_e, so dK/dE_e = K_e / E_e).  The reference has no
+    sensitivities; this is the closed form the device gradient is held to."""
+    conn = _c(conn, np.int32)
+    M = len(conn)
+    E = np.broadcast_to(np.asarray(E, dtype=np.float64), (M,))
+    nu = np.broadcast_to(np.asarray(nu, dtype=np.float64), (M,))
+    u = np.asarray(u3, dtype=np.float64).reshape(-1, 3)
+    lam = np.asarray(lam3, dtype=np.float64).reshape(-1, 3)
+    g = np.empty(M)
+    for a in range(0, M, chunk):
+        b = min(M, a + chunk)
+        Ke = ke_batch(nodes, conn[a:b], E[a:b], nu[a:b])
+        ue = u[conn[a:b]].reshape(b - a, 30)
+        le = lam[conn[a:b]].reshape(b - a, 30)
+        g[a:b] = -np.einsum("mi,mij,mj->m", le, Ke, ue) / E[a:b]
+    return g
+

@@ -194,6 +194,7 @@ int wnfea_bsr_info(void* h, long long* out) {
   out[5] = m.d_v32 != nullptr;
   out[6] = m.d_v64e != nullptr;
   out[7] = m.lanes;
+  out[8] = m.d_v16 != nullptr;
   return kOk;
 }
 

@@ -1,0 +1,189 @@
+"""Adjoint sensitivities of C3D10 responses with respect to per-element
+Young's modulus, on the device.
+
+K(E) = sum_e E_e Khat_e (each element stiffness is linear in its own
+modulus), so for a response J = c^T u with K u = f:
+
+    dJ/dE_e = -lam_e^T (K_e / E_e) u_e,   K lam = c.
+
+Compliance f^T u is self-adjoint (lam = u: no extra solve); the tip
+displacement (mean z-displacement of the tip nodes, the measure of the
+reference's I-beam benchmark, scratch/benchmark_c3d10_ibeam.py:139-143)
+needs one extra solve.  Fixed DOFs carry u = lam = 0, so the Dirichlet
+rows/columns of K do not enter the element products.  The element products
+run in one kernel: matrix-free K_e u_e (the internal-force kernel) and the
+dot with lam_e, divided by E_e (wnfea_adjoint_grad).  The reference has no
+sensitivities; this follows its element and solver conventions.
+
+``C3D10Adjoint`` keeps one assembled system and one SA-AMG hierarchy; new
+moduli are assembled in place into the same pattern, and the hierarchy is
+rebuilt only when the moduli moved by more than ``rebuild_rel``.
+``compliance(E)`` and ``tip_displacement(E)`` are torch.autograd functions
+of the modulus tensor.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .. import _lib
+from ..model import FEAModel
+
+
+def element_sensitivities(system, u, lam, E=None):
+    """g_e = -lam_e^T (K_e / E_e) u_e for every element (device FP64 tensor)."""
+    import torch
+    L = _lib.load()
+    mesh = system.mesh
+    M = mesh["n_elements"]
+    E = mesh["E"] if E is None else torch.as_tensor(E, dtype=torch.float64, device=u.device).contiguous()
+    u = u.to(torch.float64).contiguous()
+    lam = lam.to(torch.float64).contiguous()
+    if u.numel() != 3 * mesh["n_nodes"] or lam.numel() != 3 * mesh["n_nodes"]:
+        raise ValueError("u and lam must have 3 DOF per node")
+    fe = torch.empty((M, 30), dtype=torch.float64, device=u.device)
+    grad = torch.empty(M, dtype=torch.float64, device=u.device)
+    _lib.check(L.wnfea_adjoint_grad(M, _lib.dptr(mesh["conn"]), _lib.dptr(mesh["nodes"]), _lib.dptr(E),
+                                    _lib.dptr(mesh["nu"]), _lib.dptr(u), _lib.dptr(lam), _lib.dptr(fe),
+                                    _lib.dptr(grad), _lib.stream_of()), "wnfea_adjoint_grad")
+    return grad
+
+
+def tip_vector(model: FEAModel, tip_nodes, component: int = 2):
+    """c with c^T u = mean of u[component] over the tip nodes (zero on fixed DOFs)."""
+    n = len(model.mesh_nodes)
+    c = np.zeros(3 * n)
+    tip = np.asarray(tip_nodes, dtype=np.int64)
+    c[3 * tip + component] = 1.0 / max(len(tip), 1)
+    c[model.fixed_dof_mask(3)] = 0.0
+    return c
+
+
+class C3D10Adjoint:
+    """Forward solves and adjoint gradients of one C3D10 model on the device."""
+
+    def __init__(self, model: FEAModel, tol: float = 1e-10, amg_precision: str = "fp32", rebuild_rel: float = 0.05,
+                 amg_kw: dict | None = None):
+        import torch
+        from .solid_solver import assemble_c3d10_bsr
+        self.model = model
+        self.tol = tol
+        self.amg_precision = amg_precision
+        self.rebuild_rel = rebuild_rel
+        self.amg_kw = dict(amg_kw or {})
+        self.system = assemble_c3d10_bsr(model, want_fp64_equilibrated=(amg_precision == "fp64"), keep_plan=True)
+        self.E_amg = None
+        self.amg = None
+        self.E_now = self.system.mesh["E"].clone()
+        self.solves = 0
+        self._torch = torch
+
+    def close(self):
+        if self.amg is not None:
+            self.amg.close()
+            self.amg = None
+
+    # ------------------------------------------------------------ forward
+    def set_moduli(self, E):
+        """Assemble K(E) in place (E: M values); rebuild the AMG hierarchy when
+        E moved by more than rebuild_rel relative to the one it was built for."""
+        from .solid_solver import build_amg, reassemble_c3d10_values
+        torch = self._torch
+        E = torch.as_tensor(E, dtype=torch.float64, device=self.system.K.device).reshape(-1).contiguous()
+        if not torch.equal(E, self.E_now):
+            reassemble_c3d10_values(self.system, E=E)
+            self.E_now = E.clone()
+        if self.amg is None or float(((E - self.E_amg).abs() / self.E_amg).max()) > self.rebuild_rel:
+            self.close()
+            storage = ("fp64",) if self.amg_precision == "fp64" else ("fp32", "fp16")
+            self.amg = build_amg(self.system, storage=storage, **self.amg_kw)
+            self.E_amg = E.clone()
+
+    def solve(self, b=None):
+        """u = K^-1 b (b = the load vector by default), FP64-verified to tol."""
+        from .solid_solver import pcg_device, solve_mixed
+        torch = self._torch
+        sysm = self.system
+        if b is None:
+            u, _, _, rel, _ = solve_mixed(sysm, self.amg, tol=self.tol, amg_precision=self.amg_precision,
+                                          max_restarts=6)
+        else:
+            saved = sysm.F
+            try:
+                sysm.F = torch.as_tensor(b, dtype=torch.float64, device=sysm.K.device).contiguous()
+                u, _, _, rel, _ = solve_mixed(sysm, self.amg, tol=self.tol, amg_precision=self.amg_precision,
+                                              max_restarts=6)
+            finally:
+                sysm.F = saved
+        self.solves += 1
+        self.last_rel = rel
+        return u
+
+    # ------------------------------------------------------------ responses
+    def compliance_and_grad(self, E):
+        self.set_moduli(E)
+        u = self.solve()
+        C = float(self.system.F @ u)
+        return C, element_sensitivities(self.system, u, u), u
+
+    def response_and_grad(self, E, c):
+        """J = c^T u and dJ/dE (one forward and one adjoint solve)."""
+        torch = self._torch
+        self.set_moduli(E)
+        c = torch.as_tensor(c, dtype=torch.float64, device=self.system.K.device).contiguous()
+        u = self.solve()
+        lam = self.solve(c)
+        return float(c @ u), element_sensitivities(self.system, u, lam), u, lam
+
+    def compliance(self, E):
+        """Compliance f^T u(E) as a differentiable function of the modulus tensor."""
+        return _Compliance.apply(E, self)
+
+    def tip_displacement(self, E, tip_nodes, component: int = 2):
+        """Mean tip displacement u(E) as a differentiable function of the moduli."""
+        c = tip_vector(self.model, tip_nodes, component)
+        return _Response.apply(E, self, c)
+
+
+def _autograd():
+    import torch
+
+    class Compliance(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, E, problem):
+            C, g, _ = problem.compliance_and_grad(E.detach())
+            ctx.save_for_backward(g.to(E.dtype))
+            return torch.as_tensor(C, dtype=E.dtype, device=E.device)
+
+        @staticmethod
+        def backward(ctx, go):
+            (g,) = ctx.saved_tensors
+            return go * g.to(go.device), None
+
+    class Response(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, E, problem, c):
+            J, g, _, _ = problem.response_and_grad(E.detach(), c)
+            ctx.save_for_backward(g.to(E.dtype))
+            return torch.as_tensor(J, dtype=E.dtype, device=E.device)
+
+        @staticmethod
+        def backward(ctx, go):
+            (g,) = ctx.saved_tensors
+            return go * g.to(go.device), None, None
+
+    return Compliance, Response
+
+
+class _Lazy:
+    def __init__(self, idx):
+        self.idx = idx
+        self.fn = None
+
+    def apply(self, *a):
+        if self.fn is None:
+            self.fn = _autograd()[self.idx]
+        return self.fn.apply(*a)
+
+
+_Compliance = _Lazy(0)
+_Response = _Lazy(1)

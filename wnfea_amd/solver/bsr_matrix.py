@@ -146,10 +146,11 @@ class DeviceBSR:
         return (self.n_rows * self.br, self.n_cols * self.bc)
 
     def info(self) -> dict:
-        out = (_lib._ll * 8)()
+        out = (_lib._ll * 9)()
         _lib.check(self.lib.wnfea_bsr_info(self.handle, out), "wnfea_bsr_info")
         return {"n_slices": out[0], "slots": out[1], "nnzb": out[2], "sigma": out[3], "has_fp64": bool(out[4]),
-                "has_fp32": bool(out[5]), "has_fp64_equilibrated": bool(out[6]), "lanes": out[7]}
+                "has_fp32": bool(out[5]), "has_fp64_equilibrated": bool(out[6]), "lanes": out[7],
+                "has_fp16": bool(out[8])}
 
     def tune(self, waves: int = 1, xcd_remap: bool = True, unroll: int = 1):
         _lib.check(self.lib.wnfea_bsr_tune(self.handle, int(waves), int(bool(xcd_remap)), int(unroll)),

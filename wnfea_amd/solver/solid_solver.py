@@ -121,8 +121,12 @@ def mesh_to_device(model: FEAModel, dev=None) -> dict:
 
 
 def assemble_c3d10_bsr(model: FEAModel, apply_bcs: bool = True, sigma: int = 4096,
-                       want_fp64_equilibrated: bool = False, mesh: dict | None = None) -> SolidSystem:
-    """Assemble K (3x3 BSR on the device), F and the Dirichlet data of a C3D10 model."""
+                       want_fp64_equilibrated: bool = False, mesh: dict | None = None,
+                       keep_plan: bool = False) -> SolidSystem:
+    """Assemble K (3x3 BSR on the device), F and the Dirichlet data of a C3D10 model.
+
+    keep_plan keeps the device contribution lists so that new element moduli
+    can be assembled into the same pattern (``reassemble_c3d10_values``)."""
     import torch
     t0 = time.perf_counter()
     L = _lib.load()
@@ -172,10 +176,58 @@ def assemble_c3d10_bsr(model: FEAModel, apply_bcs: bool = True, sigma: int = 409
     sys_ = _finish_system(K, F, fixed_mask, fixed_dofs, n, want_fp64_equilibrated,
                           coords=np.asarray(model.mesh_nodes, dtype=np.float64), mesh=mesh)
     _sync()
+    if keep_plan:
+        mesh["plan"] = {"cptr": d_cptr, "codes": d_codes, "fix": d_fix if apply_bcs else None,
+                        "want64e": bool(want_fp64_equilibrated)}
     del Ke, d_cptr, d_codes
     torch.cuda.empty_cache()
     sys_.timings = {"plan_sec": t_plan, "layout_sec": t_layout, "assemble_total_sec": time.perf_counter() - t0}
     return sys_
+
+
+def reassemble_c3d10_values(system: SolidSystem, E=None, nu=None) -> SolidSystem:
+    """Assemble new per-element moduli (device tensors or arrays of length M)
+    into the resident K of a system built with keep_plan=True: element
+    stiffnesses, the deterministic gather, the diagonal, the scaling and the
+    equilibrated copies are recomputed in place; the pattern is unchanged."""
+    import torch
+    L = _lib.load()
+    mesh = system.mesh
+    plan = mesh.get("plan")
+    if plan is None:
+        raise ValueError("system was assembled without keep_plan=True")
+    dev = system.K.device
+    M = mesh["n_elements"]
+    for key, val in (("E", E), ("nu", nu)):
+        if val is not None:
+            t = torch.as_tensor(val, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+            if t.numel() != M:
+                raise ValueError(f"{key} must have one value per element ({M})")
+            mesh[key] = t
+    stream = _lib.stream_of()
+    Ke = torch.empty((M, 30, 30), dtype=torch.float64, device=dev)
+    bad = torch.zeros(1, dtype=torch.int32, device=dev)
+    _lib.check(L.wnfea_c3d10_ke(M, _lib.dptr(mesh["conn"]), _lib.dptr(mesh["nodes"]), _lib.dptr(mesh["E"]),
+                                _lib.dptr(mesh["nu"]), _lib.dptr(Ke), _lib.dptr(bad), stream), "wnfea_c3d10_ke")
+    K = system.K
+    _lib.check(L.wnfea_bsr_assemble(K.handle, _lib.dptr(K.d_rowidx), _lib.dptr(K.d_colidx), _lib.dptr(plan["cptr"]),
+                                    _lib.dptr(plan["codes"]), _lib.dptr(Ke),
+                                    _lib.dptr(plan["fix"]) if plan["fix"] is not None else None, None, stream),
+               "wnfea_bsr_assemble")
+    if int(bad.item()):
+        raise ValueError(f"Non-positive Jacobian determinant in C3D10 element {int(bad.item()) - 1}.")
+    del Ke
+    dblk = torch.as_tensor(_diag_block_index(K.rowptr, K.colidx), device=dev)
+    _lib.check(L.wnfea_bsr_diag(K.handle, _lib.dptr(dblk), _lib.dptr(system.diag), stream), "wnfea_bsr_diag")
+    ad = system.diag.abs()
+    system.scale.copy_(torch.where(ad > 1e-12, 1.0 / torch.sqrt(torch.where(ad > 1e-12, ad, torch.ones_like(ad))),
+                                   torch.ones_like(ad)))
+    info = K.info()
+    _lib.check(L.wnfea_bsr_equilibrate(K.handle, _lib.dptr(K.d_rowidx), _lib.dptr(K.d_colidx),
+                                       _lib.dptr(system.scale), int(bool(info["has_fp64_equilibrated"])),
+                                       int(bool(info.get("has_fp16", False))), stream), "wnfea_bsr_equilibrate")
+    _sync()
+    return system
 
 
 def system_from_csr(K_csr, F, fixed_mask: np.ndarray | None = None, coords=None,
