@@ -105,12 +105,140 @@ def spmv_only(system, info, reps: int, log) -> int:
     return 0
 
 
+def run_batch(args, log) -> int:
+    """64 seeded uneven cantilever boxes (nx in [10,60], ny = nz in [3,8],
+    2k-60k DOF each) solved as one block-diagonal system to 1e-8 per mesh,
+    against the same boxes solved one after another on the same GPU."""
+    import torch
+    from wnfea_amd.mesh.rcm import apply_rcm_to_model
+    from wnfea_amd.solver.batch import BatchedSolids, generate_box_batch
+    from wnfea_amd.solver.solid_solver import assemble_c3d10_bsr, build_amg, solve_mixed
+    t0 = time.perf_counter()
+    batch = generate_box_batch(64, seed=1)
+    models = [b[0] for b in batch]
+    n_dofs = sum(b[1]["n_dofs"] for b in batch)
+    log(f"[bench] batch: 64 boxes, {n_dofs} DOF total ({time.perf_counter() - t0:.1f}s)")
+    bs = BatchedSolids(models)
+    log(f"[bench] batched setup {bs.timings}")
+    for _ in range(args.warmup):
+        bs.solve(1e-8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        u, iters, per, _ = bs.solve(1e-8)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / args.steps
+    log(f"[bench] batched solve {t_batch * 1e3:.1f} ms, {iters} iterations, max per-mesh rel {per.max():.2e}")
+    bs.close()
+    # the same meshes one by one (each its own system and hierarchy); solve time only
+    systems = []
+    for m in models:
+        mr, _, _ = apply_rcm_to_model(m)
+        s_ = assemble_c3d10_bsr(mr)
+        systems.append((s_, build_amg(s_, storage=("fp32",))))
+    for s_, a_ in systems:
+        solve_mixed(s_, a_, tol=1e-8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seq_iters = []
+    for s_, a_ in systems:
+        seq_iters.append(solve_mixed(s_, a_, tol=1e-8)[1])
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    for _, a_ in systems:
+        a_.close()
+    log(f"[bench] one-by-one solves {t_seq * 1e3:.1f} ms (iterations {min(seq_iters)}..{max(seq_iters)})")
+    print(json.dumps({
+        "metric": "s to rel-res 1e-8 per mesh, 64 uneven C3D10 boxes in one system",
+        "value": round(t_batch, 5), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_batch * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 CG / f32 AMG", "data": "synthetic (seeded box sizes, deterministic loads)",
+        "config": {"workload": "c3d10_64_boxes_batch", "n_dofs_total": n_dofs, "seed": 1,
+                   "parallelism": "single-gpu"},
+        "dof_per_s": round(n_dofs / t_batch, 1), "iterations": iters, "max_per_mesh_rel_res": float(per.max()),
+        "one_by_one_seconds": round(t_seq, 5), "speedup_batched_vs_one_by_one": round(t_seq / t_batch, 2),
+        "setup": bs.timings}), flush=True)
+    return 0
+
+
+def run_adjoint(args, log) -> int:
+    """I-beam with per-element moduli E_e = 210 GPa (0.2 + 0.8 U(0,1)) (seed 2):
+    compliance f.u and mean tip z-displacement, their gradients with respect to
+    every E_e (one forward and one adjoint solve), the CPU FP64 reference
+    gradient (oracle element matrices with the same u and lambda) and central
+    differences (h = 1e-4 E_e) on 20 seeded elements (seed 3)."""
+    import torch
+    from oracle import oracle as O
+    from wnfea_amd.mesh.rcm import apply_rcm_to_model
+    from wnfea_amd.mesh.solid_generator import generate_c3d10_ibeam
+    from wnfea_amd.solver.adjoint import C3D10Adjoint, tip_vector
+    model, meta = generate_c3d10_ibeam(n_layers=args.layers, length=3.0 * args.layers / 300)
+    model, n2o, o2n = apply_rcm_to_model(model)
+    M = len(model.solid_elements)
+    E = 210e9 * (0.2 + 0.8 * np.random.default_rng(2).uniform(size=M))
+    model.solid_E, model.solid_nu = E, np.full(M, 0.3)
+    tip = o2n[meta["tip_nodes"]]
+    c = tip_vector(model, tip)
+    t0 = time.perf_counter()
+    prob = C3D10Adjoint(model, tol=1e-10, amg_precision="fp32")
+    prob.set_moduli(E)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    prob.compliance_and_grad(E)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    C, gC, u = prob.compliance_and_grad(E)
+    torch.cuda.synchronize()
+    t_c = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    T, gT, _, lam = prob.response_and_grad(E, c)
+    torch.cuda.synchronize()
+    t_t = time.perf_counter() - t0
+    log(f"[bench] compliance {C:.6e} + gradient {t_c * 1e3:.1f} ms; tip {T:.6e} + gradient {t_t * 1e3:.1f} ms")
+    uh, lh = u.cpu().numpy(), lam.cpu().numpy()
+    t0 = time.perf_counter()
+    refC = O.adjoint_grad(model.mesh_nodes, model.solid_elements, E, model.solid_nu, uh, uh)
+    refT = O.adjoint_grad(model.mesh_nodes, model.solid_elements, E, model.solid_nu, uh, lh)
+    t_cpu = time.perf_counter() - t0
+    gCh, gTh = gC.cpu().numpy(), gT.cpu().numpy()
+    errC = float(np.linalg.norm(gCh - refC) / np.linalg.norm(refC))
+    errT = float(np.linalg.norm(gTh - refT) / np.linalg.norm(refT))
+    rng = np.random.default_rng(3)
+    fd_err = []
+    for e in rng.choice(M, 20, replace=False):
+        h = 1e-4 * E[e]
+        vals = []
+        for sgn in (1.0, -1.0):
+            Ep = E.copy()
+            Ep[e] += sgn * h
+            vals.append(prob.compliance_and_grad(Ep)[0])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        fd_err.append(abs(fd - gCh[e]) / abs(gCh[e]))
+    prob.close()
+    log(f"[bench] gradient vs CPU FP64: compliance {errC:.2e}, tip {errT:.2e}; FD max rel err {max(fd_err):.2e}")
+    print(json.dumps({
+        "metric": "compliance + tip-displacement adjoint gradients, C3D10 I-beam, per-element E",
+        "value": round(t_c + t_t, 5), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round((t_c + t_t) * 1e3, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 CG / f32 AMG, f64 gradients",
+        "data": "synthetic (seeded moduli, deterministic loads)",
+        "config": {"workload": "c3d10_ibeam_adjoint_per_element_E", "n_dofs": meta["n_dofs"], "n_elements": M,
+                   "seed_E": 2, "seed_fd": 3, "parallelism": "single-gpu"},
+        "compliance_seconds": round(t_c, 5), "tip_seconds": round(t_t, 5), "setup_seconds": round(t_setup, 3),
+        "grad_rel_err_vs_cpu_fp64": {"compliance": errC, "tip": errT},
+        "fd_rel_err_max_20_elements": float(max(fd_err)), "cpu_reference_gradient_seconds": round(t_cpu, 3),
+        "solves": prob.solves}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", choices=["ibeam", "box"], default="ibeam")
+    ap.add_argument("--config", choices=["ibeam", "box", "batch", "adjoint"], default="ibeam",
+                    help="ibeam/box: the solve metric; batch: 64 uneven boxes in one system; "
+                         "adjoint: compliance and tip-displacement gradients on the I-beam")
     ap.add_argument("--layers", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
@@ -136,6 +264,10 @@ def main() -> int:
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     torch.cuda.set_device(0)
     set_graphs(not args.no_graphs)
+    if args.config == "batch":
+        return run_batch(args, log)
+    if args.config == "adjoint":
+        return run_adjoint(args, log)
     t0 = time.perf_counter()
     if args.config == "ibeam":
         model, meta = generate_c3d10_ibeam(n_layers=args.layers, length=3.0 * args.layers / 300)

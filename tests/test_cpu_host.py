@@ -218,3 +218,31 @@ def test_vcycle_mirror_pcg_converges():
     u = s * u_hat
     assert it < 60
     assert np.linalg.norm(F - K @ u) / np.linalg.norm(F) < 1e-7
+
+
+# ---------------------------------------------------------------- many-mesh batches
+def test_box_batch_generator_and_merge():
+    """Seeded uneven boxes inside the DOF window; the merged model is the
+    block-diagonal union (offsets, loads, supports, moduli carried along)."""
+    from wnfea_amd.solver.batch import generate_box_batch, merge_models
+    batch = generate_box_batch(8, seed=1, dof_range=(2000, 60000))
+    again = generate_box_batch(8, seed=1, dof_range=(2000, 60000))
+    assert [b[1]["grid"] for b in batch] == [b[1]["grid"] for b in again]
+    for m, meta in batch:
+        nx, ny, nz = meta["grid"]
+        assert 10 <= nx <= 60 and 3 <= ny <= 8 and ny == nz
+        assert 2000 <= meta["n_dofs"] == 3 * (2 * nx + 1) * (2 * ny + 1) ** 2 <= 60000
+    models = [b[0] for b in batch]
+    merged, offs = merge_models(models)
+    assert offs[-1] == len(merged.mesh_nodes) == sum(len(m.mesh_nodes) for m in models)
+    F = merged.load_vector(3)
+    fixed = merged.fixed_dof_mask(3)
+    for k, m in enumerate(models):
+        sl = slice(3 * offs[k], 3 * offs[k + 1])
+        assert np.array_equal(F[sl], m.load_vector(3))
+        assert np.array_equal(fixed[sl], m.fixed_dof_mask(3))
+    conn = merged.solid_elements
+    owner = np.searchsorted(offs, conn, side="right") - 1
+    assert np.all(owner == owner[:, :1])  # no element spans two meshes
+    E, nu = merged.element_moduli()
+    assert len(E) == len(conn) and np.all(E == 2.1e11)

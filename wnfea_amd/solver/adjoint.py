@@ -94,13 +94,13 @@ class C3D10Adjoint:
             self.E_now = E.clone()
         if self.amg is None or float(((E - self.E_amg).abs() / self.E_amg).max()) > self.rebuild_rel:
             self.close()
-            storage = ("fp64",) if self.amg_precision == "fp64" else ("fp32", "fp16")
+            storage = {"fp64": ("fp64",), "fp32": ("fp32",)}.get(self.amg_precision, ("fp32", "fp16"))
             self.amg = build_amg(self.system, storage=storage, **self.amg_kw)
             self.E_amg = E.clone()
 
     def solve(self, b=None):
         """u = K^-1 b (b = the load vector by default), FP64-verified to tol."""
-        from .solid_solver import pcg_device, solve_mixed
+        from .solid_solver import solve_mixed
         torch = self._torch
         sysm = self.system
         if b is None:
@@ -120,19 +120,24 @@ class C3D10Adjoint:
 
     # ------------------------------------------------------------ responses
     def compliance_and_grad(self, E):
+        """C = f^T u and dC/dE.  C is evaluated as the adjoint-corrected
+        2 f^T u - u^T K u, whose error is quadratic in the solve error."""
         self.set_moduli(E)
         u = self.solve()
-        C = float(self.system.F @ u)
+        F = self.system.F
+        C = float(2.0 * (F @ u) - u @ self.system.K.spmv(u))
         return C, element_sensitivities(self.system, u, u), u
 
     def response_and_grad(self, E, c):
-        """J = c^T u and dJ/dE (one forward and one adjoint solve)."""
+        """J = c^T u and dJ/dE (one forward and one adjoint solve); J carries
+        the adjoint correction lam^T (f - K u)."""
         torch = self._torch
         self.set_moduli(E)
         c = torch.as_tensor(c, dtype=torch.float64, device=self.system.K.device).contiguous()
         u = self.solve()
         lam = self.solve(c)
-        return float(c @ u), element_sensitivities(self.system, u, lam), u, lam
+        J = float(c @ u + lam @ (self.system.F - self.system.K.spmv(u)))
+        return J, element_sensitivities(self.system, u, lam), u, lam
 
     def compliance(self, E):
         """Compliance f^T u(E) as a differentiable function of the modulus tensor."""
