@@ -70,6 +70,32 @@ int wnfea_c3d10_stress(int M, int n_nodes, const int *conn, const double *nodes,
 int wnfea_adjoint_grad(int M, const int *conn, const double *nodes, const double *E, const double *nu,
                        const double *u, const double *lam, double *fe_work, double *grad, void *stream);
 
+/* ---------------------------------------------------------------- co-rotational beams */
+
+/* Co-rotational internal forces of 2-node beams, FP64, deterministic.
+ * conn[M][2], nodes[N][3], props[M][6] = (E, G, A, Iy, Iz, J), U[N][6]
+ * (device); fe_work[M][12] device scratch; n2e_ptr/n2e the node incidence
+ * (wnfea_node_incidence with npe = 2); F[N][6] (accumulate != 0 adds).
+ * Replaces hip_compute_beam_forces (wnfea/solver/native/hip_kernels.cpp:53-241),
+ * compute_corotational_element_forces (wnfea/solver/corotational_beam.py:178-233)
+ * and the vectorised beam path of compute_internal_forces_fast
+ * (wnfea/solver/fast_kernels.py:451-537). */
+int wnfea_beam_forces(int M, int n_nodes, const int *conn, const double *nodes, const double *props, const double *U,
+                      const int *n2e_ptr, const int *n2e, double *fe_work, double *F, int accumulate, void *stream);
+
+/* Element tangents Kt[M][12][12] = d f_e / d u_e by central differences of
+ * the co-rotational forces (h = h_rel * L0 for translations, h_rel for
+ * rotations).  Replaces the finite-difference Jacobian action of
+ * MatrixFreeJFNKOperator._matvec (wnfea/solver/jfnk_operator.py:98-134) with
+ * an assembled tangent. */
+int wnfea_beam_tangent(int M, const int *conn, const double *nodes, const double *props, const double *U,
+                       double h_rel, double *Kt, void *stream);
+
+/* Linear 12x12 beam stiffness T^T K_local T per element (device).  Replaces
+ * build_element_stiffness_3d_beam and build_transformation_matrix
+ * (wnfea/solver/assembler.py:13-148), frame rules included. */
+int wnfea_beam_ke_linear(int M, const int *conn, const double *nodes, const double *props, double *Ke, void *stream);
+
 /* ---------------------------------------------------------------- host setup */
 
 /* Reverse Cuthill-McKee order (host arrays): perm[new] = old, for a symmetric
@@ -150,6 +176,16 @@ int wnfea_bsr_get_values(void *h, double *vals_std, void *stream);
  * value path (wnfea/solver/solid_solver.py:66-127). */
 int wnfea_bsr_assemble(void *h, const int *rowidx, const int *colidx, const long long *cptr, const int *codes,
                        const double *Ke, const unsigned char *node_fix, double *vals_std, void *stream);
+
+/* Element matrices Ke[M][npe*b][npe*b] of b x b-block matrix h (b = 3 or 6)
+ * through the contribution lists (codes e*npe*npe + a*npe + c); sym != 0
+ * assembles 0.5 (Ke + Ke^T); node_fix bits as wnfea_bsr_assemble (bit c =
+ * DOF c fixed).  Replaces the beam branch of assemble_global_system_sparse
+ * (wnfea/solver/assembler.py:285-450) and hip_assemble_beam_system_csr
+ * (wnfea/solver/fast_kernels.py:666-763). */
+int wnfea_bsr_assemble_elem(void *h, int npe, int sym, const int *rowidx, const int *colidx, const long long *cptr,
+                            const int *codes, const double *Ke, const unsigned char *node_fix, double *vals_std,
+                            void *stream);
 
 /* Scalar diagonal (device) given diag_block[i] = standard index of block (i,i). */
 int wnfea_bsr_diag(void *h, const long long *diag_block, double *diag, void *stream);

@@ -1,0 +1,126 @@
+"""Pins of the co-rotational beam oracle (no GPU): the two reference
+formulations agree, rigid motions give no force to first order, the
+small-displacement limit is the linear stiffness, the FD tangent is the
+directional derivative, and the lattice generator matches the C4 counts."""
+import numpy as np
+import pytest
+
+from oracle import beam_oracle as B
+
+
+def _random_beams(rng, M, axis_aligned=False):
+    X1 = rng.uniform(-1, 1, (M, 3))
+    if axis_aligned:
+        d = np.eye(3)[rng.integers(0, 3, M)] * rng.choice([-1.0, 1.0], (M, 1))
+    else:
+        d = rng.standard_normal((M, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+    X2 = X1 + d * rng.uniform(0.5, 2.0, (M, 1))
+    r = 0.02
+    A, I = np.pi * r ** 2, np.pi * r ** 4 / 4
+    E = 70e9
+    props = np.tile([E, E / (2 * 1.33), A, I, I, 2 * I], (M, 1))
+    return X1, X2, props
+
+
+@pytest.mark.parametrize("axis_aligned", [False, True])
+def test_two_reference_formulations_agree(axis_aligned):
+    """fast_kernels.py:451-537 (batch) and corotational_beam.py:178-233 (single
+    element) give the same forces for circular sections (Iy = Iz)."""
+    rng = np.random.default_rng(0)
+    X1, X2, props = _random_beams(rng, 64, axis_aligned)
+    ue = 1e-2 * rng.standard_normal((64, 12))
+    fb = B.corot_element_forces(X1, X2, props, ue)
+    for e in range(64):
+        fs = B.corot_element(X1[e], X2[e], ue[e], *props[e])
+        assert np.abs(fb[e] - fs).max() <= 1e-9 * np.abs(fs).max(), e
+
+
+def test_rigid_motions_give_no_first_order_force():
+    rng = np.random.default_rng(1)
+    X1, X2, props = _random_beams(rng, 32)
+    t = rng.standard_normal(3)
+    ue = np.tile(np.concatenate([t, np.zeros(3)]), 2)[None].repeat(32, axis=0)
+    f = B.corot_element_forces(X1, X2, props, ue)
+    scale = props[0, 0] * props[0, 2]
+    assert np.abs(f).max() <= 1e-12 * scale
+    w = rng.standard_normal(3)
+    w /= np.linalg.norm(w)
+
+    def rot_forces(theta):
+        c, s = np.cos(theta), np.sin(theta)
+        K = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+        R = np.eye(3) + s * K + (1 - c) * K @ K
+        u1, u2 = X1 @ R.T - X1, X2 @ R.T - X2
+        th = np.tile(theta * w, (32, 1))
+        return B.corot_element_forces(X1, X2, props, np.hstack([u1, th, u2, th]))
+
+    f1, f2 = np.abs(rot_forces(1e-3)).max(), np.abs(rot_forces(2e-3)).max()
+    assert f1 <= 1e-5 * scale * 1e-3  # no first-order response
+    assert 3.0 <= f2 / f1 <= 5.0  # second order
+
+
+def test_small_displacement_limit_is_linear_stiffness():
+    """Right-handed frames (general orientation): f(eps u)/eps -> K_lin u
+    (assembler.py:13-148)."""
+    rng = np.random.default_rng(2)
+    X1, X2, props = _random_beams(rng, 16)
+    nodes = np.vstack([X1, X2])
+    conn = np.column_stack([np.arange(16), 16 + np.arange(16)])
+    K = B.ke_linear(nodes, conn, props)
+    u = rng.standard_normal((16, 12))
+    eps = 1e-7
+    f = B.corot_element_forces(X1, X2, props, eps * u) / eps
+    lin = np.einsum("eij,ej->ei", K, u)
+    assert np.linalg.norm(f - lin) <= 1e-5 * np.linalg.norm(lin)
+
+
+def test_fd_tangent_is_directional_derivative():
+    from wnfea_amd.mesh.lattice import beam_props, generate_beam_lattice
+    model, _ = generate_beam_lattice(2)
+    props = beam_props(model)
+    rng = np.random.default_rng(3)
+    U = 1e-3 * rng.standard_normal(6 * len(model.mesh_nodes))
+    Kt = B.fd_tangent(model.mesh_nodes, model.mesh_elements, props, U)
+    v = rng.standard_normal(6 * len(model.mesh_nodes))
+    dofs = B.element_dofs(model.mesh_elements)
+    h = 1e-6
+    fp = B.corot_element_forces(model.mesh_nodes[model.mesh_elements[:, 0]], model.mesh_nodes[model.mesh_elements[:, 1]],
+                                props, U[dofs] + h * v[dofs])
+    fm = B.corot_element_forces(model.mesh_nodes[model.mesh_elements[:, 0]], model.mesh_nodes[model.mesh_elements[:, 1]],
+                                props, U[dofs] - h * v[dofs])
+    dd = (fp - fm) / (2 * h)
+    assert np.linalg.norm(np.einsum("eij,ej->ei", Kt, v[dofs]) - dd) <= 1e-5 * np.linalg.norm(dd)
+
+
+def test_lattice_counts_and_loads():
+    from wnfea_amd.mesh.lattice import generate_beam_lattice, lattice_nodes_and_beams
+    nodes, beams = lattice_nodes_and_beams(40, 40, 40)
+    assert len(nodes) == 68921 and len(beams) == 201720 and 6 * len(nodes) == 413526
+    model, meta = generate_beam_lattice(3, seed=4)
+    assert meta["base_nodes"] == 16 and meta["top_nodes"] == 16
+    F = model.load_vector(6).reshape(-1, 6)
+    top = np.flatnonzero(model.mesh_nodes[:, 2] == 3.0)
+    assert np.all(F[top, 2] < 0) and np.all(F[:, 3:] == 0)
+    assert model.fixed_dof_mask(6).reshape(-1, 6)[model.mesh_nodes[:, 2] == 0].all()
+    L = np.linalg.norm(model.mesh_nodes[model.mesh_elements[:, 1]] - model.mesh_nodes[model.mesh_elements[:, 0]],
+                       axis=1)
+    assert np.allclose(L, 1.0)
+
+
+def test_oracle_newton_small_load_is_linear():
+    """Nonlinear and linear responses agree within 0.2 % at small load
+    (test_nonlinear_jfnk.py:118-158)."""
+    import scipy.sparse.linalg as spla
+    from wnfea_amd.mesh.lattice import beam_props, generate_beam_lattice
+    model, _ = generate_beam_lattice(2, lateral_sigma=10.0, compression_mean=20.0, compression_sigma=5.0)
+    props = beam_props(model)
+    fixed = model.fixed_dof_mask(6)
+    F = model.load_vector(6)
+    F[fixed] = 0.0
+    U, its = B.newton(model.mesh_nodes, model.mesh_elements, props, F, fixed, n_steps=2)
+    K0 = B.assemble(len(model.mesh_nodes), model.mesh_elements,
+                    B.fd_tangent(model.mesh_nodes, model.mesh_elements, props, np.zeros_like(F)), fixed)
+    Ulin = spla.spsolve(K0.tocsc(), F)
+    assert np.linalg.norm(U - Ulin) <= 2e-3 * np.linalg.norm(Ulin)
+    assert max(its) <= 5

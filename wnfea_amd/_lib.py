@@ -31,6 +31,10 @@ SIGNATURES = {
     "wnfea_c3d10_fint": (_i, [_i, _i, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p]),
     "wnfea_c3d10_stress": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "wnfea_adjoint_grad": (_i, [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "wnfea_beam_forces": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _p]),
+    "wnfea_beam_tangent": (_i, [_i, _p, _p, _p, _p, _d, _p, _p]),
+    "wnfea_beam_ke_linear": (_i, [_i, _p, _p, _p, _p, _p]),
+    "wnfea_bsr_assemble_elem": (_i, [_p, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p]),
     "wnfea_rcm": (_i, [_i, _p, _p, _p, _p]),
     "wnfea_node_graph": (_ll, [_i, _i, _i, _p, _p, _p, _p, _ll]),
     "wnfea_contributions": (_ll, [_i, _i, _i, _p, _p, _p, _p, _p, _ll]),

@@ -124,6 +124,27 @@ int wnfea_adjoint_grad(int M, const int* conn, const double* nodes, const double
   return launch_adjoint_dot(M, conn, E, fe_work, lam, grad, S(stream));
 }
 
+// ------------------------------------------------------------ beams
+int wnfea_beam_forces(int M, int n_nodes, const int* conn, const double* nodes, const double* props, const double* U,
+                      const int* n2e_ptr, const int* n2e, double* fe_work, double* F, int accumulate, void* stream) {
+  if (M < 0 || n_nodes < 0) return kBadArg;
+  if (M > 0 && (!conn || !nodes || !props || !U || !fe_work)) return kBadArg;
+  if (n_nodes > 0 && (!n2e_ptr || !n2e || !F)) return kBadArg;
+  return launch_beam_forces(M, n_nodes, conn, nodes, props, U, n2e_ptr, n2e, fe_work, F, accumulate, S(stream));
+}
+
+int wnfea_beam_tangent(int M, const int* conn, const double* nodes, const double* props, const double* U,
+                       double h_rel, double* Kt, void* stream) {
+  if (M < 0 || !(h_rel > 0.0) || (M > 0 && (!conn || !nodes || !props || !U || !Kt))) return kBadArg;
+  return launch_beam_tangent(M, conn, nodes, props, U, h_rel, Kt, S(stream));
+}
+
+int wnfea_beam_ke_linear(int M, const int* conn, const double* nodes, const double* props, double* Ke,
+                         void* stream) {
+  if (M < 0 || (M > 0 && (!conn || !nodes || !props || !Ke))) return kBadArg;
+  return launch_beam_ke_linear(M, conn, nodes, props, Ke, S(stream));
+}
+
 // ------------------------------------------------------------ host setup
 int wnfea_rcm(int n, const int* ptr, const int* ind, const int* seed_order, int* perm) {
   if (n < 0 || (n > 0 && (!ptr || !ind || !seed_order || !perm))) return kBadArg;
@@ -224,6 +245,14 @@ int wnfea_bsr_assemble(void* h, const int* rowidx, const int* colidx, const long
   auto* b = static_cast<WnBsr*>(h);
   if (b->m.nnzb > 0 && (!rowidx || !colidx || !cptr || !codes || !Ke)) return kBadArg;
   return assemble_gather(b->m, rowidx, colidx, cptr, codes, Ke, node_fix, vals_std, S(stream));
+}
+
+int wnfea_bsr_assemble_elem(void* h, int npe, int sym, const int* rowidx, const int* colidx, const long long* cptr,
+                            const int* codes, const double* Ke, const unsigned char* node_fix, double* vals_std,
+                            void* stream) {
+  if (!h || !rowidx || !colidx || !cptr || !codes || !Ke) return kBadArg;
+  return assemble_elem(static_cast<WnBsr*>(h)->m, npe, sym, rowidx, colidx, cptr, codes, Ke, node_fix, vals_std,
+                       S(stream));
 }
 
 int wnfea_bsr_diag(void* h, const long long* diag_block, double* diag, void* stream) {

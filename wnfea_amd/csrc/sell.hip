@@ -244,6 +244,69 @@ int assemble_gather(SellMatrix& A, const int* d_rowidx, const int* d_colidx, con
   return static_cast<int>(hipGetLastError());
 }
 
+// Generic version for b x b blocks of npe-node elements (beams: b = 6, npe =
+// 2): code = e*npe*npe + a*npe + c addresses Ke[e][b*a + r][b*c + q].
+template <int B>
+__global__ void assemble_elem_kernel(long long nnzb, int npe, int sym, const int* __restrict__ rowidx,
+                                     const int* __restrict__ colidx, const long long* __restrict__ cptr,
+                                     const int* __restrict__ codes, const double* __restrict__ Ke,
+                                     const unsigned char* __restrict__ node_fix, const long long* __restrict__ slot,
+                                     double* __restrict__ v64, double* __restrict__ vstd) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= nnzb) return;
+  const int D = npe * B, nn = npe * npe;
+  double acc[B * B];
+#pragma unroll
+  for (int t = 0; t < B * B; ++t) acc[t] = 0.0;
+  for (long long q = cptr[k]; q < cptr[k + 1]; ++q) {
+    const int code = codes[q];
+    const int e = code / nn, ab = code - e * nn, a = ab / npe, b = ab - a * npe;
+    const double* kp = Ke + (size_t)e * D * D;
+#pragma unroll
+    for (int r = 0; r < B; ++r)
+#pragma unroll
+      for (int c = 0; c < B; ++c) {
+        double v = kp[(size_t)(B * a + r) * D + B * b + c];
+        if (sym) v = 0.5 * (v + kp[(size_t)(B * b + c) * D + B * a + r]);
+        acc[r * B + c] += v;
+      }
+  }
+  const int i = rowidx[k], j = colidx[k];
+  const unsigned fi = node_fix ? node_fix[i] : 0u, fj = node_fix ? node_fix[j] : 0u;
+  const long long sl = slot[k];
+#pragma unroll
+  for (int r = 0; r < B; ++r)
+#pragma unroll
+    for (int c = 0; c < B; ++c) {
+      double v = acc[r * B + c];
+      if (((fi >> r) & 1u) || ((fj >> c) & 1u)) v = (i == j && r == c) ? 1.0 : 0.0;
+      v64[val_index<double>(sl, r * B + c, B * B)] = v;
+      if (vstd) vstd[k * B * B + r * B + c] = v;
+    }
+}
+
+int assemble_elem(SellMatrix& A, int npe, int sym, const int* d_rowidx, const int* d_colidx, const long long* cptr,
+                  const int* codes, const double* Ke, const unsigned char* node_fix, double* vals_std, hipStream_t s) {
+  if (A.br != A.bc || npe <= 0 || npe > 27) return kBadArg;
+  int r = A.ensure_values(1, 0);
+  if (r) return r;
+  if (A.nnzb == 0) return kOk;
+  const dim3 grid(ceil_div(A.nnzb, 128)), blk(128);
+  switch (A.br) {
+    case 3:
+      hipLaunchKernelGGL(assemble_elem_kernel<3>, grid, blk, 0, s, A.nnzb, npe, sym, d_rowidx, d_colidx, cptr, codes,
+                         Ke, node_fix, A.d_slot, A.d_v64, vals_std);
+      break;
+    case 6:
+      hipLaunchKernelGGL(assemble_elem_kernel<6>, grid, blk, 0, s, A.nnzb, npe, sym, d_rowidx, d_colidx, cptr, codes,
+                         Ke, node_fix, A.d_slot, A.d_v64, vals_std);
+      break;
+    default:
+      return kBadArg;
+  }
+  return static_cast<int>(hipGetLastError());
+}
+
 __global__ void extract_diag_kernel(int n_rows, int b, const long long* __restrict__ diag_block,
                                     const long long* __restrict__ slot, const double* __restrict__ v64,
                                     double* __restrict__ diag) {

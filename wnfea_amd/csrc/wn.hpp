@@ -75,6 +75,13 @@ int launch_stress(int M, const int* conn, const double* nodes, const double* E, 
                   double* gp_sig, double* gp_vm, double* cell_sig, double* cell_vm, int* bad, hipStream_t s);
 int launch_nodal_average(int n_nodes, const int* n2e_ptr, const int* n2e, const double* cell_vm, double* nodal_vm,
                          hipStream_t s);
+int launch_beam_forces(int M, int n_nodes, const int* conn, const double* nodes, const double* props,
+                       const double* U, const int* n2e_ptr, const int* n2e, double* fe, double* F, int accumulate,
+                       hipStream_t s);
+int launch_beam_tangent(int M, const int* conn, const double* nodes, const double* props, const double* U,
+                        double h_rel, double* Kt, hipStream_t s);
+int launch_beam_ke_linear(int M, const int* conn, const double* nodes, const double* props, double* Ke,
+                          hipStream_t s);
 int launch_adjoint_dot(int M, const int* conn, const double* E, const double* fe, const double* lam, double* grad,
                        hipStream_t s);
 
@@ -135,6 +142,11 @@ int sell_from_std(SellMatrix& A, const double* vals_std, int to64, int to32, hip
 int sell_to_std(const SellMatrix& A, double* vals_std, hipStream_t s);
 int assemble_gather(SellMatrix& A, const int* d_rowidx, const int* d_colidx, const long long* cptr, const int* codes,
                     const double* Ke, const unsigned char* node_fix, double* vals_std, hipStream_t s);
+// Square element matrices Ke[M][npe*b][npe*b] (b = block size of A) into A's
+// FP64 values through the contribution lists; sym != 0 assembles the
+// symmetric part 0.5 (Ke + Ke^T); node_fix bit c marks DOF c of a node fixed.
+int assemble_elem(SellMatrix& A, int npe, int sym, const int* d_rowidx, const int* d_colidx, const long long* cptr,
+                  const int* codes, const double* Ke, const unsigned char* node_fix, double* vals_std, hipStream_t s);
 int extract_diag(const SellMatrix& A, const long long* diag_block, double* diag, hipStream_t s);
 int equilibrate(SellMatrix& A, const int* d_rowidx, const int* d_colidx, const double* scale, int want64e,
                 hipStream_t s, int want16 = 0);

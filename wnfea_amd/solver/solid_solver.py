@@ -48,10 +48,11 @@ class SolidSystem:
     coords: np.ndarray | None = None
     mesh: dict = field(default_factory=dict)   # device mesh arrays for element kernels
     timings: dict = field(default_factory=dict)
+    dpn: int = 3                      # DOF per node (block size): 3 solids, 6 beams
 
     @property
     def n_dofs(self) -> int:
-        return 3 * self.n_nodes
+        return self.dpn * self.n_nodes
 
     def equilibrated_host(self) -> sp.bsr_matrix:
         """S K S as a host scipy BSR (FP64), the input of the AMG setup."""
@@ -60,9 +61,10 @@ class SolidSystem:
         s = self.scale
         rows = self.K.d_rowidx.long()
         cols = self.K.d_colidx.long()
-        sr = s.view(-1, 3)[rows]  # (nnzb, 3)
-        sc = s.view(-1, 3)[cols]
-        v = vals.view(-1, 3, 3) * sr[:, :, None] * sc[:, None, :]
+        b = self.dpn
+        sr = s.view(-1, b)[rows]  # (nnzb, b)
+        sc = s.view(-1, b)[cols]
+        v = vals.view(-1, b, b) * sr[:, :, None] * sc[:, None, :]
         out = sp.bsr_matrix((v.cpu().numpy(), self.K.colidx, self.K.rowptr), shape=self.K.shape)
         del vals, sr, sc, v
         torch.cuda.empty_cache()
@@ -84,8 +86,9 @@ def _finish_system(K: DeviceBSR, F_host: np.ndarray, fixed_mask: np.ndarray, fix
     L = _lib.load()
     dev = K.device
     stream = _lib.stream_of()
+    dpn = K.br
     dblk = torch.as_tensor(_diag_block_index(K.rowptr, K.colidx), device=dev)
-    diag = torch.empty(3 * n_nodes, dtype=torch.float64, device=dev)
+    diag = torch.empty(dpn * n_nodes, dtype=torch.float64, device=dev)
     _lib.check(L.wnfea_bsr_diag(K.handle, _lib.dptr(dblk), _lib.dptr(diag), stream), "wnfea_bsr_diag")
     ad = diag.abs()
     scale = torch.where(ad > 1e-12, 1.0 / torch.sqrt(torch.where(ad > 1e-12, ad, torch.ones_like(ad))),
@@ -94,7 +97,7 @@ def _finish_system(K: DeviceBSR, F_host: np.ndarray, fixed_mask: np.ndarray, fix
                                        int(want_fp64_equilibrated), 0, stream), "wnfea_bsr_equilibrate")
     F = torch.as_tensor(np.ascontiguousarray(F_host, dtype=np.float64), device=dev)
     return SolidSystem(K=K, F=F, fixed_mask=fixed_mask, fixed_dofs=fixed_dofs, diag=diag, scale=scale,
-                       n_nodes=n_nodes, coords=coords, mesh=mesh or {}, timings=timings or {})
+                       n_nodes=n_nodes, coords=coords, mesh=mesh or {}, timings=timings or {}, dpn=dpn)
 
 
 def mesh_to_device(model: FEAModel, dev=None) -> dict:
@@ -217,7 +220,21 @@ def reassemble_c3d10_values(system: SolidSystem, E=None, nu=None) -> SolidSystem
     if int(bad.item()):
         raise ValueError(f"Non-positive Jacobian determinant in C3D10 element {int(bad.item()) - 1}.")
     del Ke
-    dblk = torch.as_tensor(_diag_block_index(K.rowptr, K.colidx), device=dev)
+    refresh_scaling(system)
+    return system
+
+
+def refresh_scaling(system: SolidSystem) -> SolidSystem:
+    """After new FP64 values of K (same pattern): diag(K), S = diag(1/sqrt|K_ii|)
+    and the equilibrated copies (FP32, and FP64/FP16 where they exist)."""
+    import torch
+    L = _lib.load()
+    K = system.K
+    stream = _lib.stream_of()
+    dblk = system.mesh.get("_diag_block")
+    if dblk is None:
+        dblk = torch.as_tensor(_diag_block_index(K.rowptr, K.colidx), device=K.device)
+        system.mesh["_diag_block"] = dblk
     _lib.check(L.wnfea_bsr_diag(K.handle, _lib.dptr(dblk), _lib.dptr(system.diag), stream), "wnfea_bsr_diag")
     ad = system.diag.abs()
     system.scale.copy_(torch.where(ad > 1e-12, 1.0 / torch.sqrt(torch.where(ad > 1e-12, ad, torch.ones_like(ad))),
