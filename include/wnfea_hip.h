@@ -273,6 +273,22 @@ int wnfea_ir(void *solver, void *bsr, const double *scale, int pkind, const floa
              const double *f, double *u, double tol, double inner_rtol, int max_outer, int max_inner,
              int check_every, int zero_init, int *outer, int *inner, double *rel, void *stream);
 
+/* Multiple right-hand sides: PCG on nrhs (4 or 8) columns at once, B and X
+ * FP64 [n][nrhs] interleaved (device).  Per-column CG (rho, alpha, beta,
+ * ||r||^2 per column; converged columns are frozen), one FP64 SpMM per
+ * iteration (K read once for all columns), the FP32 V-cycle (amg_vt 1) on
+ * S K S for all columns, z = S M(S r).  iters[nrhs], rel[nrhs] (host) get
+ * the per-column iterations and recursive relative residuals.  No reference
+ * counterpart (the reference solves one load case per call,
+ * wnfea/solver/solid_solver.py:197-308). */
+int wnfea_pcg_multi(void *solver, void *bsr, void *amg, int amg_vt, const double *scale, int nrhs, const double *B,
+                    double *X, double rtol, int max_iter, int check_every, int *iters, double *rel, void *stream);
+
+/* Y = K X on nrhs (4 or 8) interleaved columns; prec 0 FP64, 1 FP32 values.
+ * Multi-vector form of hip_spmv_bsr6x6_fp32/_fp64
+ * (wnfea/solver/native/hip_kernels.cpp:2975,3032). */
+int wnfea_bsr_spmm(void *h, int prec, int nrhs, const void *X, void *Y, void *stream);
+
 /* r = f - K u (FP64; r may be NULL) and out[0] = ||r||, out[1] = ||f||. */
 int wnfea_residual(void *solver, void *bsr, const double *u, const double *f, double *r, double *out, void *stream);
 

@@ -246,3 +246,19 @@ def test_box_batch_generator_and_merge():
     assert np.all(owner == owner[:, :1])  # no element spans two meshes
     E, nu = merged.element_moduli()
     assert len(E) == len(conn) and np.all(E == 2.1e11)
+
+
+def test_ibeam_tip_load_cases_resultants():
+    from wnfea_amd.mesh.solid_generator import generate_c3d10_ibeam, ibeam_tip_load_cases
+    model, meta = generate_c3d10_ibeam(n_layers=2, length=0.02)
+    tip = meta["tip_nodes"]
+    B = ibeam_tip_load_cases(model.mesh_nodes, tip, n_cases=4, seed=0)
+    rng = np.random.default_rng(0)
+    X = model.mesh_nodes
+    for k in range(4):
+        F, M = rng.normal(0.0, 20e3, 3), rng.normal(0.0, 2e3, 3)
+        f = B[:, k].reshape(-1, 3)
+        assert np.allclose(f.sum(axis=0), F, rtol=0, atol=1e-6)
+        r = X - np.array([0.0, 0.0, 0.15])
+        r[:, 0] = 0.0  # moments about the tip-section centroid
+        assert np.allclose(np.cross(r, f).sum(axis=0), M, rtol=0, atol=1e-6)

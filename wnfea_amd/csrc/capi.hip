@@ -38,6 +38,9 @@ int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float
 Solver* solver_new();
 void solver_delete(Solver*);
 Workspace& solver_ws(Solver&);
+int pcg_multi(Solver& S, const SellMatrix& A, Amg* g, int vt, const double* scale, int nr, const double* B, double* X,
+              double rtol, int max_iter, int check_every, const SpmvTune& t, hipStream_t s, int* iters, double* rel);
+int bsr_spmm(const SellMatrix& A, int prec, int nr, const void* X, void* Y, const SpmvTune& t, hipStream_t s);
 void solver_set_graphs(Solver&, int enable);
 Amg* amg_new();
 void amg_delete(Amg*);
@@ -406,6 +409,25 @@ int wnfea_ir(void* solver, void* bsr, const double* scale, int pkind, const floa
   auto* B = static_cast<WnBsr*>(bsr);
   return ir_run(*static_cast<Solver*>(solver), B->m, scale, pkind, dinv32, static_cast<Amg*>(amg), amg_vt, f, u, tol,
                 inner_rtol, max_outer, max_inner, check_every, B->tune, S(stream), outer, inner, rel, zero_init);
+}
+
+// PCG on nrhs (4 or 8) interleaved columns B[n][nrhs] -> X[n][nrhs], FP64
+// vectors and SpMM on the primary values, FP32 V-cycle (amg_vt 1) on S K S.
+int wnfea_pcg_multi(void* solver, void* bsr, void* amg, int amg_vt, const double* scale, int nrhs, const double* B,
+                    double* X, double rtol, int max_iter, int check_every, int* iters, double* rel, void* stream) {
+  if (!solver || !bsr || !amg || !B || !X || max_iter < 0 || (nrhs != 4 && nrhs != 8)) return kBadArg;
+  auto* Bm = static_cast<WnBsr*>(bsr);
+  if (!Bm->m.d_v64) return kNotReady;
+  Solver& Sv = *static_cast<Solver*>(solver);
+  return pcg_multi(Sv, Bm->m, static_cast<Amg*>(amg), amg_vt, scale, nrhs, B, X, rtol, max_iter, check_every,
+                   Bm->tune, S(stream), iters, rel);
+}
+
+// Y = K X on nrhs (4 or 8) interleaved columns; prec 0 FP64, 1 FP32 values.
+int wnfea_bsr_spmm(void* h, int prec, int nrhs, const void* X, void* Y, void* stream) {
+  if (!h || !X || !Y) return kBadArg;
+  auto* b = static_cast<WnBsr*>(h);
+  return bsr_spmm(b->m, prec, nrhs, X, Y, b->tune, S(stream));
 }
 
 // r = f - K u (FP64 primary values) and out[0] = ||r||, out[1] = ||f||.

@@ -24,58 +24,9 @@
 #include <memory>
 #include <type_traits>
 #include <utility>
-#include "wn.hpp"
+#include "solver.hpp"
 
 namespace wn {
-
-struct AmgLevel {
-  SellMatrix* A = nullptr;  // not owned for level 0
-  bool own_A = false;
-  SellMatrix P, R;
-  bool has_PR = false;
-  int n = 0;  // scalar rows
-  double* dinv64 = nullptr;
-  float* dinv32 = nullptr;
-  double lmax = 2.0, ratio = 30.0;
-  int degree = 2;
-  // level vectors (b, x, tmp, d, res) per arithmetic precision
-  double* v64[5] = {};
-  float* v32[5] = {};
-};
-
-struct Amg {
-  std::vector<std::unique_ptr<AmgLevel>> L;
-  int store = 3;  // bit0 FP64, bit1 FP32, bit2 FP16 values
-  int n_coarse = 0;
-  double* inv = nullptr;  // n_coarse^2, row-major
-  SpmvTune tune;
-  ~Amg() { release(); }
-  void release() {
-    for (auto& lp : L) {
-      AmgLevel& l = *lp;
-      if (l.own_A && l.A) {
-        l.A->release();
-        delete l.A;
-      }
-      l.P.release();
-      l.R.release();
-      for (auto* p : l.v64) if (p) (void)hipFree(p);
-      for (auto* p : l.v32) if (p) (void)hipFree(p);
-      if (l.dinv64) (void)hipFree(l.dinv64);
-      if (l.dinv32) (void)hipFree(l.dinv32);
-    }
-    L.clear();
-    if (inv) (void)hipFree(inv);
-    inv = nullptr;
-  }
-};
-
-template <typename T> T** level_vecs(AmgLevel& l);
-template <> double** level_vecs<double>(AmgLevel& l) { return l.v64; }
-template <> float** level_vecs<float>(AmgLevel& l) { return l.v32; }
-template <typename T> T* level_dinv(AmgLevel& l);
-template <> double* level_dinv<double>(AmgLevel& l) { return l.dinv64; }
-template <> float* level_dinv<float>(AmgLevel& l) { return l.dinv32; }
 
 int amg_alloc_vectors(Amg& g, int prec_mask) {
   for (auto& lp : g.L) {
@@ -115,27 +66,6 @@ __global__ __launch_bounds__(256) void coarse_gemv_kernel(int n, const double* _
   if (threadIdx.x == 0) x[i] = T(s);
 }
 
-struct ChebCoef {
-  double c1[16], c2[16];
-};
-
-static ChebCoef cheb_coefs(double lmax, double ratio, int deg) {
-  ChebCoef c{};
-  const double lmin = lmax / ratio;
-  const double th = 0.5 * (lmax + lmin), de = 0.5 * (lmax - lmin), sig = th / de;
-  double rho = 1.0 / sig;
-  c.c1[0] = 0.0;
-  c.c2[0] = 1.0 / th;
-  for (int k = 1; k < deg && k < 16; ++k) {
-    const double rn = 1.0 / (2.0 * sig - rho);
-    c.c1[k] = rn * rho;
-    c.c2[k] = 2.0 * rn / de;
-    rho = rn;
-  }
-  return c;
-}
-
-// VT: values of this level; VTC: values of the levels below it.
 template <typename VT, typename VTC, typename T>
 static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t s) {
   AmgLevel& L = *g.L[l];
@@ -212,87 +142,6 @@ int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream
 }
 
 // ------------------------------------------------------------------ PCG
-template <typename T>
-struct KVec {
-  T *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
-  long long n = 0;
-  int ensure(long long n_) {
-    if (n_ <= n && r) return kOk;
-    release();
-    n = n_;
-    const size_t bytes = sizeof(T) * std::max<long long>(n, 1);
-    WN_HIP(hipMalloc(&r, bytes));
-    WN_HIP(hipMalloc(&z, bytes));
-    WN_HIP(hipMalloc(&p, bytes));
-    WN_HIP(hipMalloc(&q, bytes));
-    return kOk;
-  }
-  void release() {
-    for (T* v : {r, z, p, q})
-      if (v) (void)hipFree(v);
-    r = z = p = q = nullptr;
-    n = 0;
-  }
-};
-
-struct Solver {
-  Workspace ws;
-  int* flags = nullptr;  // [0] done, [1] iterations
-  int* h_flags = nullptr;
-  double* h_sc = nullptr;
-  KVec<double> k64;
-  KVec<float> k32;
-  // preconditioner input/output when it runs in FP32 under an FP64 Krylov loop
-  float* rp32 = nullptr;
-  float* zp32 = nullptr;
-  long long n_p = 0;
-  double* r64 = nullptr;  // FP64 residual of the refinement loop
-  float* rhs32 = nullptr;
-  float* d32 = nullptr;
-  long long n_ir = 0;
-  // Replayable chunk of PCG iterations (HIP graph) and the key of everything
-  // its kernel arguments were taken from: a solve with the same key replays
-  // it, any other solve captures a new one.
-  int use_graphs = 1;
-  hipStream_t cap = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  std::vector<uint64_t> gkey;
-  void drop_graph() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    gexec = nullptr;
-    gkey.clear();
-  }
-  int init() {
-    if (flags) return kOk;
-    WN_HIP(hipMalloc(&flags, sizeof(int) * 4));
-    WN_HIP(hipMemset(flags, 0, sizeof(int) * 4));
-    WN_HIP(hipHostMalloc(&h_flags, sizeof(int) * 4, 0));
-    WN_HIP(hipHostMalloc(&h_sc, sizeof(double) * kScCount, 0));
-    return ws.alloc(4096);
-  }
-  int ensure_p(long long n) {
-    if (n <= n_p && rp32) return kOk;
-    if (rp32) (void)hipFree(rp32);
-    if (zp32) (void)hipFree(zp32);
-    WN_HIP(hipMalloc(&rp32, sizeof(float) * std::max<long long>(n, 1)));
-    WN_HIP(hipMalloc(&zp32, sizeof(float) * std::max<long long>(n, 1)));
-    n_p = n;
-    return kOk;
-  }
-  ~Solver() {
-    drop_graph();
-    if (cap) (void)hipStreamDestroy(cap);
-    ws.release();
-    k64.release();
-    k32.release();
-    if (flags) (void)hipFree(flags);
-    if (h_flags) (void)hipHostFree(h_flags);
-    if (h_sc) (void)hipHostFree(h_sc);
-    for (void* p : {(void*)r64, (void*)rhs32, (void*)d32, (void*)rp32, (void*)zp32})
-      if (p) (void)hipFree(p);
-  }
-};
-
 __global__ void pcg_start_kernel(double* sc, int* flags, double rtol2) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     sc[kScRR0] = sc[kScRR];

@@ -248,3 +248,31 @@ def generate_c3d10_ibeam(
         "build_time_sec": time.perf_counter() - t0,
     }
     return model, meta
+
+
+def ibeam_tip_load_cases(coords: np.ndarray, tip_nodes, n_cases: int = 32, seed: int = 0,
+                         force_sigma: float = 20e3, torque_sigma: float = 2e3, centroid=(0.0, 0.15)) -> np.ndarray:
+    """Seeded tip load cases on the I-beam, (3 N, n_cases): per case a resultant
+    force F ~ N(0, force_sigma) per component split equally over the tip
+    nodes, and a moment M ~ N(0, torque_sigma) per component about the section
+    centroid: M_x as the torsion couple f = M_x (e_x x r) / sum |r|^2, M_y and
+    M_z as axial forces linear over the section, f_x = M_y z' / sum z'^2 and
+    f_x = -M_z y' / sum y'^2 (y', z' measured from the centroid).  Each part
+    has exactly the stated resultant."""
+    rng = np.random.default_rng(seed)
+    tip = np.asarray(tip_nodes, dtype=np.int64)
+    ry = coords[tip, 1] - centroid[0]
+    rz = coords[tip, 2] - centroid[1]
+    s_yz, s_y, s_z = float(np.sum(ry * ry + rz * rz)), float(np.sum(ry * ry)), float(np.sum(rz * rz))
+    out = np.zeros((3 * len(coords), n_cases))
+    for k in range(n_cases):
+        F = rng.normal(0.0, force_sigma, 3)
+        M = rng.normal(0.0, torque_sigma, 3)
+        f = np.tile(F / len(tip), (len(tip), 1))
+        f[:, 1] += M[0] * (-rz) / s_yz
+        f[:, 2] += M[0] * ry / s_yz
+        f[:, 0] += M[1] * rz / s_z - M[2] * ry / s_y
+        out[3 * tip, k] = f[:, 0]
+        out[3 * tip + 1, k] = f[:, 1]
+        out[3 * tip + 2, k] = f[:, 2]
+    return out
