@@ -213,9 +213,12 @@ def run_adjoint(args, log) -> int:
             Ep[e] += sgn * h
             vals.append(prob.compliance_and_grad(Ep)[0])
         fd = (vals[0] - vals[1]) / (2 * h)
-        fd_err.append(abs(fd - gCh[e]) / abs(gCh[e]))
+        fd_err.append((abs(fd - gCh[e]) / abs(gCh[e]), abs(fd - gCh[e]) / np.abs(gCh).max(), abs(gCh[e])))
     prob.close()
-    log(f"[bench] gradient vs CPU FP64: compliance {errC:.2e}, tip {errT:.2e}; FD max rel err {max(fd_err):.2e}")
+    fd_rel = [a for a, _, _ in fd_err]
+    fd_scaled = [b for _, b, _ in fd_err]
+    log(f"[bench] gradient vs CPU FP64: compliance {errC:.2e}, tip {errT:.2e}; FD error / max|g| "
+        f"{max(fd_scaled):.2e}, per-element relative median {np.median(fd_rel):.2e} max {max(fd_rel):.2e}")
     print(json.dumps({
         "metric": "compliance + tip-displacement adjoint gradients, C3D10 I-beam, per-element E",
         "value": round(t_c + t_t, 5), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -226,7 +229,11 @@ def run_adjoint(args, log) -> int:
                    "seed_E": 2, "seed_fd": 3, "parallelism": "single-gpu"},
         "compliance_seconds": round(t_c, 5), "tip_seconds": round(t_t, 5), "setup_seconds": round(t_setup, 3),
         "grad_rel_err_vs_cpu_fp64": {"compliance": errC, "tip": errT},
-        "fd_rel_err_max_20_elements": float(max(fd_err)), "cpu_reference_gradient_seconds": round(t_cpu, 3),
+        "fd_20_elements": {"max_err_over_max_abs_grad": float(max(fd_scaled)),
+                           "median_rel_err": float(np.median(fd_rel)), "max_rel_err": float(max(fd_rel)),
+                           "note": "per-element relative error grows where |g_e| is small: the FD difference "
+                                   "of C is then near FP64 round-off of C"},
+        "cpu_reference_gradient_seconds": round(t_cpu, 3),
         "solves": prob.solves}), flush=True)
     return 0
 
@@ -370,7 +377,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
-    ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine"], default="fp32")
+    ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine", "fp16_coarse"], default="fp32")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV launches (and a 256 MiB copy for counter calibration), for rocprofv3 --pmc")
@@ -462,7 +469,7 @@ def main() -> int:
     log(f"[bench] fp64 AMG-PCG: {t_fp64 * 1e3:.1f} ms/solve iters {it64} rel {rel64:.2e}")
     err_u = float(torch.linalg.norm(u - x64) / torch.linalg.norm(x64))
     others = {}
-    for other in ("fp32", "fp16", "fp16_fine"):
+    for other in ("fp32", "fp16", "fp16_fine", "fp16_coarse"):
         if other == args.amg_precision:
             continue
         solve_mixed(system, amg, tol=1e-8, amg_precision=other)

@@ -262,3 +262,11 @@ def test_ibeam_tip_load_cases_resultants():
         r = X - np.array([0.0, 0.0, 0.15])
         r[:, 0] = 0.0  # moments about the tip-section centroid
         assert np.allclose(np.cross(r, f).sum(axis=0), M, rtol=0, atol=1e-6)
+
+
+def test_precision_config_presets():
+    from wnfea_amd.solver import mixed_precision as mp
+    assert mp.MIXED_FP32.amg_precision == "fp32" and mp.FULL_FP64.amg_precision == "fp64"
+    assert mp.TRI_PRECISION.use_tri_precision and mp.TRI_PRECISION.switch_tol == 1e-2
+    assert "Tri-precision" in mp.TRI_PRECISION.summary() and "Uniform" in mp.FULL_FP64.summary()
+    assert abs(mp.MIXED_FP32.inner_eps_scale - np.sqrt(np.finfo(np.float32).eps)) == 0.0

@@ -210,3 +210,14 @@ def test_spmm_matches_spmv(c0_system):
     for k in range(8):
         y = c0_system.K.spmv(X[:, k].contiguous())
         assert float(torch.linalg.norm(Y[:, k] - y) / torch.linalg.norm(y)) <= 1e-14
+
+
+@pytest.mark.parametrize("policy", ["MIXED_FP32", "TRI_PRECISION", "FULL_FP64"])
+def test_precision_policies(c0_system, c0_amg, c0_oracle, policy):
+    """The reference's presets (mixed_precision.py:96-122) on the device
+    solver: FP64-verified 1e-8 and the displacement bars."""
+    from wnfea_amd.solver import mixed_precision as mp
+    _, _, u_ref = c0_oracle
+    u, it, rel, _ = mp.solve_with_policy(c0_system, c0_amg, getattr(mp, policy), tol=1e-8)
+    assert rel <= 1e-8
+    assert _rel(u.cpu().numpy(), u_ref) <= 1e-6

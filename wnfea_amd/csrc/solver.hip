@@ -117,8 +117,9 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
 }
 
 // One V-cycle, z = M r.  vt: 0 FP64 values and vectors, 1 FP32 values, 2 FP16
-// values on every level, 3 FP16 values on the finest level and FP32 below it
-// (vt >= 1: FP32 vectors and arithmetic).
+// values on every level, 3 FP16 values on the finest level and FP32 below it,
+// 4 FP32 on the finest level and FP16 below it (vt >= 1: FP32 vectors and
+// arithmetic).
 int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s) {
   if (g.L.empty()) return kNotReady;
   const float* r32 = static_cast<const float*>(r);
@@ -136,6 +137,9 @@ int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream
     case 3:
       if ((g.store & 6) != 6) return kNotReady;
       return vcycle<_Float16, float, float>(g, 0, r32, z32, skip, s);
+    case 4:
+      if ((g.store & 6) != 6) return kNotReady;
+      return vcycle<float, _Float16, float>(g, 0, r32, z32, skip, s);
     default:
       return kBadMode;
   }

@@ -305,7 +305,7 @@ def _as_system(K, F, want_fp64_equilibrated=False) -> SolidSystem:
     return system_from_csr(K, F, want_fp64_equilibrated=want_fp64_equilibrated)
 
 
-_VT = {"fp64": 0, "fp32": 1, "fp16": 2, "fp16_fine": 3}
+_VT = {"fp64": 0, "fp32": 1, "fp16": 2, "fp16_fine": 3, "fp16_coarse": 4}
 
 
 def pcg_device(system: SolidSystem, b=None, rtol: float = 1e-8, max_iter: int = 2000, preconditioner: str = "jacobi",
@@ -388,7 +388,7 @@ def solve_mixed(system: SolidSystem, amg, tol: float = 1e-8, amg_precision: str 
 
 
 def solve_multi(system: SolidSystem, amg, B, tol: float = 1e-8, group: int = 8, max_iter: int = 2000,
-                check_every: int = 16, margin: float = 0.5, max_restarts: int = 3):
+                check_every: int = 16, margin: float = 0.5, max_restarts: int = 6):
     """Many load cases at once: every column of B (device FP64, n_dofs x k) to
     ||b_k - K u_k|| <= tol ||b_k|| (FP64-verified per column).
 
@@ -396,7 +396,8 @@ def solve_multi(system: SolidSystem, amg, B, tol: float = 1e-8, group: int = 8, 
     SpMM per iteration reads K once for the whole group, and the V-cycle runs
     on all columns of the group at once (FP64 CG per column, FP32 SA-AMG on
     S K S).  A column whose true residual misses tol after the run is
-    restarted on its true residual.  Returns (U (n x k), iterations per
+    restarted on its true residual, until it meets tol or a restart no longer
+    halves it (the FP64 residual floor).  Returns (U (n x k), iterations per
     column, relative residual per column, seconds)."""
     import torch
     L = _lib.load()
@@ -422,6 +423,7 @@ def solve_multi(system: SolidSystem, amg, B, tol: float = 1e-8, group: int = 8, 
         Xg = torch.zeros_like(Bg)
         bn = torch.linalg.norm(Bg, dim=0)
         rhs = Bg
+        r_prev = None
         for step in range(max_restarts + 1):
             dX = torch.empty_like(Bg)
             # relative target per column scaled to the original right-hand side
@@ -439,9 +441,13 @@ def solve_multi(system: SolidSystem, amg, B, tol: float = 1e-8, group: int = 8, 
                        "wnfea_bsr_spmm")
             R = Bg - KX
             r = torch.linalg.norm(R, dim=0) / torch.where(bn > 0, bn, torch.ones_like(bn))
-            if bool((r[:m] <= tol).all()) or step == max_restarts:
+            todo = r > tol
+            if r_prev is not None:
+                todo &= r <= 0.5 * r_prev  # still improving: not yet at the FP64 floor
+            if not bool(todo[:m].any()) or step == max_restarts:
                 break
-            rhs = torch.where((r > tol)[None, :], R, torch.zeros_like(R)).contiguous()
+            r_prev = r
+            rhs = torch.where(todo[None, :], R, torch.zeros_like(R)).contiguous()
         rels[g0:g0 + m] = r[:m].cpu().numpy()
         U[:, g0:g0 + m] = Xg[:, :m]
     _sync()
