@@ -179,9 +179,10 @@ static int launch_spmm(const SellMatrix& A, const XT* x, const Op& op, const Spm
   const SellView<VT> v = A.view<VT>();
   if (A.n_slices == 0) return kOk;
   const int grid = t.xcd_remap ? 8 * ((A.n_slices + 7) / 8) : A.n_slices;
-  // two waves per slice for 3x3 blocks; the 6x6 accumulators leave no LDS for more
+  // four waves per slice where the partial accumulators fit in LDS (3-row
+  // blocks), one for the 6-row blocks
   if constexpr (BR * NR <= 32) {
-    hipLaunchKernelGGL((sell_spmm_kernel<BR, BC, VT, XT, 2, NR, Op>), dim3(grid), dim3(128), 0, s, v, x, op,
+    hipLaunchKernelGGL((sell_spmm_kernel<BR, BC, VT, XT, 4, NR, Op>), dim3(grid), dim3(256), 0, s, v, x, op,
                        t.xcd_remap, t.flags, t.guard);
   } else {
     hipLaunchKernelGGL((sell_spmm_kernel<BR, BC, VT, XT, 1, NR, Op>), dim3(grid), dim3(64), 0, s, v, x, op,
@@ -421,25 +422,38 @@ static int vcycle_mr(Amg& g, MultiState& M, int l, const T* b, T* x, const int* 
 }
 
 // ---------------------------------------------------------------- PCG on NR columns
+// The vector kernels walk the flat n*NR arrays (fully coalesced); with a
+// grid stride that is a multiple of NR every thread stays on one column
+// k = t mod NR, and the per-column sums meet in a shuffle over the lanes of
+// equal k, then across the waves of the block (fixed order).
+template <int NR>
+__device__ __forceinline__ void store_column_partials(double v, double* partials) {
+  static_assert(NR <= 64 && (NR & (NR - 1)) == 0, "NR must be a power of two <= 64");
+#pragma unroll
+  for (int off = 32; off >= NR; off >>= 1) v += __shfl_xor(v, off, 64);
+  __shared__ double sm[kMrBlock / 64][NR];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane < NR) sm[wid][lane] = v;
+  __syncthreads();
+  if (threadIdx.x < NR) {
+    double t = 0.0;
+    for (int w = 0; w < kMrBlock / 64; ++w) t += sm[w][threadIdx.x];
+    partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+  }
+}
+
 // partial dots of r_k . (S z_k) (z in ZT, S per DOF)
 template <typename ZT, int NR>
 __global__ __launch_bounds__(kMrBlock) void mr_rho_kernel(long long n, const double* __restrict__ r,
                                                           const ZT* __restrict__ z, const double* __restrict__ scale,
                                                           double* partials, const int* skip) {
   if (skip && skip[0]) return;
-  double acc[NR];
-#pragma unroll
-  for (int k = 0; k < NR; ++k) acc[k] = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const double si = scale ? scale[i] : 1.0;
-#pragma unroll
-    for (int k = 0; k < NR; ++k) acc[k] += r[i * NR + k] * double(z[i * NR + k]) * si;
-  }
-  __shared__ double sm[kMrBlock / 64];
-  double v[NR];
-#pragma unroll
-  for (int k = 0; k < NR; ++k) v[k] = block_sum(acc[k], sm);
-  store_block_partials<NR>(v, partials);
+  double acc = 0.0;
+  const long long total = n * NR;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x)
+    acc += r[t] * double(z[t]) * (scale ? scale[t / NR] : 1.0);
+  store_column_partials<NR>(acc, partials);
 }
 
 // p_k = S z_k + beta_k p_k (beta 0 on the first iteration or for a done column)
@@ -470,29 +484,18 @@ __global__ __launch_bounds__(kMrBlock) void mr_update_xr_kernel(long long n, dou
                                                                 const double* __restrict__ scale, double* partials,
                                                                 const double* __restrict__ sc, const int* flags) {
   if (flags[0]) return;
-  double alpha[NR], acc[NR];
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    alpha[k] = sc[kMsAlpha * NR + k];
-    acc[k] = 0.0;
+  const double alpha = sc[kMsAlpha * NR + (threadIdx.x & (NR - 1))];
+  double acc = 0.0;
+  const long long total = n * NR;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    x[t] += alpha * p[t];
+    const double ri = r[t] - alpha * q[t];
+    r[t] = ri;
+    if (rp) rp[t] = float((scale ? scale[t / NR] : 1.0) * ri);
+    acc += ri * ri;
   }
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const double si = scale ? scale[i] : 1.0;
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      const long long o = i * NR + k;
-      x[o] += alpha[k] * p[o];
-      const double ri = r[o] - alpha[k] * q[o];
-      r[o] = ri;
-      if (rp) rp[o] = float(si * ri);
-      acc[k] += ri * ri;
-    }
-  }
-  __shared__ double sm[kMrBlock / 64];
-  double v[NR];
-#pragma unroll
-  for (int k = 0; k < NR; ++k) v[k] = block_sum(acc[k], sm);
-  store_block_partials<NR>(v, partials);
+  store_column_partials<NR>(acc, partials);
 }
 
 enum MrFin { kMrRho = 0, kMrGamma = 1, kMrRR = 2, kMrStart = 3 };
@@ -561,32 +564,23 @@ static int mr_fin(MultiState& M, int nb, int mode, double rtol2, hipStream_t s) 
   return static_cast<int>(hipGetLastError());
 }
 
-// ||b_k||^2 partials and r = b, rp = S r (FP32)
+// ||b_k||^2 partials and r = b, x = 0, rp = S r (FP32)
 template <int NR>
 __global__ __launch_bounds__(kMrBlock) void mr_start_kernel(long long n, const double* __restrict__ b,
                                                             double* __restrict__ x, double* __restrict__ r,
                                                             float* __restrict__ rp, const double* __restrict__ scale,
                                                             double* partials) {
-  double acc[NR];
-#pragma unroll
-  for (int k = 0; k < NR; ++k) acc[k] = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const double si = scale ? scale[i] : 1.0;
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      const long long o = i * NR + k;
-      const double bi = b[o];
-      x[o] = 0.0;
-      r[o] = bi;
-      if (rp) rp[o] = float(si * bi);
-      acc[k] += bi * bi;
-    }
+  double acc = 0.0;
+  const long long total = n * NR;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const double bi = b[t];
+    x[t] = 0.0;
+    r[t] = bi;
+    if (rp) rp[t] = float((scale ? scale[t / NR] : 1.0) * bi);
+    acc += bi * bi;
   }
-  __shared__ double sm[kMrBlock / 64];
-  double v[NR];
-#pragma unroll
-  for (int k = 0; k < NR; ++k) v[k] = block_sum(acc[k], sm);
-  store_block_partials<NR>(v, partials);
+  store_column_partials<NR>(acc, partials);
 }
 
 // q = A p with gamma partials (FP64)
@@ -620,7 +614,7 @@ static int pcg_multi_run(Solver& S, const SellMatrix& A, Amg* g, int vt, const d
   t.flags = M.flags;
   t.guard = 1;
   if (check_every <= 0) check_every = 8;
-  const int gv = mr_grid(n);
+  const int gv = mr_grid(n * NR);
   hipLaunchKernelGGL((mr_start_kernel<NR>), dim3(gv), dim3(kMrBlock), 0, s, n, B, X, M.r, M.rp32, scale, M.partials);
   if ((r = static_cast<int>(hipGetLastError()))) return r;
   if ((r = mr_fin<NR>(M, gv, kMrStart, rtol * rtol, s))) return r;

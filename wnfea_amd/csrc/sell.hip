@@ -53,12 +53,16 @@ SellMatrix& SellMatrix::equilibrated() {
 }
 
 // Lanes per block row: enough lanes that short matrices with long rows still
-// spread over many wavefronts (about 2048 x 64 lane-rows), but no more than
-// an eighth of the average row length, so that every lane has work.
+// spread over many wavefronts (about 2048 x 64 lane-rows), and in any case
+// as many as keep about six blocks per lane (more row splitting shortens the
+// dependent load chain of each lane: the fine C3D10 operator with ~25 blocks
+// per row runs 15-20 % faster with 4 lanes per row than with 1), but no more
+// than an eighth of the average row length for the first rule.
 static int auto_lanes(int n_rows, long long nnzb) {
   const double avg = n_rows > 0 ? double(nnzb) / n_rows : 0.0;
   int L = 1;
   while (L < 64 && (long long)n_rows * L < 64LL * 2048 && L * 8 < avg) L *= 2;
+  while (L < 64 && 2 * L * 6 <= avg) L *= 2;
   return L;
 }
 

@@ -123,7 +123,7 @@ def mesh_to_device(model: FEAModel, dev=None) -> dict:
             "E_host": E, "nu_host": nu}
 
 
-def assemble_c3d10_bsr(model: FEAModel, apply_bcs: bool = True, sigma: int = 4096,
+def assemble_c3d10_bsr(model: FEAModel, apply_bcs: bool = True, sigma: int = 16384,
                        want_fp64_equilibrated: bool = False, mesh: dict | None = None,
                        keep_plan: bool = False) -> SolidSystem:
     """Assemble K (3x3 BSR on the device), F and the Dirichlet data of a C3D10 model.
