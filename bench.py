@@ -38,6 +38,24 @@ def spmv_bytes(nnzb: int, nb: int, b: int, sv: int, sx: int) -> int:
     return nnzb * (b * b * sv + 4) + 4 * (nb + 1) + 2 * b * nb * sx
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sell_spmv_ibeam_1.05M_hbm_traffic.md")
+
+
+def measured_traffic(precision: str):
+    """HBM bytes per launch of the I-beam SpMV from the committed rocprofv3
+    --pmc FETCH_SIZE / WRITE_SIZE summary (counters scaled by a 256 MiB copy
+    kernel), or None when that summary is absent."""
+    import re
+    try:
+        txt = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s+fp32=(\d+)\s+fp64=(\d+)", txt, re.M)
+    if not m:
+        return None
+    return int(m.group(1) if precision == "fp32" else m.group(2))
+
+
 def time_spmv(K, precision: str, reps: int) -> float:
     """Average device time (s) of one SpMV launch, HIP events on the launch stream."""
     import torch
@@ -377,7 +395,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
-    ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine", "fp16_coarse", "fp16_smooth"], default="fp32")
+    ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine", "fp16_coarse"], default="fp32")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV launches (and a 256 MiB copy for counter calibration), for rocprofv3 --pmc")
@@ -469,7 +487,7 @@ def main() -> int:
     log(f"[bench] fp64 AMG-PCG: {t_fp64 * 1e3:.1f} ms/solve iters {it64} rel {rel64:.2e}")
     err_u = float(torch.linalg.norm(u - x64) / torch.linalg.norm(x64))
     others = {}
-    for other in ("fp32", "fp16", "fp16_fine", "fp16_coarse", "fp16_smooth"):
+    for other in ("fp32", "fp16", "fp16_fine", "fp16_coarse"):
         if other == args.amg_precision:
             continue
         solve_mixed(system, amg, tol=1e-8, amg_precision=other)
@@ -551,8 +569,10 @@ def main() -> int:
         "spmv_gbs": {"fp32": round(gbs32, 1), "fp64": round(gbs64, 1), "fp32_us": round(t32 * 1e6, 2),
                      "fp64_us": round(t64 * 1e6, 2), "bytes_fp32": b32, "bytes_fp64": b64},
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "sell_spmv_kernel<3,3,float,float,1,OpAxpby> (FP32 3x3 BSR SpMV)"},
+                     "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": measured_traffic("fp32"),
+                     "algorithmic_bytes": b32,
+                     "kernel": f"sell_spmv_kernel<3,3,float,float,{tune['waves']},{tune['unroll']},OpAxpby> "
+                               f"(FP32 3x3 BSR SpMV, {info['lanes']} lanes per block row)"},
         "spmv_tune": tune,
         "setup": {"mesh_sec": round(t_mesh, 2), "rcm_sec": round(t_rcm, 2), "assembly_sec": round(t_asm, 2),
                   "amg_setup_sec": round(t_amg, 2), "amg": st},

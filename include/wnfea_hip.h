@@ -238,9 +238,8 @@ int wnfea_amg_finalize(void *h, int waves, int xcd_remap, int unroll);
 
 /* z = M r, one V-cycle (device vectors).  vt: 0 FP64 values and vectors;
  * 1 FP32 values, 2 FP16 values on every level, 3 FP16 values on the finest
- * level and FP32 below, 4 FP32 on the finest level and FP16 below, 5 FP16 in
- * the finest level's smoother only (its defect, transfer operators and the
- * coarser levels FP32) (vt >= 1: FP32 vectors and arithmetic). */
+ * level and FP32 below, 4 FP32 on the finest level and FP16 below (vt >= 1:
+ * FP32 vectors and arithmetic). */
 int wnfea_amg_vcycle(void *h, int vt, const void *r, void *z, void *stream);
 
 /* ---------------------------------------------------------------- solvers */

@@ -43,7 +43,7 @@ def test_amg_pcg_fp64(c0_system, c0_amg, c0_oracle, precision):
     assert _rel(x.cpu().numpy(), u_ref) <= 1e-8
 
 
-@pytest.mark.parametrize("amg_precision", ["fp32", "fp16", "fp16_fine", "fp16_coarse", "fp16_smooth"])
+@pytest.mark.parametrize("amg_precision", ["fp32", "fp16", "fp16_fine", "fp16_coarse"])
 def test_mixed_precision_preconditioner(c0_system, c0_amg, c0_oracle, amg_precision):
     """FP64 CG with an FP32/FP16 V-cycle: FP64-verified 1e-8 and the 1e-6 mixed-mode displacement bar."""
     from wnfea_amd.solver.solid_solver import pcg_device, solve_mixed, true_residual

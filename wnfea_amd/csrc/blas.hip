@@ -138,15 +138,26 @@ __global__ void normalise_kernel(long long n, const T* __restrict__ x, T* __rest
 }
 
 // One-block finalisation of a two-launch reduction (see common.hpp).
-__global__ __launch_bounds__(256) void dot_finalize_kernel(const double* __restrict__ partials, int nb, int K, int mode,
-                                                           double* out, double* sc, int* flags, int first,
-                                                           const int* skip) {
+__global__ __launch_bounds__(1024) void dot_finalize_kernel(const double* __restrict__ partials, int nb, int K,
+                                                            int mode, double* out, double* sc, int* flags, int first,
+                                                            const int* skip) {
   if (skip && *skip) return;
-  __shared__ double red[2][4];
+  __shared__ double red[2][16];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int k = 0; k < K; ++k) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partials[(size_t)k * nb + i];
+    // four independent accumulators keep several loads in flight per thread
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const double* p = partials + (size_t)k * nb;
+    const int stride = blockDim.x;
+    int i = threadIdx.x;
+    for (; i + 3 * stride < nb; i += 4 * stride) {
+      a0 += p[i];
+      a1 += p[i + stride];
+      a2 += p[i + 2 * stride];
+      a3 += p[i + 3 * stride];
+    }
+    for (; i < nb; i += stride) a0 += p[i];
+    double s = (a0 + a1) + (a2 + a3);
     s = wave_sum(s);
     if (lane == 0) red[k][wid] = s;
   }
@@ -182,8 +193,8 @@ __global__ __launch_bounds__(256) void dot_finalize_kernel(const double* __restr
 
 int launch_finalize(const double* partials, int nb, int K, int mode, double* out, double* sc, int* flags, int first,
                     const int* skip, hipStream_t s) {
-  hipLaunchKernelGGL(dot_finalize_kernel, dim3(1), dim3(256), 0, s, partials, nb, K, mode, out, sc, flags, first,
-                     skip);
+  hipLaunchKernelGGL(dot_finalize_kernel, dim3(1), dim3(nb > 4096 ? 1024 : 256), 0, s, partials, nb, K, mode, out, sc,
+                     flags, first, skip);
   return static_cast<int>(hipGetLastError());
 }
 

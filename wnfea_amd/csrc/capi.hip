@@ -353,8 +353,7 @@ int wnfea_amg_finalize(void* h, int waves, int xcd_remap, int unroll) {
 }
 
 // z = M r (one V-cycle); vt 0 = FP64, 1 = FP32, 2 = FP16 values, 3 = FP16 on the
-// finest level and FP32 below, 4 = FP32 finest and FP16 below, 5 = FP16 in the finest
-// smoother only (vt >= 1: FP32 vectors)
+// finest level and FP32 below, 4 = FP32 finest and FP16 below (vt >= 1: FP32 vectors)
 int wnfea_amg_vcycle(void* h, int vt, const void* r, void* z, void* stream) {
   if (!h || !r || !z) return kBadArg;
   return amg_apply(*static_cast<Amg*>(h), vt, r, z, nullptr, S(stream));
@@ -382,7 +381,7 @@ int wnfea_solver_set_graphs(void* h, int enable) {
 int wnfea_pcg(void* solver, void* bsr, int prec, int pkind, const void* dinv, void* amg, int amg_vt,
               const double* scale, const void* b, void* x, double rtol, int max_iter, int check_every, int* iters,
               double* rel, void* stream) {
-  if (!solver || !bsr || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2 || amg_vt < 0 || amg_vt > 5)
+  if (!solver || !bsr || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2 || amg_vt < 0 || amg_vt > 4)
     return kBadArg;
   if ((pkind == 1 && !dinv) || (pkind == 2 && !amg)) return kBadArg;
   auto* B = static_cast<WnBsr*>(bsr);
@@ -406,7 +405,7 @@ int wnfea_ir(void* solver, void* bsr, const double* scale, int pkind, const floa
              const double* f, double* u, double tol, double inner_rtol, int max_outer, int max_inner,
              int check_every, int zero_init, int* outer, int* inner, double* rel, void* stream) {
   if (!solver || !bsr || !scale || !f || !u || pkind < 0 || pkind > 2) return kBadArg;
-  if ((pkind == 1 && !dinv32) || (pkind == 2 && (!amg || amg_vt < 1 || amg_vt > 5))) return kBadArg;
+  if ((pkind == 1 && !dinv32) || (pkind == 2 && (!amg || amg_vt < 1 || amg_vt > 4))) return kBadArg;
   auto* B = static_cast<WnBsr*>(bsr);
   return ir_run(*static_cast<Solver*>(solver), B->m, scale, pkind, dinv32, static_cast<Amg*>(amg), amg_vt, f, u, tol,
                 inner_rtol, max_outer, max_inner, check_every, B->tune, S(stream), outer, inner, rel, zero_init);

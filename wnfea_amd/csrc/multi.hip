@@ -84,7 +84,9 @@ __global__ __launch_bounds__(64 * W) void sell_spmm_kernel(SellView<VT> A, const
 #pragma unroll
       for (int t = 0; t < BR * NR; ++t) acc[t] += __shfl_xor(acc[t], off, 64);
     const int row = A.lane_row[slot_lane];
-    if (wave == 0 && row >= 0 && (lane & (A.lanes - 1)) == 0) op.row(row, acc, dot);
+    // every lane of the row holds the full sums after the butterfly: the row's
+    // L lanes share the NR columns of the epilogue (column k on lane k mod L)
+    if (wave == 0 && row >= 0) op.row(row, acc, dot, lane & (A.lanes - 1), A.lanes);
   }
   if constexpr (Op::kDots > 0) {
     __shared__ double sred[W];
@@ -102,11 +104,12 @@ struct MOpStore {  // y = A x, or y += A x with add
   YT* y;
   int add;
   double* partials = nullptr;
-  __device__ __forceinline__ void row(int i, const AT (&acc)[BR * NR], double*) const {
+  __device__ __forceinline__ void row(int i, const AT (&acc)[BR * NR], double*, int q, int L) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r)
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
+        if ((k & (L - 1)) != q) continue;
         const long long o = ((long long)i * BR + r) * NR + k;
         y[o] = add ? YT(AT(y[o]) + acc[r * NR + k]) : YT(acc[r * NR + k]);
       }
@@ -119,11 +122,12 @@ struct MOpSpmvDot {  // q = A p, gamma_k = p_k . q_k
   T* y;
   const T* p;
   double* partials;
-  __device__ __forceinline__ void row(int i, const T (&acc)[BR * NR], double* d) const {
+  __device__ __forceinline__ void row(int i, const T (&acc)[BR * NR], double* d, int q, int L) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r)
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
+        if ((k & (L - 1)) != q) continue;
         const long long o = ((long long)i * BR + r) * NR + k;
         y[o] = acc[r * NR + k];
         d[k] += double(p[o]) * double(acc[r * NR + k]);
@@ -141,12 +145,13 @@ struct MOpCheb {  // res = b - A x_in; d = c1 d + c2 dinv res; x_out = x_in + d
   T* d;
   T c1, c2;
   double* partials = nullptr;
-  __device__ __forceinline__ void row(int i, const T (&acc)[BR * NR], double*) const {
+  __device__ __forceinline__ void row(int i, const T (&acc)[BR * NR], double*, int q, int L) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
       const T di = dinv[(long long)i * BR + r];
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
+        if ((k & (L - 1)) != q) continue;
         const long long o = ((long long)i * BR + r) * NR + k;
         const T dn = c1 * d[o] + c2 * di * (b[o] - acc[r * NR + k]);
         d[o] = dn;
@@ -162,11 +167,12 @@ struct MOpDefect {
   const T* b;
   T* res;
   double* partials = nullptr;
-  __device__ __forceinline__ void row(int i, const T (&acc)[BR * NR], double*) const {
+  __device__ __forceinline__ void row(int i, const T (&acc)[BR * NR], double*, int q, int L) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r)
 #pragma unroll
       for (int k = 0; k < NR; ++k) {
+        if ((k & (L - 1)) != q) continue;
         const long long o = ((long long)i * BR + r) * NR + k;
         res[o] = b[o] - acc[r * NR + k];
       }
@@ -503,14 +509,24 @@ enum MrFin { kMrRho = 0, kMrGamma = 1, kMrRR = 2, kMrStart = 3 };
 // One block: column sums of the partials in a fixed order, then the per-column
 // scalar updates of the CG recurrences.
 template <int NR>
-__global__ __launch_bounds__(256) void mr_finalize_kernel(const double* __restrict__ partials, int nb, int mode,
-                                                          double* sc, int* flags, double rtol2) {
+__global__ __launch_bounds__(1024) void mr_finalize_kernel(const double* __restrict__ partials, int nb, int mode,
+                                                           double* sc, int* flags, double rtol2) {
   if (mode != kMrStart && flags[0]) return;
-  __shared__ double red[NR][4];
+  __shared__ double red[NR][16];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int k = 0; k < NR; ++k) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partials[(size_t)k * nb + i];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const double* p = partials + (size_t)k * nb;
+    const int stride = blockDim.x;
+    int i = threadIdx.x;
+    for (; i + 3 * stride < nb; i += 4 * stride) {
+      a0 += p[i];
+      a1 += p[i + stride];
+      a2 += p[i + 2 * stride];
+      a3 += p[i + 3 * stride];
+    }
+    for (; i < nb; i += stride) a0 += p[i];
+    double s = (a0 + a1) + (a2 + a3);
     s = wave_sum(s);
     if (lane == 0) red[k][wid] = s;
   }
@@ -560,7 +576,7 @@ __global__ __launch_bounds__(256) void mr_finalize_kernel(const double* __restri
 
 template <int NR>
 static int mr_fin(MultiState& M, int nb, int mode, double rtol2, hipStream_t s) {
-  hipLaunchKernelGGL((mr_finalize_kernel<NR>), dim3(1), dim3(256), 0, s, M.partials, nb, mode, M.sc, M.flags, rtol2);
+  hipLaunchKernelGGL((mr_finalize_kernel<NR>), dim3(1), dim3(1024), 0, s, M.partials, nb, mode, M.sc, M.flags, rtol2);
   return static_cast<int>(hipGetLastError());
 }
 

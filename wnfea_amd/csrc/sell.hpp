@@ -171,7 +171,9 @@ __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const
 #pragma unroll
       for (int r = 0; r < BR; ++r) acc[r] += __shfl_xor(acc[r], off, 64);
     const int row = A.lane_row[slot_lane];
-    if (wave == 0 && row >= 0 && (lane & (A.lanes - 1)) == 0) op.row(row, acc, dot);
+    // every lane of the row holds the full sums after the butterfly: the row's
+    // L lanes share its BR outputs (output r on lane r mod L)
+    if (wave == 0 && row >= 0) op.row(row, acc, dot, lane & (A.lanes - 1), A.lanes);
   }
   if constexpr (Op::kDots > 0) {
     __shared__ double sred[W];
@@ -190,9 +192,10 @@ struct OpAxpby {
   YT* y;
   AT alpha, beta;
   double* partials = nullptr;
-  __device__ __forceinline__ void row(int i, const AT (&acc)[BR], double (&)[2]) const {
+  __device__ __forceinline__ void row(int i, const AT (&acc)[BR], double (&)[2], int q, int L) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
+      if ((r & (L - 1)) != q) continue;
       const long long k = (long long)i * BR + r;
       AT v = alpha * acc[r];
       if (beta != AT(0)) v += beta * AT(y[k]);
@@ -208,9 +211,10 @@ struct OpSpmvDot {
   YT* y;
   const YT* p;        // same vector as the gathered x (square matrix)
   double* partials;
-  __device__ __forceinline__ void row(int i, const AT (&acc)[BR], double (&d)[2]) const {
+  __device__ __forceinline__ void row(int i, const AT (&acc)[BR], double (&d)[2], int q, int L) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
+      if ((r & (L - 1)) != q) continue;
       const long long k = (long long)i * BR + r;
       y[k] = YT(acc[r]);
       d[0] += double(p[k]) * double(acc[r]);
@@ -228,9 +232,10 @@ struct OpResidual {
   float* rs;            // may be null
   const double* scale;  // per-DOF scale (may be null = 1)
   double* partials;     // [0] ||r||^2, [1] ||scale r||^2 partials
-  __device__ __forceinline__ void row(int i, const AT (&acc)[BR], double (&d)[2]) const {
+  __device__ __forceinline__ void row(int i, const AT (&acc)[BR], double (&d)[2], int q, int L) const {
 #pragma unroll
     for (int r0 = 0; r0 < BR; ++r0) {
+      if ((r0 & (L - 1)) != q) continue;
       const long long k = (long long)i * BR + r0;
       const double v = b[k] - double(acc[r0]);
       const double vs = scale ? scale[k] * v : v;
@@ -254,9 +259,10 @@ struct OpCheb {
   T* d;
   T c1, c2;
   double* partials = nullptr;
-  __device__ __forceinline__ void row(int i, const T (&acc)[BR], double (&)[2]) const {
+  __device__ __forceinline__ void row(int i, const T (&acc)[BR], double (&)[2], int q, int L) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
+      if ((r & (L - 1)) != q) continue;
       const long long k = (long long)i * BR + r;
       const T res = b[k] - acc[r];
       const T dn = c1 * d[k] + c2 * dinv[k] * res;
@@ -273,9 +279,10 @@ struct OpDefect {
   const T* b;
   T* res;
   double* partials = nullptr;
-  __device__ __forceinline__ void row(int i, const T (&acc)[BR], double (&)[2]) const {
+  __device__ __forceinline__ void row(int i, const T (&acc)[BR], double (&)[2], int q, int L) const {
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
+      if ((r & (L - 1)) != q) continue;
       const long long k = (long long)i * BR + r;
       res[k] = b[k] - acc[r];
     }
@@ -288,9 +295,10 @@ struct OpAddTo {
   static constexpr int kDots = 0;
   T* x;
   double* partials = nullptr;
-  __device__ __forceinline__ void row(int i, const T (&acc)[BR], double (&)[2]) const {
+  __device__ __forceinline__ void row(int i, const T (&acc)[BR], double (&)[2], int q, int L) const {
 #pragma unroll
-    for (int r = 0; r < BR; ++r) x[(long long)i * BR + r] += acc[r];
+    for (int r = 0; r < BR; ++r)
+      if ((r & (L - 1)) == q) x[(long long)i * BR + r] += acc[r];
   }
 };
 

@@ -66,10 +66,9 @@ __global__ __launch_bounds__(256) void coarse_gemv_kernel(int n, const double* _
   if (threadIdx.x == 0) x[i] = T(s);
 }
 
-// One V-cycle from level l.  VT: values of this level's defect, restriction
-// and prolongation; VTS: values of this level's smoother (default VT); VTC:
-// values of every coarser level; T: vector and arithmetic type.
-template <typename VT, typename VTC, typename T, typename VTS = VT>
+// One V-cycle from level l.  VT: values of this level; VTC: values of every
+// coarser level; T: vector and arithmetic type.
+template <typename VT, typename VTC, typename T>
 static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t s) {
   AmgLevel& L = *g.L[l];
   SpmvTune t = g.tune;
@@ -101,7 +100,7 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   hipLaunchKernelGGL(cheb_first_kernel<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, b, dinv, d, cur, T(c.c2[0]), skip);
   if ((r = static_cast<int>(hipGetLastError()))) return r;
   for (int k = 1; k < deg; ++k) {
-    if ((r = spmv_cheb<VTS, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
+    if ((r = spmv_cheb<VT, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
     std::swap(cur, oth);
   }
   // defect and restriction
@@ -113,7 +112,7 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   if ((r = spmv_addto<VT, T>(L.P, nv[1], cur, t, s))) return r;  // cur == post here
   oth = (cur == x) ? tmp : x;
   for (int k = 0; k < deg; ++k) {
-    if ((r = spmv_cheb<VTS, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
+    if ((r = spmv_cheb<VT, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
     std::swap(cur, oth);
   }
   return kOk;
@@ -121,9 +120,10 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
 
 // One V-cycle, z = M r.  vt: 0 FP64 values and vectors, 1 FP32 values, 2 FP16
 // values on every level, 3 FP16 values on the finest level and FP32 below it,
-// 4 FP32 on the finest level and FP16 below it, 5 FP16 values in the finest
-// level's smoother with its defect, restriction, prolongation and the coarser
-// levels in FP32 (vt >= 1: FP32 vectors and arithmetic).
+// 4 FP32 on the finest level and FP16 below it (vt >= 1: FP32 vectors and
+// arithmetic).  (FP16 values in the finest smoother with an FP32 defect make
+// the cycle nonsymmetric by O(eps16) and CG then stalls, so that split is not
+// offered.)
 int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s) {
   if (g.L.empty()) return kNotReady;
   const float* r32 = static_cast<const float*>(r);
@@ -144,9 +144,7 @@ int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream
     case 4:
       if ((g.store & 6) != 6) return kNotReady;
       return vcycle<float, _Float16, float>(g, 0, r32, z32, skip, s);
-    case 5:
-      if ((g.store & 6) != 6) return kNotReady;
-      return vcycle<float, float, float, _Float16>(g, 0, r32, z32, skip, s);
+
     default:
       return kBadMode;
   }
