@@ -397,6 +397,8 @@ def main() -> int:
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine", "fp16_coarse"], default="fp32")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
+    ap.add_argument("--spmv-shape", default=None,
+                    help="waves,unroll,xcd_remap for the SpMV instead of autotuning (e.g. 4,1,0)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV launches (and a 256 MiB copy for counter calibration), for rocprofv3 --pmc")
     ap.add_argument("--ir", action="store_true", help="also time the restart-IR (FP32 Krylov) scheme")
@@ -444,7 +446,11 @@ def main() -> int:
     info = system.K.info()
     log(f"[bench] rcm {t_rcm:.1f}s assembly {t_asm:.1f}s nnzb {info['nnzb']} slots {info['slots']}")
     tune = {"waves": 1, "unroll": 1, "xcd_remap": True}
-    if not args.no_autotune:
+    if args.spmv_shape:
+        w, u, x = (int(v) for v in args.spmv_shape.split(","))
+        tune = {"waves": w, "unroll": u, "xcd_remap": bool(x)}
+        system.K.tune(w, bool(x), u)
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         tune = autotune_spmv(system.K, "fp32", use_cache=False)
         tune.pop("table", None)
