@@ -80,13 +80,14 @@ __global__ __launch_bounds__(64 * W) void sell_spmm_kernel(SellView<VT> A, const
           for (int t = 0; t < BR * NR; ++t) acc[t] += red[w][t][lane];
       }
     }
-    for (int off = A.lanes >> 1; off > 0; off >>= 1)
+    const int rps = 64 / A.lanes;  // rows per slice; a row's lanes are r, r + rps, ...
+    for (int off = 32; off >= rps; off >>= 1)
 #pragma unroll
       for (int t = 0; t < BR * NR; ++t) acc[t] += __shfl_xor(acc[t], off, 64);
     const int row = A.lane_row[slot_lane];
     // every lane of the row holds the full sums after the butterfly: the row's
-    // L lanes share the NR columns of the epilogue (column k on lane k mod L)
-    if (wave == 0 && row >= 0) op.row(row, acc, dot, lane & (A.lanes - 1), A.lanes);
+    // L lanes share the NR columns of the epilogue (column k on sub-lane k mod L)
+    if (wave == 0 && row >= 0) op.row(row, acc, dot, lane / rps, A.lanes);
   }
   if constexpr (Op::kDots > 0) {
     __shared__ double sred[W];

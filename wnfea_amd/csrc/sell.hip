@@ -104,7 +104,7 @@ int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* row
       const int g = order[k];
       const int rl = rowptr[g + 1] - rowptr[g];
       for (int q = 0; q < lanes; ++q) {
-        const int ln = s * 64 + r * lanes + q;
+        const int ln = s * 64 + q * rows_per_slice + r;
         row[ln] = g;
         len[ln] = rl > q ? (rl - q + lanes - 1) / lanes : 0;
         steps = std::max(steps, len[ln]);
@@ -121,7 +121,7 @@ int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* row
       if (k >= n_rows) break;
       const int g = order[k];
       for (int j = 0; j < rowptr[g + 1] - rowptr[g]; ++j) {
-        const long long sl = off[s] + 64LL * (j / lanes) + r * lanes + (j % lanes);
+        const long long sl = off[s] + 64LL * (j / lanes) + (j % lanes) * rows_per_slice + r;
         col[sl] = colidx[rowptr[g] + j];
         slot[rowptr[g] + j] = sl;
       }

@@ -1,13 +1,15 @@
 // Device layout of block-sparse matrices: sliced ELL over block rows.
 //
-// Each block row is owned by L consecutive lanes of a wavefront (L = 1 for
-// long matrices with short rows such as the fine 3x3 operator; L up to 64 for
-// short matrices with long rows such as restriction and coarse operators), so
-// a slice holds 64/L rows.  Rows are first sorted by block count inside
-// windows of `sigma` consecutive rows (stable, longest first), so the rows
-// sharing a slice have nearly equal lengths.  Block j of the row in slice
-// position r goes to lane l = r*L + j%L at step t = j/L; its slot is
-// off[s] + t*64 + l.  The BB entries of the 64 blocks of one step form a
+// Each block row is owned by L lanes of a wavefront (L from the shape: 4 for
+// the fine 3x3 operator, up to 64 for short matrices with long rows such as
+// restriction and coarse operators), so a slice holds R = 64/L rows.  Rows
+// are first sorted by block count inside windows of `sigma` consecutive rows
+// (stable, longest first), so the rows sharing a slice have nearly equal
+// lengths.  Block j of the row in slice position r goes to lane
+// l = (j mod L)*R + r at step t = j/L; its slot is off[s] + t*64 + l.  The
+// sub-lane index is the high part of the lane number, so in the last, partial
+// step of a slice the active lanes form one contiguous run and the fetched
+// cache lines carry no idle lanes' padding.  The BB entries of the 64 blocks of one step form a
 // region of 64*BB values starting at (off[s] + t*64)*BB, cut into chunks of
 // V = 16 bytes / sizeof(value) entries: chunk g of lane l holds entries
 // g*V .. g*V+V-1 (row-major inside the block) at region + g*64*V + l*V, and
@@ -166,14 +168,16 @@ __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const
           for (int r = 0; r < BR; ++r) acc[r] += red[w][r][lane];
       }
     }
-    // the L lanes of a row hold partial sums: butterfly over the low bits
-    for (int off = A.lanes >> 1; off > 0; off >>= 1)
+    // the L lanes of a row (lane numbers r, r + R, r + 2R, ...) hold partial
+    // sums: butterfly over the high bits of the lane number
+    const int rps = 64 / A.lanes;
+    for (int off = 32; off >= rps; off >>= 1)
 #pragma unroll
       for (int r = 0; r < BR; ++r) acc[r] += __shfl_xor(acc[r], off, 64);
     const int row = A.lane_row[slot_lane];
     // every lane of the row holds the full sums after the butterfly: the row's
-    // L lanes share its BR outputs (output r on lane r mod L)
-    if (wave == 0 && row >= 0) op.row(row, acc, dot, lane & (A.lanes - 1), A.lanes);
+    // L lanes share its BR outputs (output r on sub-lane r mod L)
+    if (wave == 0 && row >= 0) op.row(row, acc, dot, lane / rps, A.lanes);
   }
   if constexpr (Op::kDots > 0) {
     __shared__ double sred[W];
