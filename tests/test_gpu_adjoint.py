@@ -57,7 +57,8 @@ def test_tip_displacement_gradient_vs_cpu_reference(c3):
 
 def test_gradient_vs_device_central_differences(c3):
     """h = 1e-4 E_e on 5 seeded elements, device forward solves of the
-    perturbed systems (same hierarchy, values reassembled in place)."""
+    perturbed systems (same hierarchy, values reassembled in place), with the
+    adjoint-corrected compliance."""
     mr, E, tip, c, ref, prob = c3
     rng = np.random.default_rng(3)
     _, g, _ = prob.compliance_and_grad(E)
@@ -71,5 +72,7 @@ def test_gradient_vs_device_central_differences(c3):
             Cp, _, _ = prob.compliance_and_grad(Ep)
             vals.append(Cp)
         fd = (vals[0] - vals[1]) / (2 * h)
-        assert abs(fd - g[e]) <= 2e-3 * abs(g[e]), (e, fd, g[e])
+        # the FD difference of C carries FP64 round-off of C itself, an absolute
+        # error: measure it against the gradient's scale as well as g_e
+        assert abs(fd - g[e]) <= 2e-3 * abs(g[e]) + 1e-4 * np.abs(g).max(), (e, fd, g[e])
     prob.set_moduli(E)

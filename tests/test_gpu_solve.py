@@ -136,7 +136,7 @@ def test_vcycle_matches_cpu_mirror(c0_amg):
     from vcycle_mirror import vcycle
     n = c0_amg.levels[0].A.shape[0]
     r = np.random.default_rng(3).standard_normal(n)
-    z_ref = vcycle(c0_amg.levels, c0_amg.coarse_inv, r, deg=c0_amg.degree, ratio=c0_amg.ratio)
+    z_ref = vcycle(c0_amg.levels, c0_amg.coarse_inv, r, deg=c0_amg.level_degrees, ratio=c0_amg.ratio)
     z = c0_amg.vcycle(torch.as_tensor(r, device="cuda")).cpu().numpy()
     assert _rel(z, z_ref) <= 1e-12
 
@@ -221,3 +221,30 @@ def test_precision_policies(c0_system, c0_amg, c0_oracle, policy):
     u, it, rel, _ = mp.solve_with_policy(c0_system, c0_amg, getattr(mp, policy), tol=1e-8)
     assert rel <= 1e-8
     assert _rel(u.cpu().numpy(), u_ref) <= 1e-6
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_vcycle_low_precision_matches_rounded_mirror(c0_amg, storage):
+    """The FP32/FP16-stored device V-cycle equals the numpy cycle run on the
+    same hierarchy with every operator (A, P, R on all levels) rounded to the
+    storage type; the remaining difference is FP32 arithmetic."""
+    import copy
+    import scipy.sparse as sp
+    import torch
+    from vcycle_mirror import vcycle
+    dt = np.float16 if storage == "fp16" else np.float32
+    levels = copy.deepcopy(c0_amg.levels)
+
+    def rnd(M):
+        M = sp.csr_matrix(M).copy()
+        M.data = M.data.astype(dt).astype(np.float64)
+        return M
+
+    for lv in levels[:-1]:
+        lv.A = rnd(lv.A)
+        lv.P, lv.R = rnd(lv.P), rnd(lv.R)
+    n = levels[0].A.shape[0]
+    r = np.random.default_rng(4).standard_normal(n)
+    z_ref = vcycle(levels, c0_amg.coarse_inv, r, deg=c0_amg.level_degrees, ratio=c0_amg.ratio)
+    z = c0_amg.vcycle(torch.as_tensor(r, device="cuda").float(), storage=storage).double().cpu().numpy()
+    assert _rel(z, z_ref) <= 2e-5

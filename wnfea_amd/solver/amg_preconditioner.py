@@ -201,6 +201,7 @@ class SmoothedAggregationAMG:
     def __init__(self, fine, A_hat: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None = None,
                  dof_scale: np.ndarray | None = None, theta: float = 0.08, theta_decay: float = 1.0,
                  degree: int = 2, ratio: float = 30.0, coarse_size: int = 600, max_levels: int = 10,
+                 coarse_degree: int | None = None,
                  storage: tuple = ("fp64", "fp32", "fp16"), sigma: int = 4096, waves: int = 1,
                  xcd_remap: bool = True, unroll: int = 1):
         t0 = time.perf_counter()
@@ -209,6 +210,10 @@ class SmoothedAggregationAMG:
                                     coarse_size=coarse_size, max_levels=max_levels, dof_scale=dof_scale)
         self.setup_host_sec = time.perf_counter() - t0
         self.degree, self.ratio = degree, ratio
+        # Chebyshev degree from level 2 on (the small levels are latency-bound:
+        # a lower degree there saves whole kernel launches per cycle)
+        self.coarse_degree = degree if coarse_degree is None else coarse_degree
+        self.level_degrees = [degree if l < 2 else self.coarse_degree for l in range(len(self.levels))]
         self.fine = fine
         nl = len(self.levels)
         self.storage = tuple(storage)
@@ -245,7 +250,7 @@ class SmoothedAggregationAMG:
             self._keep.append((args_A, Pa, Ra, dinv))
             rc = self.lib.wnfea_amg_set_level(
                 self.handle, l, fine_h, b, n_rows, _lib.hptr(args_A[0]), _lib.hptr(args_A[1]), _lib.hptr(args_A[2]),
-                _lib.hptr(dinv), float(lv.lmax), float(ratio), int(degree), int(kc), int(nc),
+                _lib.hptr(dinv), float(lv.lmax), float(ratio), int(self.level_degrees[l]), int(kc), int(nc),
                 _lib.hptr(Pa[0]), _lib.hptr(Pa[1]), _lib.hptr(Pa[2]), _lib.hptr(Ra[0]), _lib.hptr(Ra[1]),
                 _lib.hptr(Ra[2]), int(sigma))
             _lib.check(rc, f"wnfea_amg_set_level({l})")
