@@ -256,7 +256,7 @@ __global__ void mr_cheb_first_kernel(long long n, const T* __restrict__ b, const
 
 // X[i][k] = sum_j inv[i][j] B[j][k] on the coarsest level (one block per row).
 template <typename T, int NR>
-__global__ __launch_bounds__(256) void mr_coarse_kernel(int n, const double* __restrict__ inv,
+__global__ __launch_bounds__(256) void mr_coarse_kernel(int n, const T* __restrict__ inv,
                                                         const T* __restrict__ b, T* __restrict__ x,
                                                         const int* skip) {
   if (skip && *skip) return;
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void mr_coarse_kernel(int n, const double* __r
 #pragma unroll
   for (int k = 0; k < NR; ++k) acc[k] = 0.0;
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    const double a = inv[(size_t)i * n + j];
+    const double a = double(inv[(size_t)i * n + j]);
 #pragma unroll
     for (int k = 0; k < NR; ++k) acc[k] += a * double(b[(size_t)j * NR + k]);
   }
@@ -391,8 +391,9 @@ static int vcycle_mr(Amg& g, MultiState& M, int l, const T* b, T* x, const int* 
   t.guard = skip ? 1 : 0;
   const long long n = L.n;
   if (l == (int)g.L.size() - 1) {
-    if (L.n != g.n_coarse || !g.inv) return kNotReady;
-    hipLaunchKernelGGL((mr_coarse_kernel<T, NR>), dim3(std::max(L.n, 1)), dim3(256), 0, s, L.n, g.inv, b, x, skip);
+    const T* inv = coarse_inv<T>(g);
+    if (L.n != g.n_coarse || !inv) return kNotReady;
+    hipLaunchKernelGGL((mr_coarse_kernel<T, NR>), dim3(std::max(L.n, 1)), dim3(256), 0, s, L.n, inv, b, x, skip);
     return static_cast<int>(hipGetLastError());
   }
   MrLevel<T>& V = mr_levels<T>(M)[l];

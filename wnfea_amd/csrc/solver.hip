@@ -52,15 +52,17 @@ __global__ void cheb_first_kernel(long long n, const T* __restrict__ b, const T*
   }
 }
 
-// x = inv * b on the coarsest level: one 256-thread block per row, FP64.
+// x = inv * b on the coarsest level: one 256-thread block per row; the
+// inverse is stored in the cycle's type T (its bytes dominate), the dot
+// product accumulates in FP64.
 template <typename T>
-__global__ __launch_bounds__(256) void coarse_gemv_kernel(int n, const double* __restrict__ inv,
+__global__ __launch_bounds__(256) void coarse_gemv_kernel(int n, const T* __restrict__ inv,
                                                           const T* __restrict__ b, T* __restrict__ x,
                                                           const int* skip) {
   if (skip && *skip) return;
   const int i = blockIdx.x;
   double s = 0.0;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) s += inv[(size_t)i * n + j] * double(b[j]);
+  for (int j = threadIdx.x; j < n; j += blockDim.x) s += double(inv[(size_t)i * n + j]) * double(b[j]);
   __shared__ double sm[4];
   s = block_sum(s, sm);
   if (threadIdx.x == 0) x[i] = T(s);
@@ -76,8 +78,9 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   t.guard = skip ? 1 : 0;
   const long long n = L.n;
   if (l == (int)g.L.size() - 1) {
-    if (L.n != g.n_coarse || !g.inv) return kNotReady;
-    hipLaunchKernelGGL(coarse_gemv_kernel<T>, dim3(std::max(L.n, 1)), dim3(256), 0, s, L.n, g.inv, b, x, skip);
+    const T* inv = coarse_inv<T>(g);
+    if (L.n != g.n_coarse || !inv) return kNotReady;
+    hipLaunchKernelGGL(coarse_gemv_kernel<T>, dim3(std::max(L.n, 1)), dim3(256), 0, s, L.n, inv, b, x, skip);
     return static_cast<int>(hipGetLastError());
   }
   T** v = level_vecs<T>(L);
@@ -187,6 +190,7 @@ static void key_amg(std::vector<uint64_t>& k, const Amg* g) {
   if (!g) return;
   for (int v : {g->store, g->n_coarse, (int)g->L.size()}) key_push(k, v);
   key_push(k, g->inv);
+  key_push(k, g->inv32);
   key_tune(k, g->tune);
   for (const auto& lp : g->L) {
     const AmgLevel& l = *lp;
@@ -483,9 +487,18 @@ int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const i
 
 int amg_set_coarse(Amg& g, int n, const double* inv) {
   if (g.inv) (void)hipFree(g.inv);
+  if (g.inv32) (void)hipFree(g.inv32);
   g.inv = nullptr;
-  WN_HIP(hipMalloc(&g.inv, sizeof(double) * (size_t)n * n));
-  WN_HIP(hipMemcpy(g.inv, inv, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice));
+  g.inv32 = nullptr;
+  const size_t nn = (size_t)n * n;
+  WN_HIP(hipMalloc(&g.inv, sizeof(double) * nn));
+  WN_HIP(hipMemcpy(g.inv, inv, sizeof(double) * nn, hipMemcpyHostToDevice));
+  if (g.store & 6) {
+    WN_HIP(hipMalloc(&g.inv32, sizeof(float) * nn));
+    const int r = launch_cast_scale<double, float>((long long)nn, g.inv, nullptr, 1.0, g.inv32, nullptr);
+    if (r) return r;
+    WN_HIP(hipDeviceSynchronize());
+  }
   g.n_coarse = n;
   return kOk;
 }

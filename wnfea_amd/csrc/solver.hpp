@@ -28,7 +28,8 @@ struct Amg {
   std::vector<std::unique_ptr<AmgLevel>> L;
   int store = 3;  // bit0 FP64, bit1 FP32, bit2 FP16 values
   int n_coarse = 0;
-  double* inv = nullptr;  // n_coarse^2, row-major
+  double* inv = nullptr;   // n_coarse^2, row-major
+  float* inv32 = nullptr;  // FP32 copy for the FP32/FP16 cycles (when FP32 values are stored)
   SpmvTune tune;
   ~Amg() { release(); }
   void release() {
@@ -47,7 +48,9 @@ struct Amg {
     }
     L.clear();
     if (inv) (void)hipFree(inv);
+    if (inv32) (void)hipFree(inv32);
     inv = nullptr;
+    inv32 = nullptr;
   }
 };
 
@@ -165,5 +168,10 @@ struct Solver {
 };
 
 int amg_alloc_vectors(Amg& g, int prec_mask);
+
+// Coarsest-level inverse in the cycle's arithmetic type.
+template <typename T> const T* coarse_inv(const Amg& g);
+template <> inline const double* coarse_inv<double>(const Amg& g) { return g.inv; }
+template <> inline const float* coarse_inv<float>(const Amg& g) { return g.inv32; }
 
 }  // namespace wn

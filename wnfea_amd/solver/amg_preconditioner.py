@@ -113,15 +113,17 @@ class AMGLevelData:
 
 
 def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None = None, theta: float = 0.08,
-             theta_decay: float = 1.0, omega: float = 4.0 / 3.0, coarse_size: int = 600, max_levels: int = 10,
+             theta_decay: float = 1.0, omega: float = 4.0 / 3.0, coarse_size: int = 12000, max_levels: int = 10,
              power_iters: int = 15, lmax_safety: float = 1.05,
              dof_scale: np.ndarray | None = None) -> tuple[list[AMGLevelData], np.ndarray]:
     """Host SA hierarchy of a block matrix.
 
     When A = S K S is the equilibrated operator (S = diag(1/sqrt|K_ii|)), its
     near-null-space is S^-1 B: pass dof_scale = sqrt|K_ii| so the rigid-body
-    modes are scaled accordingly.  Returns (levels, coarse_inverse).  Level
-    l > 0 operators are 6x6-block."""
+    modes are scaled accordingly.  Coarsening stops at the first level with
+    at most coarse_size rows (default 12 000: a dense inverse of up to 1.1 GB
+    in FP64).  Returns (levels, coarse_inverse).  Level l > 0 operators are
+    6x6-block."""
     L = _lib.load()
     A = sp.bsr_matrix(A)
     A.sort_indices()
@@ -180,8 +182,17 @@ def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None
         levels.append(AMGLevelData(A=A, dinv=dinv, lmax=lmax, P=P, R=R, n_aggregates=na))
         A, B = Ac, Bc
         th *= theta_decay
+    # exact coarse solve: the dense inverse of the coarsest operator (SPD),
+    # applied as one GEMV per cycle on the device.  A large coarsest level pays:
+    # on the I-beam an exact solve on ~11 k rows instead of ~170 rows takes CG
+    # from 90 to 50 iterations.
     Acd = levels[-1].A.toarray()
-    inv = np.linalg.inv(Acd)
+    try:
+        import scipy.linalg as sla
+        c = sla.cho_factor(Acd, lower=True, check_finite=False)
+        inv = sla.cho_solve(c, np.eye(len(Acd)), check_finite=False)
+    except np.linalg.LinAlgError:
+        inv = np.linalg.inv(Acd)
     inv = 0.5 * (inv + inv.T)
     return levels, inv
 
@@ -200,7 +211,7 @@ class SmoothedAggregationAMG:
 
     def __init__(self, fine, A_hat: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None = None,
                  dof_scale: np.ndarray | None = None, theta: float = 0.08, theta_decay: float = 1.0,
-                 degree: int = 2, ratio: float = 30.0, coarse_size: int = 600, max_levels: int = 10,
+                 degree: int = 2, ratio: float = 30.0, coarse_size: int = 12000, max_levels: int = 10,
                  coarse_degree: int | None = None,
                  storage: tuple = ("fp64", "fp32", "fp16"), sigma: int = 4096, waves: int = 1,
                  xcd_remap: bool = True, unroll: int = 1):
