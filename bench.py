@@ -256,6 +256,34 @@ def run_adjoint(args, log) -> int:
     return 0
 
 
+def tune_spmv(K, shape: str | None, log) -> dict:
+    """SpMV launch shapes for the FP32 and the FP64 values of K: fixed by
+    `shape` ("waves,unroll,xcd_remap" for both, or "fp32 shape:fp64 shape"),
+    or each autotuned."""
+    from wnfea_amd.solver.autotune import autotune_spmv
+    if shape:
+        parts = shape.split(":")
+        out = {}
+        for prec, text in zip(("fp32", "fp64"), (parts[0], parts[-1])):
+            w, u, x = (int(v) for v in text.split(","))
+            K.tune(w, bool(x), u, precision=prec)
+            out[prec] = {"waves": w, "unroll": u, "xcd_remap": bool(x)}
+        return out
+    t0 = time.perf_counter()
+    out = {}
+    for prec in ("fp64", "fp32"):
+        c = autotune_spmv(K, prec, use_cache=False)
+        out[prec] = {k: c[k] for k in ("waves", "unroll", "xcd_remap", "seconds")}
+    log(f"[bench] autotune {out} ({time.perf_counter() - t0:.1f}s)")
+    return out
+
+
+def apply_tune(amg, tunes: dict) -> None:
+    """The cycle's FP64 shape (the constructor set the FP32 one for both)."""
+    t = tunes["fp64"]
+    amg.set_tune("fp64", t["waves"], t["unroll"], t["xcd_remap"])
+
+
 def run_multirhs(args, log) -> int:
     """C2a: 32 seeded tip load cases (forces N(0, 20 kN), moments N(0, 2 kN m)
     per component, seed 0) on the I-beam, solved in groups of 8 columns (one
@@ -263,14 +291,15 @@ def run_multirhs(args, log) -> int:
     import torch
     from wnfea_amd.mesh.rcm import apply_rcm_to_model
     from wnfea_amd.mesh.solid_generator import generate_c3d10_ibeam, ibeam_tip_load_cases
-    from wnfea_amd.solver.autotune import autotune_spmv
     from wnfea_amd.solver.solid_solver import assemble_c3d10_bsr, build_amg, solve_mixed, solve_multi
     model, meta = generate_c3d10_ibeam(n_layers=args.layers, length=3.0 * args.layers / 300)
     model, n2o, o2n = apply_rcm_to_model(model)
     system = assemble_c3d10_bsr(model)
-    tune = autotune_spmv(system.K, "fp32", use_cache=False)
+    tunes = tune_spmv(system.K, args.spmv_shape, log)
+    tune = tunes["fp32"]
     amg = build_amg(system, storage=("fp32",), waves=tune["waves"], unroll=tune["unroll"],
                     xcd_remap=tune["xcd_remap"])
+    apply_tune(amg, tunes)
     B = ibeam_tip_load_cases(model.mesh_nodes, o2n[meta["tip_nodes"]], n_cases=32, seed=0)
     B[system.fixed_mask] = 0.0
     Bt = torch.as_tensor(B, device="cuda")
@@ -398,7 +427,8 @@ def main() -> int:
     ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine", "fp16_coarse"], default="fp32")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
     ap.add_argument("--spmv-shape", default=None,
-                    help="waves,unroll,xcd_remap for the SpMV instead of autotuning (e.g. 4,1,0)")
+                    help="waves,unroll,xcd_remap of the SpMV instead of autotuning, for both value precisions "
+                         "(e.g. 4,1,0) or as fp32:fp64 (e.g. 2,2,1:4,1,1)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV launches (and a 256 MiB copy for counter calibration), for rocprofv3 --pmc")
     ap.add_argument("--ir", action="store_true", help="also time the restart-IR (FP32 Krylov) scheme")
@@ -445,20 +475,15 @@ def main() -> int:
     t_asm = time.perf_counter() - t0
     info = system.K.info()
     log(f"[bench] rcm {t_rcm:.1f}s assembly {t_asm:.1f}s nnzb {info['nnzb']} slots {info['slots']}")
-    tune = {"waves": 1, "unroll": 1, "xcd_remap": True}
-    if args.spmv_shape:
-        w, u, x = (int(v) for v in args.spmv_shape.split(","))
-        tune = {"waves": w, "unroll": u, "xcd_remap": bool(x)}
-        system.K.tune(w, bool(x), u)
-    elif not args.no_autotune:
-        t0 = time.perf_counter()
-        tune = autotune_spmv(system.K, "fp32", use_cache=False)
-        tune.pop("table", None)
-        log(f"[bench] autotune {tune} ({time.perf_counter() - t0:.1f}s)")
+    if args.no_autotune and not args.spmv_shape:
+        args.spmv_shape = "1,1,1"
+    tunes = tune_spmv(system.K, args.spmv_shape, log)
+    tune = tunes["fp32"]
     if args.spmv_only:
         return spmv_only(system, info, args.spmv_reps, log)
     t0 = time.perf_counter()
     amg = build_amg(system, waves=tune["waves"], unroll=tune["unroll"], xcd_remap=tune["xcd_remap"])
+    apply_tune(amg, tunes)
     t_amg = time.perf_counter() - t0
     st = amg.stats()
     log(f"[bench] amg setup {t_amg:.1f}s levels {[l['n'] for l in st['levels']]} "
@@ -579,7 +604,7 @@ def main() -> int:
                      "algorithmic_bytes": b32,
                      "kernel": f"sell_spmv_kernel<3,3,float,float,{tune['waves']},{tune['unroll']},OpAxpby> "
                                f"(FP32 3x3 BSR SpMV, {info['lanes']} lanes per block row)"},
-        "spmv_tune": tune,
+        "spmv_tune": tunes,
         "setup": {"mesh_sec": round(t_mesh, 2), "rcm_sec": round(t_rcm, 2), "assembly_sec": round(t_asm, 2),
                   "amg_setup_sec": round(t_amg, 2), "amg": st},
         "cpu_baseline": cpu,

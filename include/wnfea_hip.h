@@ -162,6 +162,12 @@ int wnfea_bsr_info(void *h, long long *out);
  * reference's launch sizes are fixed heuristics, wnfea/solver/fast_kernels.py:243-260). */
 int wnfea_bsr_tune(void *h, int waves, int xcd_remap, int unroll);
 
+/* The same for one value precision only (prec 0: FP64 values, 1: FP32 and
+ * FP16 values): the fastest shapes differ between the precisions.  No
+ * reference counterpart (wnfea/solver/fast_kernels.py:243-260 has one fixed
+ * launch size). */
+int wnfea_bsr_tune_precision(void *h, int prec, int waves, int xcd_remap, int unroll);
+
 /* Values from the standard layout (device FP64 [nnzb][br*bc]); which: 1 FP64, 2 FP32, 3 both. */
 int wnfea_bsr_set_values(void *h, const double *vals_std, int which, void *stream);
 
@@ -235,6 +241,11 @@ int wnfea_amg_set_coarse(void *h, int n, const double *inv);
 /* Allocate the cycle vectors of the stored precisions and set the SpMV
  * launch shape (as wnfea_bsr_tune). */
 int wnfea_amg_finalize(void *h, int waves, int xcd_remap, int unroll);
+
+/* SpMV launch shape of the cycle for one value precision (prec as in
+ * wnfea_bsr_tune_precision), after wnfea_amg_finalize.  No reference
+ * counterpart (wnfea/solver/fast_kernels.py:243-260). */
+int wnfea_amg_tune_precision(void *h, int prec, int waves, int xcd_remap, int unroll);
 
 /* z = M r, one V-cycle (device vectors).  vt: 0 FP64 values and vectors;
  * 1 FP32 values, 2 FP16 values on every level, 3 FP16 values on the finest

@@ -248,3 +248,35 @@ def test_vcycle_low_precision_matches_rounded_mirror(c0_amg, storage):
     z_ref = vcycle(levels, c0_amg.coarse_inv, r, deg=c0_amg.level_degrees, ratio=c0_amg.ratio)
     z = c0_amg.vcycle(torch.as_tensor(r, device="cuda").float(), storage=storage).double().cpu().numpy()
     assert _rel(z, z_ref) <= 2e-5
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32"])
+def test_coarse_only_cycle_is_inverse_product(storage):
+    """A hierarchy of one level (1 575 DOF, below the coarse size): the cycle
+    is the packed symmetric tile product of the exact inverse, so it must equal
+    inv @ r (FP64: to round-off; FP32: against the inverse rounded to FP32,
+    the result rounded once).  1 575 = 24 * 64 + 39 exercises a partial last
+    tile and off-diagonal tiles on both sides."""
+    import torch
+    from wnfea_amd.mesh.rcm import apply_rcm_to_model
+    from wnfea_amd.mesh.solid_generator import generate_c3d10_structured_block
+    from wnfea_amd.solver.solid_solver import assemble_c3d10_bsr, build_amg
+    m, _ = generate_c3d10_structured_block(1.0, 0.1, 0.1, 10, 2, 2, tip_load_total=-1000.0, tets_per_hex=6,
+                                           load_direction="z")
+    mr, _, _ = apply_rcm_to_model(m)
+    system = assemble_c3d10_bsr(mr, want_fp64_equilibrated=True)
+    amg = build_amg(system, storage=("fp64", "fp32"))
+    assert len(amg.levels) == 1
+    inv = amg.coarse_inv
+    n = inv.shape[0]
+    assert n == 1575 and n % 64 != 0
+    r = np.random.default_rng(7).standard_normal(n)
+    if storage == "fp64":
+        z = amg.vcycle(torch.as_tensor(r, device="cuda")).cpu().numpy()
+        assert _rel(z, inv @ r) <= 1e-13
+    else:
+        r32 = r.astype(np.float32)
+        z_ref = inv.astype(np.float32).astype(np.float64) @ r32.astype(np.float64)
+        z = amg.vcycle(torch.as_tensor(r32, device="cuda")).double().cpu().numpy()
+        assert np.max(np.abs(z - z_ref) / np.maximum(np.abs(z_ref), 1e-30 + np.abs(z_ref).max() * 1e-6)) <= 2e-7
+    amg.close()

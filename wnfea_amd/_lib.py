@@ -45,6 +45,7 @@ SIGNATURES = {
     "wnfea_bsr_destroy": (None, [_p]),
     "wnfea_bsr_info": (_i, [_p, _pll]),
     "wnfea_bsr_tune": (_i, [_p, _i, _i, _i]),
+    "wnfea_bsr_tune_precision": (_i, [_p, _i, _i, _i, _i]),
     "wnfea_bsr_set_values": (_i, [_p, _p, _i, _p]),
     "wnfea_bsr_get_values": (_i, [_p, _p, _p]),
     "wnfea_bsr_assemble": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p]),
@@ -57,6 +58,7 @@ SIGNATURES = {
     "wnfea_amg_set_level": (_i, [_p, _i, _p, _i, _i, _p, _p, _p, _p, _d, _d, _i, _i, _i, _p, _p, _p, _p, _p, _p, _i]),
     "wnfea_amg_set_coarse": (_i, [_p, _i, _p]),
     "wnfea_amg_finalize": (_i, [_p, _i, _i, _i]),
+    "wnfea_amg_tune_precision": (_i, [_p, _i, _i, _i, _i]),
     "wnfea_amg_vcycle": (_i, [_p, _i, _p, _p, _p]),
     "wnfea_solver_create": (_p, []),
     "wnfea_solver_destroy": (None, [_p]),

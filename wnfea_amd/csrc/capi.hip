@@ -50,6 +50,7 @@ int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const i
                   int n_coarse_rows, const int* p_rowptr, const int* p_colidx, const double* p_vals,
                   const int* r_rowptr, const int* r_colidx, const double* r_vals, int sigma);
 int amg_set_coarse(Amg& g, int n, const double* inv);
+int amg_set_tune(Amg& g, int prec, int waves, int xcd_remap, int unroll);
 int amg_finalize(Amg& g, const SpmvTune& t);
 }  // namespace wn
 
@@ -222,12 +223,21 @@ int wnfea_bsr_info(void* h, long long* out) {
   return kOk;
 }
 
+static bool shape_ok(int waves, int unroll) {
+  return (waves == 1 || waves == 2 || waves == 4) && (unroll == 1 || unroll == 2 || unroll == 4);
+}
+
+// launch shape of every value precision
 int wnfea_bsr_tune(void* h, int waves, int xcd_remap, int unroll) {
-  if (!h || (waves != 1 && waves != 2 && waves != 4) || (unroll != 1 && unroll != 2 && unroll != 4)) return kBadArg;
-  auto* b = static_cast<WnBsr*>(h);
-  b->tune.waves = waves;
-  b->tune.xcd_remap = xcd_remap ? 1 : 0;
-  b->tune.unroll = unroll;
+  if (!h || !shape_ok(waves, unroll)) return kBadArg;
+  static_cast<WnBsr*>(h)->tune.set(-1, waves, xcd_remap ? 1 : 0, unroll);
+  return kOk;
+}
+
+// launch shape of one value precision: prec 0 FP64 values, 1 FP32 and FP16 values
+int wnfea_bsr_tune_precision(void* h, int prec, int waves, int xcd_remap, int unroll) {
+  if (!h || !shape_ok(waves, unroll) || (prec != 0 && prec != 1)) return kBadArg;
+  static_cast<WnBsr*>(h)->tune.set(prec, waves, xcd_remap ? 1 : 0, unroll);
   return kOk;
 }
 
@@ -344,12 +354,17 @@ int wnfea_amg_set_coarse(void* h, int n, const double* inv) {
 
 // allocate the cycle vectors of the stored precisions; set the SpMV launch shape
 int wnfea_amg_finalize(void* h, int waves, int xcd_remap, int unroll) {
-  if (!h || (waves != 1 && waves != 2 && waves != 4) || (unroll != 1 && unroll != 2 && unroll != 4)) return kBadArg;
+  if (!h || !shape_ok(waves, unroll)) return kBadArg;
   SpmvTune t;
-  t.waves = waves;
-  t.xcd_remap = xcd_remap ? 1 : 0;
-  t.unroll = unroll;
+  t.set(-1, waves, xcd_remap ? 1 : 0, unroll);
   return amg_finalize(*static_cast<Amg*>(h), t);
+}
+
+// SpMV launch shape of the cycle for one value precision (prec 0 FP64 values,
+// 1 FP32 and FP16 values), after wnfea_amg_finalize
+int wnfea_amg_tune_precision(void* h, int prec, int waves, int xcd_remap, int unroll) {
+  if (!h || !shape_ok(waves, unroll) || (prec != 0 && prec != 1)) return kBadArg;
+  return amg_set_tune(*static_cast<Amg*>(h), prec, waves, xcd_remap ? 1 : 0, unroll);
 }
 
 // z = M r (one V-cycle); vt 0 = FP64, 1 = FP32, 2 = FP16 values, 3 = FP16 on the

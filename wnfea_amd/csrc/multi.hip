@@ -44,8 +44,9 @@ __global__ __launch_bounds__(64 * W) void sell_spmm_kernel(SellView<VT> A, const
 #pragma unroll
   for (int k = 0; k < (Op::kDots > 0 ? Op::kDots : 1); ++k) dot[k] = 0.0;
   if (s < A.n_slices) {
-    const int slot_lane = s * 64 + lane;
-    const int len = A.lane_len[slot_lane];
+    const int rps = 64 / A.lanes;  // rows per slice; a row's lanes are r, r + rps, ...
+    const int2 ri = A.rinfo[(long long)s * rps + lane % rps];
+    const int len = lane_steps(ri.y, lane / rps, A.lanes);
     const long long base = A.off[s];
     AT acc[BR * NR];
 #pragma unroll
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(64 * W) void sell_spmm_kernel(SellView<VT> A, const
     const int* __restrict__ colp = A.col + base + lane;
     const VT* __restrict__ valp = A.val + base * BB;
     for (int j = wave; j < len; j += W) {
-      const int c = colp[(long long)j * 64];
+      const int c = load_col(colp + (long long)j * 64);
       VT v[BB];
       load_block<VT, BB>(valp + (long long)j * 64 * BB, lane, v);
 #pragma unroll
@@ -80,11 +81,10 @@ __global__ __launch_bounds__(64 * W) void sell_spmm_kernel(SellView<VT> A, const
           for (int t = 0; t < BR * NR; ++t) acc[t] += red[w][t][lane];
       }
     }
-    const int rps = 64 / A.lanes;  // rows per slice; a row's lanes are r, r + rps, ...
     for (int off = 32; off >= rps; off >>= 1)
 #pragma unroll
       for (int t = 0; t < BR * NR; ++t) acc[t] += __shfl_xor(acc[t], off, 64);
-    const int row = A.lane_row[slot_lane];
+    const int row = ri.x;
     // every lane of the row holds the full sums after the butterfly: the row's
     // L lanes share the NR columns of the epilogue (column k on sub-lane k mod L)
     if (wave == 0 && row >= 0) op.row(row, acc, dot, lane / rps, A.lanes);
@@ -185,15 +185,16 @@ template <int BR, int BC, typename VT, typename XT, int NR, class Op>
 static int launch_spmm(const SellMatrix& A, const XT* x, const Op& op, const SpmvTune& t, hipStream_t s) {
   const SellView<VT> v = A.view<VT>();
   if (A.n_slices == 0) return kOk;
-  const int grid = t.xcd_remap ? 8 * ((A.n_slices + 7) / 8) : A.n_slices;
+  const int xcd = t.xcd_for<VT>();
+  const int grid = xcd ? 8 * ((A.n_slices + 7) / 8) : A.n_slices;
   // four waves per slice where the partial accumulators fit in LDS (3-row
   // blocks), one for the 6-row blocks
   if constexpr (BR * NR <= 32) {
     hipLaunchKernelGGL((sell_spmm_kernel<BR, BC, VT, XT, 4, NR, Op>), dim3(grid), dim3(256), 0, s, v, x, op,
-                       t.xcd_remap, t.flags, t.guard);
+                       xcd, t.flags, t.guard);
   } else {
     hipLaunchKernelGGL((sell_spmm_kernel<BR, BC, VT, XT, 1, NR, Op>), dim3(grid), dim3(64), 0, s, v, x, op,
-                       t.xcd_remap, t.flags, t.guard);
+                       xcd, t.flags, t.guard);
   }
   return static_cast<int>(hipGetLastError());
 }
@@ -614,8 +615,9 @@ static int mr_spmm_dot(const SellMatrix& A, const double* p, double* q, double* 
 #undef CALL
 }
 
-static int spmm_grid(const SellMatrix& A, const SpmvTune& t) {
-  return t.xcd_remap ? 8 * ((A.n_slices + 7) / 8) : A.n_slices;
+// grid of the FP64-value SpMM (its block partials feed the CG dots)
+static int spmm_grid64(const SellMatrix& A, const SpmvTune& t) {
+  return t.xcd_for<double>() ? 8 * ((A.n_slices + 7) / 8) : A.n_slices;
 }
 
 // PCG on NR columns of A X = B from X = 0; preconditioner: AMG V-cycle in
@@ -636,7 +638,7 @@ static int pcg_multi_run(Solver& S, const SellMatrix& A, Amg* g, int vt, const d
   hipLaunchKernelGGL((mr_start_kernel<NR>), dim3(gv), dim3(kMrBlock), 0, s, n, B, X, M.r, M.rp32, scale, M.partials);
   if ((r = static_cast<int>(hipGetLastError()))) return r;
   if ((r = mr_fin<NR>(M, gv, kMrStart, rtol * rtol, s))) return r;
-  const int gs = spmm_grid(A, t);
+  const int gs = spmm_grid64(A, t);
   int done_all = 0;
   for (int it = 0; it < max_iter && !done_all;) {
     const int chunk = std::min(check_every, max_iter - it);

@@ -29,12 +29,13 @@ void SellMatrix::release() {
     delete eq_alias;  // non-owning view: nothing of its own to free
     eq_alias = nullptr;
   }
-  for (void* p : {(void*)d_off, (void*)d_len, (void*)d_row, (void*)d_col, (void*)d_slot, (void*)d_v64, (void*)d_v64e,
+  for (void* p : {(void*)d_off, (void*)d_rinfo, (void*)d_col, (void*)d_slot, (void*)d_v64, (void*)d_v64e,
                   (void*)d_v32, (void*)d_v16})
     if (p) (void)hipFree(p);
   d_v16 = nullptr;
   d_off = nullptr;
-  d_len = d_row = d_col = nullptr;
+  d_rinfo = nullptr;
+  d_col = nullptr;
   d_slot = nullptr;
   d_v64 = nullptr;
   d_v64e = nullptr;
@@ -47,7 +48,7 @@ SellMatrix& SellMatrix::equilibrated() {
   SellMatrix& a = *eq_alias;
   a.br = br; a.bc = bc; a.n_rows = n_rows; a.n_cols = n_cols; a.n_slices = n_slices; a.sigma = sigma;
   a.lanes = lanes;
-  a.nnzb = nnzb; a.slots = slots; a.d_off = d_off; a.d_len = d_len; a.d_row = d_row; a.d_col = d_col;
+  a.nnzb = nnzb; a.slots = slots; a.d_off = d_off; a.d_rinfo = d_rinfo; a.d_col = d_col;
   a.d_slot = d_slot; a.d_v64 = d_v64e; a.d_v64e = nullptr; a.d_v32 = d_v32; a.d_v16 = d_v16;
   return a;
 }
@@ -95,7 +96,7 @@ int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* row
     });
   }
   std::vector<long long> off(n_slices + 1, 0);
-  std::vector<int> len((size_t)n_slices * 64, 0), row((size_t)n_slices * 64, -1);
+  std::vector<int2> rinfo((size_t)n_slices * rows_per_slice, make_int2(-1, 0));
   for (int s = 0; s < n_slices; ++s) {
     int steps = 0;
     for (int r = 0; r < rows_per_slice; ++r) {
@@ -103,12 +104,8 @@ int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* row
       if (k >= n_rows) break;
       const int g = order[k];
       const int rl = rowptr[g + 1] - rowptr[g];
-      for (int q = 0; q < lanes; ++q) {
-        const int ln = s * 64 + q * rows_per_slice + r;
-        row[ln] = g;
-        len[ln] = rl > q ? (rl - q + lanes - 1) / lanes : 0;
-        steps = std::max(steps, len[ln]);
-      }
+      rinfo[(size_t)k] = make_int2(g, rl);
+      steps = std::max(steps, (rl + lanes - 1) / lanes);  // sub-lane 0 has the most steps
     }
     off[s + 1] = off[s] + 64LL * steps;
   }
@@ -129,13 +126,11 @@ int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* row
       // step count, so they are never read.
     }
   WN_HIP(hipMalloc(&d_off, sizeof(long long) * off.size()));
-  WN_HIP(hipMalloc(&d_len, sizeof(int) * len.size()));
-  WN_HIP(hipMalloc(&d_row, sizeof(int) * row.size()));
+  WN_HIP(hipMalloc(&d_rinfo, sizeof(int2) * rinfo.size()));
   WN_HIP(hipMalloc(&d_col, sizeof(int) * col.size()));
   WN_HIP(hipMalloc(&d_slot, sizeof(long long) * slot.size()));
   WN_HIP(hipMemcpy(d_off, off.data(), sizeof(long long) * off.size(), hipMemcpyHostToDevice));
-  WN_HIP(hipMemcpy(d_len, len.data(), sizeof(int) * len.size(), hipMemcpyHostToDevice));
-  WN_HIP(hipMemcpy(d_row, row.data(), sizeof(int) * row.size(), hipMemcpyHostToDevice));
+  WN_HIP(hipMemcpy(d_rinfo, rinfo.data(), sizeof(int2) * rinfo.size(), hipMemcpyHostToDevice));
   WN_HIP(hipMemcpy(d_col, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
   WN_HIP(hipMemcpy(d_slot, slot.data(), sizeof(long long) * slot.size(), hipMemcpyHostToDevice));
   return kOk;
@@ -384,9 +379,10 @@ static void launch_w(const SellView<VT>& v, const XT* x, const Op& op, int grid,
                        flags, guard);
 }
 
+template <typename VT>
 static int sell_grid(const SellMatrix& A, const SpmvTune& t) {
   const int per = (A.n_slices + 7) / 8;
-  return t.xcd_remap ? per * 8 : A.n_slices;
+  return t.xcd_for<VT>() ? per * 8 : A.n_slices;
 }
 
 template <int BR, int BC, typename VT, typename XT, class Op>
@@ -394,16 +390,17 @@ static int launch_sell(const SellMatrix& A, const XT* x, const Op& op, const Spm
   if (A.n_slices == 0) return kOk;
   const SellView<VT> v = A.view<VT>();
   if (!v.val) return kNotReady;
-  const int grid = sell_grid(A, t);
-  switch (t.waves) {
+  const int grid = sell_grid<VT>(A, t);
+  const int u = t.unroll_for<VT>(), xcd = t.xcd_for<VT>();
+  switch (t.waves_for<VT>()) {
     case 2:
-      launch_w<BR, BC, VT, XT, 2, Op>(v, x, op, grid, t.unroll, t.xcd_remap, t.flags, t.guard, s);
+      launch_w<BR, BC, VT, XT, 2, Op>(v, x, op, grid, u, xcd, t.flags, t.guard, s);
       break;
     case 4:
-      launch_w<BR, BC, VT, XT, 4, Op>(v, x, op, grid, t.unroll, t.xcd_remap, t.flags, t.guard, s);
+      launch_w<BR, BC, VT, XT, 4, Op>(v, x, op, grid, u, xcd, t.flags, t.guard, s);
       break;
     default:
-      launch_w<BR, BC, VT, XT, 1, Op>(v, x, op, grid, t.unroll, t.xcd_remap, t.flags, t.guard, s);
+      launch_w<BR, BC, VT, XT, 1, Op>(v, x, op, grid, u, xcd, t.flags, t.guard, s);
   }
   return static_cast<int>(hipGetLastError());
 }
@@ -448,7 +445,7 @@ int spmv_dot(const SellMatrix& A, const VT* p, VT* y, Workspace& w, int with_alp
   if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
 #undef CALL
   if (r || A.n_slices == 0) return r;
-  return launch_finalize(w.partials, sell_grid(A, t), 1, with_alpha ? kFinGamma : kFinStore, w.sc + kScGamma, w.sc,
+  return launch_finalize(w.partials, sell_grid<VT>(A, t), 1, with_alpha ? kFinGamma : kFinStore, w.sc + kScGamma, w.sc,
                          nullptr, 0, t.guard ? t.flags : nullptr, s);
 }
 template int spmv_dot<double>(const SellMatrix&, const double*, double*, Workspace&, int, const SpmvTune&,
@@ -468,7 +465,7 @@ int spmv_residual64(const SellMatrix& A, const double* x, const double* b, doubl
   if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
 #undef CALL
   if (e || A.n_slices == 0) return e;
-  return launch_finalize(w.partials, sell_grid(A, t), 2, kFinStore, out, w.sc, nullptr, 0,
+  return launch_finalize(w.partials, sell_grid<double>(A, t), 2, kFinStore, out, w.sc, nullptr, 0,
                          t.guard ? t.flags : nullptr, s);
 }
 

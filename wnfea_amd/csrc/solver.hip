@@ -9,9 +9,10 @@
 //   recurse
 //   prolongate  : x += P x_{l+1}                       (1 SpMV)
 //   post-smooth : Chebyshev(deg) from x                (deg SpMV kernels)
-// Coarsest level: x = A_c^{-1} b with the FP64 inverse (dense GEMV in FP64 for
-// every working precision).  Pre- and post-smoothers use the same polynomial,
-// so the cycle is a symmetric preconditioner for PCG.
+// Coarsest level: x = A_c^{-1} b with the exact inverse (computed in FP64,
+// stored as packed upper-triangle tiles in the cycle's type, products
+// accumulated in FP64).  Pre- and post-smoothers use the same polynomial, so
+// the cycle is a symmetric preconditioner for PCG.
 //
 // PCG keeps rho/alpha/beta/||r||^2 on the device; a done flag raised by the
 // x/r update makes every later kernel of the loop return at once, so the host
@@ -68,6 +69,151 @@ __global__ __launch_bounds__(256) void coarse_gemv_kernel(int n, const T* __rest
   if (threadIdx.x == 0) x[i] = T(s);
 }
 
+// ---- packed symmetric coarse inverse ----------------------------------------
+// The inverse is symmetric, so only its upper-triangle 64 x 64 tiles (I <= J)
+// are kept: tile k = I*nt - I(I-1)/2 + (J - I), tile-major, each tile
+// row-major, zero beyond n.  One 256-thread block per tile reads the tile once
+// and produces both products it takes part in: the row partial A_IJ x_J (for
+// y_I) and, off the diagonal, the column partial A_IJ^T x_I (for y_J).  The
+// partials (FP64) go to part[(2k + side) * 64 + r]; a second kernel sums, for
+// each tile row, its nt partials in a fixed order, so the result is
+// reproducible bit for bit.  This halves the bytes of the dense product.
+constexpr int kTile = 64;
+
+__host__ __device__ inline long long sym_tile_index(int I, int J, int nt) {
+  return (long long)I * nt - (long long)I * (I - 1) / 2 + (J - I);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void sym_tile_pack_kernel(int n, int nt, const double* __restrict__ inv,
+                                                           const int2* __restrict__ tij, T* __restrict__ tiles) {
+  const long long k = blockIdx.x;
+  const int2 ij = tij[k];
+  T* dst = tiles + k * (kTile * kTile);
+  for (int e = threadIdx.x; e < kTile * kTile; e += blockDim.x) {
+    const int i = ij.x * kTile + e / kTile, j = ij.y * kTile + e % kTile;
+    dst[e] = (i < n && j < n) ? T(inv[(size_t)i * n + j]) : T(0);
+  }
+}
+
+// Thread t of the block holds V = 16 B / sizeof(T) consecutive columns
+// col0 = (t % CPR) * V of NI rows t / CPR + RPI * m: every load instruction
+// of the block covers 4 KiB of the tile contiguously.
+template <typename T>
+__global__ __launch_bounds__(256) void sym_tile_gemv_kernel(int n, const T* __restrict__ tiles,
+                                                           const int2* __restrict__ tij, const T* __restrict__ x,
+                                                           double* __restrict__ part, const int* skip) {
+  if (skip && *skip) return;
+  constexpr int V = 16 / sizeof(T);
+  constexpr int CPR = kTile / V;    // 16-byte chunks per tile row
+  constexpr int RPI = 256 / CPR;    // rows per block-wide load instruction
+  constexpr int NI = kTile / RPI;   // load instructions per thread
+  using Chunk = VecN<T, V>;
+  const long long k = blockIdx.x;
+  const int2 ij = tij[k];
+  const int t = threadIdx.x;
+  const int c0 = (t % CPR) * V, r0 = t / CPR;
+  const Chunk* src = reinterpret_cast<const Chunk*>(tiles + k * (kTile * kTile));
+  Chunk a[NI];
+#pragma unroll
+  for (int m = 0; m < NI; ++m) a[m] = src[t + 256 * m];
+  double xj[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const int j = ij.y * kTile + c0 + v;
+    xj[v] = j < n ? double(x[j]) : 0.0;
+  }
+  double rowp[NI], colp[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) colp[v] = 0.0;
+#pragma unroll
+  for (int m = 0; m < NI; ++m) {
+    const int i = ij.x * kTile + r0 + RPI * m;
+    const double xi = i < n ? double(x[i]) : 0.0;
+    double s = 0.0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const double av = double(a[m].e[v]);
+      s += av * xj[v];
+      colp[v] += av * xi;
+    }
+    rowp[m] = s;
+  }
+  // row partials: sum over the CPR lanes of a row (all in one wavefront)
+#pragma unroll
+  for (int m = 0; m < NI; ++m) {
+#pragma unroll
+    for (int off = CPR / 2; off > 0; off >>= 1) rowp[m] += __shfl_xor(rowp[m], off, 64);
+  }
+  double* prow = part + (2 * k) * kTile;
+  if (t % CPR == 0) {
+#pragma unroll
+    for (int m = 0; m < NI; ++m) prow[r0 + RPI * m] = rowp[m];
+  }
+  if (ij.x == ij.y) return;  // the diagonal tile has no transposed product
+  // column partials: lanes with the same column inside a wavefront, then the
+  // four wavefronts in order through LDS
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+#pragma unroll
+    for (int off = CPR; off < 64; off <<= 1) colp[v] += __shfl_xor(colp[v], off, 64);
+  }
+  __shared__ double cs[4][kTile];
+  const int w = t / 64, lane = t % 64;
+  if (lane < CPR) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) cs[w][c0 + v] = colp[v];
+  }
+  __syncthreads();
+  if (t < kTile) part[(2 * k + 1) * kTile + t] = ((cs[0][t] + cs[1][t]) + cs[2][t]) + cs[3][t];
+}
+
+// y_I = sum over m of (m < I ? column partial of tile (m, I) : row partial of
+// tile (I, m)); kGroups groups of 64 threads take every kGroups-th m, then
+// add in group order.
+constexpr int kRedGroups = 16;
+
+template <typename T>
+__global__ __launch_bounds__(kTile * kRedGroups) void sym_tile_reduce_kernel(int n, int nt,
+                                                                            const double* __restrict__ part,
+                                                                            T* __restrict__ y, const int* skip) {
+  if (skip && *skip) return;
+  const int I = blockIdx.x;
+  const int g = threadIdx.x / kTile, r = threadIdx.x % kTile;
+  double s = 0.0;
+  for (int m = g; m < nt; m += kRedGroups) {
+    const long long idx = m < I ? 2 * sym_tile_index(m, I, nt) + 1 : 2 * sym_tile_index(I, m, nt);
+    s += part[idx * kTile + r];
+  }
+  __shared__ double red[kRedGroups][kTile];
+  red[g][r] = s;
+  __syncthreads();
+  const int i = I * kTile + threadIdx.x;
+  if (threadIdx.x < kTile && i < n) {
+    double v = red[0][threadIdx.x];
+#pragma unroll
+    for (int k = 1; k < kRedGroups; ++k) v += red[k][threadIdx.x];
+    y[i] = T(v);
+  }
+}
+
+template <typename T>
+static int coarse_apply(const Amg& g, const T* b, T* x, const int* skip, hipStream_t s) {
+  const T* tiles = coarse_tiles<T>(g);
+  if (tiles && g.tpart && g.tij) {
+    hipLaunchKernelGGL(sym_tile_gemv_kernel<T>, dim3((unsigned)g.ntiles), dim3(256), 0, s, g.n_coarse, tiles, g.tij,
+                       b, g.tpart, skip);
+    hipLaunchKernelGGL(sym_tile_reduce_kernel<T>, dim3(g.nt), dim3(kTile * kRedGroups), 0, s, g.n_coarse, g.nt,
+                       g.tpart, x, skip);
+    return static_cast<int>(hipGetLastError());
+  }
+  const T* inv = coarse_inv<T>(g);
+  if (!inv) return kNotReady;
+  hipLaunchKernelGGL(coarse_gemv_kernel<T>, dim3(std::max(g.n_coarse, 1)), dim3(256), 0, s, g.n_coarse, inv, b, x,
+                     skip);
+  return static_cast<int>(hipGetLastError());
+}
+
 // One V-cycle from level l.  VT: values of this level; VTC: values of every
 // coarser level; T: vector and arithmetic type.
 template <typename VT, typename VTC, typename T>
@@ -78,10 +224,8 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   t.guard = skip ? 1 : 0;
   const long long n = L.n;
   if (l == (int)g.L.size() - 1) {
-    const T* inv = coarse_inv<T>(g);
-    if (L.n != g.n_coarse || !inv) return kNotReady;
-    hipLaunchKernelGGL(coarse_gemv_kernel<T>, dim3(std::max(L.n, 1)), dim3(256), 0, s, L.n, inv, b, x, skip);
-    return static_cast<int>(hipGetLastError());
+    if (L.n != g.n_coarse) return kNotReady;
+    return coarse_apply<T>(g, b, x, skip, s);
   }
   T** v = level_vecs<T>(L);
   T* tmp = v[2];
@@ -175,13 +319,13 @@ static void key_push(std::vector<uint64_t>& k, P v) {
 static void key_matrix(std::vector<uint64_t>& k, const SellMatrix& A) {
   for (int v : {A.br, A.bc, A.n_rows, A.n_cols, A.n_slices, A.sigma, A.lanes}) key_push(k, v);
   key_push(k, A.slots);
-  for (const void* p : {(const void*)A.d_off, (const void*)A.d_len, (const void*)A.d_row, (const void*)A.d_col,
+  for (const void* p : {(const void*)A.d_off, (const void*)A.d_rinfo, (const void*)A.d_col,
                         (const void*)A.d_v64, (const void*)A.d_v64e, (const void*)A.d_v32, (const void*)A.d_v16})
     key_push(k, p);
 }
 
 static void key_tune(std::vector<uint64_t>& k, const SpmvTune& t) {
-  for (int v : {t.waves, t.unroll, t.xcd_remap, t.guard}) key_push(k, v);
+  for (int v : {t.waves, t.unroll, t.xcd_remap, t.waves64, t.unroll64, t.xcd_remap64, t.guard}) key_push(k, v);
   key_push(k, t.flags);
 }
 
@@ -189,8 +333,9 @@ static void key_amg(std::vector<uint64_t>& k, const Amg* g) {
   key_push(k, g);
   if (!g) return;
   for (int v : {g->store, g->n_coarse, (int)g->L.size()}) key_push(k, v);
-  key_push(k, g->inv);
-  key_push(k, g->inv32);
+  for (const void* p : {(const void*)g->inv, (const void*)g->inv32, (const void*)g->tiles64, (const void*)g->tiles32,
+                        (const void*)g->tij, (const void*)g->tpart})
+    key_push(k, p);
   key_tune(k, g->tune);
   for (const auto& lp : g->L) {
     const AmgLevel& l = *lp;
@@ -486,10 +631,7 @@ int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const i
 }
 
 int amg_set_coarse(Amg& g, int n, const double* inv) {
-  if (g.inv) (void)hipFree(g.inv);
-  if (g.inv32) (void)hipFree(g.inv32);
-  g.inv = nullptr;
-  g.inv32 = nullptr;
+  g.release_coarse();
   const size_t nn = (size_t)n * n;
   WN_HIP(hipMalloc(&g.inv, sizeof(double) * nn));
   WN_HIP(hipMemcpy(g.inv, inv, sizeof(double) * nn, hipMemcpyHostToDevice));
@@ -499,7 +641,35 @@ int amg_set_coarse(Amg& g, int n, const double* inv) {
     if (r) return r;
     WN_HIP(hipDeviceSynchronize());
   }
+  // packed upper-triangle tiles for the single-vector cycle
+  g.nt = (n + kTile - 1) / kTile;
+  g.ntiles = (long long)g.nt * (g.nt + 1) / 2;
+  std::vector<int2> tij((size_t)g.ntiles);
+  for (int I = 0; I < g.nt; ++I)
+    for (int J = I; J < g.nt; ++J) tij[(size_t)sym_tile_index(I, J, g.nt)] = make_int2(I, J);
+  const size_t tsz = (size_t)g.ntiles * kTile * kTile;
+  WN_HIP(hipMalloc(&g.tij, sizeof(int2) * tij.size()));
+  WN_HIP(hipMemcpy(g.tij, tij.data(), sizeof(int2) * tij.size(), hipMemcpyHostToDevice));
+  WN_HIP(hipMalloc(&g.tpart, sizeof(double) * 2 * (size_t)g.ntiles * kTile));
+  if (g.store & 1) {
+    WN_HIP(hipMalloc(&g.tiles64, sizeof(double) * tsz));
+    hipLaunchKernelGGL(sym_tile_pack_kernel<double>, dim3((unsigned)g.ntiles), dim3(256), 0, nullptr, n, g.nt, g.inv,
+                       g.tij, g.tiles64);
+    WN_HIP(hipGetLastError());
+  }
+  if (g.store & 6) {
+    WN_HIP(hipMalloc(&g.tiles32, sizeof(float) * tsz));
+    hipLaunchKernelGGL(sym_tile_pack_kernel<float>, dim3((unsigned)g.ntiles), dim3(256), 0, nullptr, n, g.nt, g.inv,
+                       g.tij, g.tiles32);
+    WN_HIP(hipGetLastError());
+  }
+  WN_HIP(hipDeviceSynchronize());
   g.n_coarse = n;
+  return kOk;
+}
+
+int amg_set_tune(Amg& g, int prec, int waves, int xcd_remap, int unroll) {
+  g.tune.set(prec, waves, xcd_remap, unroll);
   return kOk;
 }
 

@@ -30,6 +30,14 @@ struct Amg {
   int n_coarse = 0;
   double* inv = nullptr;   // n_coarse^2, row-major
   float* inv32 = nullptr;  // FP32 copy for the FP32/FP16 cycles (when FP32 values are stored)
+  // upper-triangle 64 x 64 tiles of the inverse (symmetric product), their
+  // (I, J) table and the FP64 partial products
+  int nt = 0;
+  long long ntiles = 0;
+  double* tiles64 = nullptr;
+  float* tiles32 = nullptr;
+  int2* tij = nullptr;
+  double* tpart = nullptr;
   SpmvTune tune;
   ~Amg() { release(); }
   void release() {
@@ -47,10 +55,20 @@ struct Amg {
       if (l.dinv32) (void)hipFree(l.dinv32);
     }
     L.clear();
-    if (inv) (void)hipFree(inv);
-    if (inv32) (void)hipFree(inv32);
+    release_coarse();
+  }
+  void release_coarse() {
+    for (void* p : {(void*)inv, (void*)inv32, (void*)tiles64, (void*)tiles32, (void*)tij, (void*)tpart})
+      if (p) (void)hipFree(p);
     inv = nullptr;
     inv32 = nullptr;
+    tiles64 = nullptr;
+    tiles32 = nullptr;
+    tij = nullptr;
+    tpart = nullptr;
+    nt = 0;
+    ntiles = 0;
+    n_coarse = 0;
   }
 };
 
@@ -173,5 +191,8 @@ int amg_alloc_vectors(Amg& g, int prec_mask);
 template <typename T> const T* coarse_inv(const Amg& g);
 template <> inline const double* coarse_inv<double>(const Amg& g) { return g.inv; }
 template <> inline const float* coarse_inv<float>(const Amg& g) { return g.inv32; }
+template <typename T> const T* coarse_tiles(const Amg& g);
+template <> inline const double* coarse_tiles<double>(const Amg& g) { return g.tiles64; }
+template <> inline const float* coarse_tiles<float>(const Amg& g) { return g.tiles32; }
 
 }  // namespace wn

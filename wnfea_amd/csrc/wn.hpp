@@ -22,8 +22,7 @@ struct SellMatrix {
   int n_rows = 0, n_cols = 0, n_slices = 0, sigma = 0, lanes = 1;
   long long nnzb = 0, slots = 0;
   long long* d_off = nullptr;
-  int* d_len = nullptr;
-  int* d_row = nullptr;
+  int2* d_rinfo = nullptr;  // [n_slices * 64/lanes] {block row or -1, blocks in the row}
   int* d_col = nullptr;
   long long* d_slot = nullptr;  // standard block index -> slot
   double* d_v64 = nullptr;   // primary FP64 values (the assembled K)
@@ -45,22 +44,33 @@ struct SellMatrix {
 };
 
 template <> inline SellView<double> SellMatrix::view<double>() const {
-  return SellView<double>{n_rows, n_slices, lanes, d_off, d_len, d_row, d_col, d_v64};
+  return SellView<double>{n_rows, n_slices, lanes, d_off, d_rinfo, d_col, d_v64};
 }
 template <> inline SellView<float> SellMatrix::view<float>() const {
-  return SellView<float>{n_rows, n_slices, lanes, d_off, d_len, d_row, d_col, d_v32};
+  return SellView<float>{n_rows, n_slices, lanes, d_off, d_rinfo, d_col, d_v32};
 }
 template <> inline SellView<_Float16> SellMatrix::view<_Float16>() const {
-  return SellView<_Float16>{n_rows, n_slices, lanes, d_off, d_len, d_row, d_col, d_v16};
+  return SellView<_Float16>{n_rows, n_slices, lanes, d_off, d_rinfo, d_col, d_v16};
 }
 
 // SpMV launch knobs (picked by the autotuner; defaults are sane).
 struct SpmvTune {
+  // launch shape for 4- and 2-byte values ...
   int waves = 1;               // wavefronts per slice (1, 2, 4)
   int unroll = 1;              // blocks in flight per lane (1, 2, 4; capped by register use)
   int xcd_remap = 1;           // contiguous slice ranges per XCD
+  // ... and for 8-byte values (the best shapes differ: more waves per slice
+  // for the longer FP64 block stream)
+  int waves64 = 1, unroll64 = 1, xcd_remap64 = 1;
   const int* flags = nullptr;  // device loop flags (see guard_skip in sell.hpp)
   int guard = 0;               // 0 always run, 1 skip once the done flag is set
+  template <typename VT> int waves_for() const { return sizeof(VT) == 8 ? waves64 : waves; }
+  template <typename VT> int unroll_for() const { return sizeof(VT) == 8 ? unroll64 : unroll; }
+  template <typename VT> int xcd_for() const { return sizeof(VT) == 8 ? xcd_remap64 : xcd_remap; }
+  void set(int prec, int w, int x, int u) {  // prec 0: 8-byte values, 1: 4/2-byte values, -1: both
+    if (prec <= 0) { waves64 = w; xcd_remap64 = x; unroll64 = u; }
+    if (prec != 0) { waves = w; xcd_remap = x; unroll = u; }
+  }
 };
 
 // -------- element kernels (element.hip)

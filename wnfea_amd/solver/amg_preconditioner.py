@@ -212,7 +212,7 @@ class SmoothedAggregationAMG:
     def __init__(self, fine, A_hat: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None = None,
                  dof_scale: np.ndarray | None = None, theta: float = 0.08, theta_decay: float = 1.0,
                  degree: int = 2, ratio: float = 30.0, coarse_size: int = 12000, max_levels: int = 10,
-                 coarse_degree: int | None = None,
+                 coarse_degree: int | None = None, level_degrees: list | None = None,
                  storage: tuple = ("fp64", "fp32", "fp16"), sigma: int = 4096, waves: int = 1,
                  xcd_remap: bool = True, unroll: int = 1):
         t0 = time.perf_counter()
@@ -225,6 +225,8 @@ class SmoothedAggregationAMG:
         # a lower degree there saves whole kernel launches per cycle)
         self.coarse_degree = degree if coarse_degree is None else coarse_degree
         self.level_degrees = [degree if l < 2 else self.coarse_degree for l in range(len(self.levels))]
+        if level_degrees is not None:  # explicit per-level degrees (the last entry repeats)
+            self.level_degrees = [int(level_degrees[min(l, len(level_degrees) - 1)]) for l in range(len(self.levels))]
         self.fine = fine
         nl = len(self.levels)
         self.storage = tuple(storage)
@@ -272,6 +274,13 @@ class SmoothedAggregationAMG:
                    "wnfea_amg_finalize")
         self._keep = None
         self.setup_sec = time.perf_counter() - t0
+
+    def set_tune(self, precision: str, waves: int, unroll: int, xcd_remap: bool):
+        """SpMV launch shape of the cycle for "fp64" values or for "fp32" (and
+        FP16) values; the constructor's shape applies to both until then."""
+        _lib.check(self.lib.wnfea_amg_tune_precision(self.handle, 0 if precision == "fp64" else 1, int(waves),
+                                                     int(bool(xcd_remap)), int(unroll)),
+                   "wnfea_amg_tune_precision")
 
     def stats(self) -> dict:
         out = []

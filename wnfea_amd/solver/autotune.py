@@ -41,17 +41,23 @@ def _store_cache(cache: dict) -> None:
         pass  # a read-only home only costs a re-tune next time
 
 
+# bumped whenever the SpMV kernels change, so that cached choices of an older
+# build are not reused
+_KERNEL_REV = 2
+
+
 def _key(K, precision: str) -> str:
     import torch
     name = torch.cuda.get_device_name(K.device)
     info = K.info()
-    return f"{name}|{K.br}x{K.bc}|rows={K.n_rows}|nnzb={K.nnzb}|lanes={info['lanes']}|{precision}"
+    return (f"rev{_KERNEL_REV}|{name}|{K.br}x{K.bc}|rows={K.n_rows}|nnzb={K.nnzb}|lanes={info['lanes']}|"
+            f"{precision}")
 
 
 def time_config(K, precision: str, waves: int, unroll: int, xcd: bool, reps: int = 20) -> float:
     """Average seconds per SpMV launch for one launch shape."""
     import torch
-    K.tune(waves, xcd, unroll)
+    K.tune(waves, xcd, unroll, precision=precision)
     dt = torch.float32 if precision == "fp32" else torch.float64
     x = torch.ones(K.shape[1], dtype=dt, device=K.device)
     y = torch.empty(K.shape[0], dtype=dt, device=K.device)
@@ -68,19 +74,20 @@ def time_config(K, precision: str, waves: int, unroll: int, xcd: bool, reps: int
 
 
 def autotune_spmv(K, precision: str = "fp32", reps: int = 20, use_cache: bool = True) -> dict:
-    """Pick and apply the fastest launch shape for K; returns the choice and timings."""
+    """Pick and apply the fastest launch shape for K's `precision` values
+    ("fp32" also sets the FP16 values' shape); returns the choice and timings."""
     key = _key(K, precision)
     cache = _load_cache() if use_cache else {}
     if key in cache:
         c = cache[key]
-        K.tune(c["waves"], c["xcd_remap"], c["unroll"])
+        K.tune(c["waves"], c["xcd_remap"], c["unroll"], precision=precision)
         return dict(c, cached=True)
     table = []
     for waves, unroll, xcd in itertools.product((1, 2, 4), (1, 2, 4), (True, False)):
         table.append({"waves": waves, "unroll": unroll, "xcd_remap": xcd,
                       "seconds": time_config(K, precision, waves, unroll, xcd, reps)})
     best = min(table, key=lambda r: r["seconds"])
-    K.tune(best["waves"], best["xcd_remap"], best["unroll"])
+    K.tune(best["waves"], best["xcd_remap"], best["unroll"], precision=precision)
     choice = {"waves": best["waves"], "unroll": best["unroll"], "xcd_remap": best["xcd_remap"],
               "seconds": best["seconds"]}
     if use_cache:

@@ -152,10 +152,20 @@ class DeviceBSR:
                 "has_fp32": bool(out[5]), "has_fp64_equilibrated": bool(out[6]), "lanes": out[7],
                 "has_fp16": bool(out[8])}
 
-    def tune(self, waves: int = 1, xcd_remap: bool = True, unroll: int = 1):
-        _lib.check(self.lib.wnfea_bsr_tune(self.handle, int(waves), int(bool(xcd_remap)), int(unroll)),
-                   "wnfea_bsr_tune")
-        self.tuning = {"waves": int(waves), "xcd_remap": bool(xcd_remap), "unroll": int(unroll)}
+    def tune(self, waves: int = 1, xcd_remap: bool = True, unroll: int = 1, precision: str | None = None):
+        """SpMV launch shape for every value precision, or for one ("fp64", or
+        "fp32" which also covers the FP16 values)."""
+        shape = {"waves": int(waves), "xcd_remap": bool(xcd_remap), "unroll": int(unroll)}
+        if precision is None:
+            _lib.check(self.lib.wnfea_bsr_tune(self.handle, int(waves), int(bool(xcd_remap)), int(unroll)),
+                       "wnfea_bsr_tune")
+            self.tuning = {"fp64": dict(shape), "fp32": dict(shape)}
+            return
+        key = "fp64" if precision == "fp64" else "fp32"
+        _lib.check(self.lib.wnfea_bsr_tune_precision(self.handle, 0 if key == "fp64" else 1, int(waves),
+                                                     int(bool(xcd_remap)), int(unroll)),
+                   "wnfea_bsr_tune_precision")
+        self.tuning = dict(getattr(self, "tuning", {}) or {}, **{key: shape})
 
     def set_values(self, vals_std, which: int = 3):
         """Values in standard block layout (nnzb, br*bc) FP64 device tensor; which: 1 FP64, 2 FP32, 3 both."""

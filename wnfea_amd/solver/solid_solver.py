@@ -494,11 +494,20 @@ def true_residual(system: SolidSystem, u) -> float:
     return out[0] / out[1] if out[1] > 0 else 0.0
 
 
-def build_amg(system: SolidSystem, storage: tuple = ("fp64", "fp32", "fp16"), **kw):
+def build_amg(system: SolidSystem, storage: tuple = ("fp64", "fp32", "fp16"), autotune: bool = True, **kw):
     """SA-AMG of the system (rigid-body near-null-space from the node
     coordinates; translations only when the system carries none).  ``storage``
-    lists the value precisions the V-cycle may run with."""
+    lists the value precisions the V-cycle may run with.  Unless a launch
+    shape is given (waves/unroll/xcd_remap) or ``autotune`` is off, the SpMV
+    launch shapes of K's FP64 and FP32 values are autotuned (cached per
+    matrix shape) and used by K and by the cycle alike."""
     from .amg_preconditioner import SmoothedAggregationAMG
+    tunes = None
+    if autotune and not any(k in kw for k in ("waves", "unroll", "xcd_remap")):
+        from .autotune import autotune_spmv
+        tunes = {p: autotune_spmv(system.K, p) for p in ("fp64", "fp32")}
+        kw.update(waves=tunes["fp32"]["waves"], unroll=tunes["fp32"]["unroll"],
+                  xcd_remap=tunes["fp32"]["xcd_remap"])
     info = system.K.info()
     need64 = "fp64" in storage and not info["has_fp64_equilibrated"]
     need16 = "fp16" in storage
@@ -513,6 +522,9 @@ def build_amg(system: SolidSystem, storage: tuple = ("fp64", "fp32", "fp16"), **
     amg = SmoothedAggregationAMG(system.K, A_hat, system.coords, system.fixed_mask, dof_scale=dof_scale,
                                  storage=storage, **kw)
     del A_hat
+    if tunes:
+        t = tunes["fp64"]
+        amg.set_tune("fp64", t["waves"], t["unroll"], t["xcd_remap"])
     return amg
 
 
