@@ -154,6 +154,53 @@ def test_stress_field_vs_oracle(c0_model, c0_oracle):
     assert np.abs(nv - nvo).max() <= 1e-9 * nvo.max()
 
 
+def test_vtu_export_stresses_match_oracle(c0_model, c0_oracle, tmp_path):
+    """The .vtu written for the solved C0 box carries the oracle's element and
+    nodal von Mises and element stress tensors (to the file's 9 digits)."""
+    import copy
+    import xml.etree.ElementTree as ET
+    from oracle import oracle as O
+    _, mr, _ = c0_model
+    _, _, u_ref = c0_oracle
+    m = copy.copy(mr)
+    m.displacements = u_ref
+    path = m.export_vtu(tmp_path / "c0.vtu")
+    arrays = {da.get("Name"): np.array(da.text.split(), dtype=np.float64)
+              for da in ET.parse(path).getroot().iter("DataArray")}
+    E, nu = mr.element_moduli()
+    cso, cvo, nvo = O.stress_field(mr.mesh_nodes, mr.solid_elements, E, nu, u_ref)
+    np.testing.assert_allclose(arrays["VonMises"], cvo, rtol=1e-8, atol=1e-9 * cvo.max())
+    np.testing.assert_allclose(arrays["VonMises_Nodal"], nvo, rtol=1e-8, atol=1e-9 * nvo.max())
+    np.testing.assert_allclose(arrays["StressTensor"].reshape(-1, 6), cso, rtol=1e-8, atol=1e-9 * np.abs(cso).max())
+    np.testing.assert_allclose(arrays["Displacement"], u_ref, rtol=1e-8, atol=1e-9 * np.abs(u_ref).max())
+
+
+def test_gmsh_box_solves_like_generated_box(tmp_path):
+    """A C3D10 box written as a gmsh file (type-11 node order) and read back
+    solves to the same displacements as the generated box."""
+    from test_cpu_io import _to_gmsh, _write
+    from wnfea_amd.mesh.gmsh_reader import get_nodes_on_plane, load_msh
+    from wnfea_amd.mesh.solid_generator import generate_c3d10_structured_block
+    from wnfea_amd.solver.solid_solver import assemble_c3d10_sparse_3dof, solve_c3d10_system
+    from wnfea_amd.boundary.conditions import LoadDef, create_fixed_support
+    gen, _ = generate_c3d10_structured_block(0.6, 0.1, 0.1, 12, 2, 2, tip_load_total=-1000.0, tets_per_hex=6,
+                                             load_direction="z")
+    nodes, elems = np.asarray(gen.mesh_nodes), np.asarray(gen.solid_elements, dtype=np.int64)
+    path = tmp_path / "box.msh"
+    _, order = _write(path, nodes, _to_gmsh(nodes, elems), 4, False)
+    n, e, m = load_msh(path, material_name="Steel")
+    root = get_nodes_on_plane(n, "x", 0.0)
+    tip = get_nodes_on_plane(n, "x", 0.6)
+    m.supports = [create_fixed_support(int(i), is_geometry_node=False) for i in root]
+    m.loads = [LoadDef(node_id=int(i), fz=-1000.0 / len(tip), is_geometry_node=False) for i in tip]
+    K, F, _ = assemble_c3d10_sparse_3dof(m)
+    u, _ = solve_c3d10_system(K, F, rtol=1e-10)
+    Kg, Fg, _ = assemble_c3d10_sparse_3dof(gen)
+    ug, _ = solve_c3d10_system(Kg, Fg, rtol=1e-10)
+    u_back = u.reshape(-1, 3)
+    np.testing.assert_allclose(u_back, ug.reshape(-1, 3)[order], rtol=0, atol=1e-8 * np.abs(ug).max())
+
+
 def test_many_mesh_batch_matches_oracle():
     """Six uneven boxes solved as one block-diagonal system: every mesh meets
     1e-8 on its own FP64 residual and matches its own exact solution."""

@@ -27,6 +27,7 @@ class PropertyAssignment:
 
 @dataclass
 class FEAModel:
+    source_file: str = ""
     materials: dict[str, MaterialDef] = field(default_factory=dict)
     sections: dict = field(default_factory=dict)
     mesh_nodes: np.ndarray | None = None
@@ -59,6 +60,23 @@ class FEAModel:
         self.displacements = None
         self.element_results = None
         self.reaction_forces = None
+
+    def clear_mesh(self) -> None:
+        """Drop the mesh and everything downstream; geometry-node supports
+        and loads stay (``model.py:176-188``)."""
+        self.mesh_nodes = None
+        self.mesh_elements = None
+        self.solid_elements = None
+        self.solid_materials.clear()
+        self.element_properties.clear()
+        self.supports = [s for s in self.supports if s.is_geometry_node]
+        self.loads = [ld for ld in self.loads if ld.is_geometry_node]
+        self.clear_results()
+
+    def export_vtu(self, filepath, compute_stresses: bool = True, precision: int = 8) -> str:
+        """ParaView .vtu of this model (``model.py:191-211``)."""
+        from .results.vtu_exporter import VTUExporter
+        return VTUExporter(precision=precision).export(self, filepath, compute_stresses=compute_stresses)
 
     # ---- vectorised views used by the device path (not in the reference) ----
     def element_moduli(self, default_E: float = 2.1e11, default_nu: float = 0.3) -> tuple[np.ndarray, np.ndarray]:
