@@ -54,6 +54,36 @@ int wnfea_c3d10_fint(int M, int n_nodes, const int *conn, const double *nodes, c
                      const double *U, int stride, const int *n2e_ptr, const int *n2e, double *fe_work, double *F,
                      int accumulate, void *stream);
 
+/* Matrix-free C3D10 operator K (FP64) with the assembled matrix's Dirichlet
+ * rule (a DOF flagged in node_fix, bit c of its node's byte, keeps a unit
+ * diagonal and no coupling), stored as per-Gauss-point inverse Jacobians
+ * and weights (320 B per element) instead of K (about 3 KB per element of
+ * FP64 blocks): the reference's matrix-free f_int kernel used as the
+ * operator of a solve (wnfea/solver/native/hip_kernels.cpp:283-425,
+ * c3d10_internal_forces_kernel).  The mesh arrays (device: conn, nodes, E,
+ * nu, node_fix (may be null), n2e_ptr/n2e from wnfea_node_incidence) stay
+ * owned by the caller and must outlive the handle.  *bad = e + 1 for an
+ * element with det J <= 0 (left 0 otherwise; device int). */
+int wnfea_mf_create(int M, int n_nodes, const int *conn, const double *nodes, const double *E, const double *nu,
+                    const unsigned char *node_fix, const int *n2e_ptr, const int *n2e, int *bad, void **handle,
+                    void *stream);
+void wnfea_mf_destroy(void *handle);
+
+/* y = K u, device FP64 vectors of 3 * n_nodes (matrix-free, deterministic). */
+int wnfea_mf_apply(void *handle, const double *u, double *y, void *stream);
+
+/* r = f - K u (matrix-free) and out[0] = ||r||, out[1] = ||f|| (host). */
+int wnfea_mf_residual(void *solver, void *handle, const double *u, const double *f, double *r, double *out,
+                      void *stream);
+
+/* FP64 PCG on the matrix-free operator from x = 0; pkind, dinv, amg, amg_vt
+ * and scale as in wnfea_pcg (the AMG hierarchy still comes from an assembled
+ * operator).  Replaces the operator of the reference's solves,
+ * wnfea/solver/solid_solver.py:197-308. */
+int wnfea_pcg_mf(void *solver, void *handle, int pkind, const double *dinv, void *amg, int amg_vt,
+                 const double *scale, const double *b, double *x, double rtol, int max_iter, int check_every,
+                 int *iters, double *rel, void *stream);
+
 /* Gauss-point stresses (optional gp_sig[M][4][6], gp_vm[M][4]), cell means
  * cell_sig[M][6], cell_vm[M], and (if nodal_vm) the nodal average of cell_vm.
  * U has 3 doubles per node.  Replaces element_stresses_c3d10 and

@@ -327,3 +327,35 @@ def test_coarse_only_cycle_is_inverse_product(storage):
         z = amg.vcycle(torch.as_tensor(r32, device="cuda")).double().cpu().numpy()
         assert np.max(np.abs(z - z_ref) / np.maximum(np.abs(z_ref), 1e-30 + np.abs(z_ref).max() * 1e-6)) <= 2e-7
     amg.close()
+
+
+def test_matrix_free_operator_matches_oracle_matrix(c0_system, c0_oracle):
+    """The matrix-free K (quadrature data + on-the-fly B^T D B, Dirichlet rule
+    of the assembled matrix) applies the oracle's assembled CSR K, and its
+    residual equals the assembled one."""
+    import torch
+    from wnfea_amd.solver.solid_solver import MatrixFreeOperator, true_residual
+    K, F, u_ref = c0_oracle
+    op = MatrixFreeOperator(c0_system)
+    rng = np.random.default_rng(11)
+    for _ in range(2):
+        u = rng.standard_normal(c0_system.n_dofs)
+        y = op.apply(torch.as_tensor(u, device="cuda")).cpu().numpy()
+        y_ref = K @ u
+        assert _rel(y, y_ref) <= 1e-13
+    ud = torch.as_tensor(u_ref + 1e-6 * rng.standard_normal(len(u_ref)) * np.abs(u_ref).max(), device="cuda")
+    assert abs(op.residual(ud) - true_residual(c0_system, ud)) <= 1e-10 * true_residual(c0_system, ud)
+    op.close()
+
+
+@pytest.mark.parametrize("amg_precision", ["fp64", "fp32"])
+def test_matrix_free_solve_matches_oracle(c0_system, c0_amg, c0_oracle, amg_precision):
+    from wnfea_amd.solver.solid_solver import MatrixFreeOperator, solve_mixed, true_residual
+    _, _, u_ref = c0_oracle
+    op = MatrixFreeOperator(c0_system)
+    u, it, steps, rel, _ = solve_mixed(c0_system, c0_amg, tol=1e-8, amg_precision=amg_precision, operator=op)
+    assert rel <= 1e-8 and true_residual(c0_system, u) <= 1e-8
+    assert _rel(u.cpu().numpy(), u_ref) <= (1e-8 if amg_precision == "fp64" else 1e-6)
+    u2, it2, _, _, _ = solve_mixed(c0_system, c0_amg, tol=1e-8, amg_precision=amg_precision)
+    assert abs(it - it2) <= 2
+    op.close()

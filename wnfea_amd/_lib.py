@@ -66,6 +66,11 @@ SIGNATURES = {
     "wnfea_pcg": (_i, [_p, _p, _i, _i, _p, _p, _i, _p, _p, _p, _d, _i, _i, _pi, _pd, _p]),
     "wnfea_ir": (_i, [_p, _p, _p, _i, _p, _p, _i, _p, _p, _d, _d, _i, _i, _i, _i, _pi, _pi, _pd, _p]),
     "wnfea_residual": (_i, [_p, _p, _p, _p, _p, _pd, _p]),
+    "wnfea_mf_create": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, ctypes.POINTER(_p), _p]),
+    "wnfea_mf_destroy": (None, [_p]),
+    "wnfea_mf_apply": (_i, [_p, _p, _p, _p]),
+    "wnfea_mf_residual": (_i, [_p, _p, _p, _p, _p, _pd, _p]),
+    "wnfea_pcg_mf": (_i, [_p, _p, _i, _p, _p, _i, _p, _p, _p, _d, _i, _i, _pi, _pd, _p]),
     "wnfea_pcg_multi": (_i, [_p, _p, _p, _i, _p, _i, _p, _p, _d, _i, _i, _pi, _pd, _p]),
     "wnfea_bsr_spmm": (_i, [_p, _i, _i, _p, _p, _p]),
 }

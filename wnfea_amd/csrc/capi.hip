@@ -31,7 +31,7 @@ int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream
 template <typename T>
 int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, int amg_vt, const double* scale,
             const T* b, T* x, double rtol, int max_iter, int check_every, const SpmvTune& tune0, hipStream_t s,
-            int* iters, double* rel);
+            int* iters, double* rel, const MfOp* mf = nullptr);
 int ir_run(Solver& S, SellMatrix& A, const double* scale, int pkind, const float* dinv32, Amg* amg, int amg_vt,
            const double* f, double* u, double tol, double inner_rtol, int max_outer, int max_inner, int check_every,
            const SpmvTune& tune, hipStream_t s, int* outer_out, int* inner_out, double* rel_out, int zero_init);
@@ -443,6 +443,83 @@ int wnfea_bsr_spmm(void* h, int prec, int nrhs, const void* X, void* Y, void* st
   if (!h || !X || !Y) return kBadArg;
   auto* b = static_cast<WnBsr*>(h);
   return bsr_spmm(b->m, prec, nrhs, X, Y, b->tune, S(stream));
+}
+
+// ---------------------------------------------------------------- matrix-free operator
+struct WnMf {
+  MfOp op;
+};
+
+int wnfea_mf_create(int M, int n_nodes, const int* conn, const double* nodes, const double* E, const double* nu,
+                    const unsigned char* node_fix, const int* n2e_ptr, const int* n2e, int* bad, void** out,
+                    void* stream) {
+  if (!out) return kBadArg;
+  *out = nullptr;
+  if (M < 0 || n_nodes <= 0 || !n2e_ptr || (M > 0 && (!conn || !nodes || !E || !nu || !n2e))) return kBadArg;
+  auto* h = new WnMf();
+  MfOp& op = h->op;
+  op.M = M;
+  op.n_nodes = n_nodes;
+  op.conn = conn;
+  op.E = E;
+  op.nu = nu;
+  op.fix = node_fix;
+  op.n2e_ptr = n2e_ptr;
+  op.n2e = n2e;
+  const int r = mf_setup(op, nodes, bad, S(stream));
+  if (r) {
+    mf_release(op);
+    delete h;
+    return r;
+  }
+  *out = h;
+  return kOk;
+}
+
+void wnfea_mf_destroy(void* h) {
+  if (!h) return;
+  auto* m = static_cast<WnMf*>(h);
+  mf_release(m->op);
+  delete m;
+}
+
+// y = K u (device FP64 vectors of 3 * n_nodes)
+int wnfea_mf_apply(void* h, const double* u, double* y, void* stream) {
+  if (!h || !u || !y) return kBadArg;
+  return mf_apply(static_cast<WnMf*>(h)->op, u, nullptr, y, nullptr, 0, nullptr, 0, S(stream));
+}
+
+// r = f - K u and out[0] = ||r||, out[1] = ||f|| (as wnfea_residual, matrix-free)
+int wnfea_mf_residual(void* solver, void* h, const double* u, const double* f, double* r, double* out, void* stream) {
+  if (!solver || !h || !u || !f || !r || !out) return kBadArg;
+  const MfOp& op = static_cast<WnMf*>(h)->op;
+  Workspace& w = solver_ws(*static_cast<Solver*>(solver));
+  const int nb = mf_grid(op);
+  int e = w.alloc(nb + 8);
+  if (e) return e;
+  if ((e = mf_apply(op, u, f, r, w.partials, 2, nullptr, 0, S(stream)))) return e;
+  if ((e = launch_finalize(w.partials, nb, 1, kFinStore, w.sc + kScTrue, w.sc, nullptr, 0, nullptr, S(stream))))
+    return e;
+  if ((e = launch_dot<double>(3LL * op.n_nodes, f, f, w, w.sc + kScF2, S(stream)))) return e;
+  double hs[kScCount];
+  WN_HIP(hipMemcpyAsync(hs, w.sc, sizeof(double) * kScCount, hipMemcpyDeviceToHost, S(stream)));
+  WN_HIP(hipStreamSynchronize(S(stream)));
+  out[0] = std::sqrt(hs[kScTrue]);
+  out[1] = std::sqrt(hs[kScF2]);
+  return kOk;
+}
+
+// FP64 PCG with the matrix-free operator (pkind and preconditioner as in wnfea_pcg)
+int wnfea_pcg_mf(void* solver, void* h, int pkind, const double* dinv, void* amg, int amg_vt, const double* scale,
+                 const double* b, double* x, double rtol, int max_iter, int check_every, int* iters, double* rel,
+                 void* stream) {
+  if (!solver || !h || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2 || amg_vt < 0 || amg_vt > 4)
+    return kBadArg;
+  if ((pkind == 1 && !dinv) || (pkind == 2 && !amg)) return kBadArg;
+  static const SellMatrix none;
+  SpmvTune t;
+  return pcg_run<double>(*static_cast<Solver*>(solver), none, pkind, dinv, static_cast<Amg*>(amg), amg_vt, scale, b,
+                         x, rtol, max_iter, check_every, t, S(stream), iters, rel, &static_cast<WnMf*>(h)->op);
 }
 
 // r = f - K u (FP64 primary values) and out[0] = ||r||, out[1] = ||f||.

@@ -16,6 +16,7 @@
 // block pair (a <= b) produces a full 3x3 block and also writes its transpose:
 // Ke comes out exactly symmetric, as the reference's 0.5 (Ke + Ke^T).
 #include "common.hpp"
+#include "wn.hpp"
 
 namespace wn {
 
@@ -357,6 +358,167 @@ __global__ void adjoint_dot_kernel(int M, const int* __restrict__ conn, const do
   grad[e] = -s / E[e];
 }
 
+// ---------------------------------------------------------------- matrix-free K u
+// Quadrature data of the matrix-free operator ("partial assembly"): for each
+// element and Gauss point g the inverse Jacobian J^-1 (row-major) and
+// det J / 24, stored qd[(g*10 + k)*M + e] so that the threads of a block read
+// them coalesced.  A non-positive det J flags the element (bad = e + 1).
+__global__ __launch_bounds__(128) void c3d10_qdata_kernel(int M, const int* __restrict__ conn,
+                                                          const double* __restrict__ nodes, double* __restrict__ qd,
+                                                          int* bad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M) return;
+  double X[10][3];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const int n = conn[(size_t)e * 10 + i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) X[i][c] = nodes[(size_t)n * 3 + c];
+  }
+  for (int g = 0; g < 4; ++g) {
+    double J[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 10; ++i) s = fma(c_dndxi[g][r][i], X[i][c], s);
+        J[r][c] = s;
+      }
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    if (!(det > 0.0) && bad) atomicCAS(bad, 0, e + 1);
+    const double id = det != 0.0 ? 1.0 / det : 0.0;
+    const double inv[9] = {c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id,
+                           (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id, c01 * id,
+                           (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id,
+                           c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id,
+                           (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) qd[((size_t)g * 10 + k) * M + e] = inv[k];
+    qd[((size_t)g * 10 + 9) * M + e] = det / 24.0;
+  }
+}
+
+// fe[e] = K_e (m .* u)_e from the quadrature data, where m zeroes the DOFs
+// marked in node_fix (bit c: DOF c of the node fixed).  One thread per
+// element; the B matrix rows are formed per node on the fly, twice per Gauss
+// point (strain, then B^T sigma), so only u_e, the 30 outputs and one Gauss
+// point's data are live.
+__global__ __launch_bounds__(128) void c3d10_pa_elem_kernel(int M, const int* __restrict__ conn,
+                                                            const double* __restrict__ qd,
+                                                            const double* __restrict__ E,
+                                                            const double* __restrict__ nu,
+                                                            const unsigned char* __restrict__ fix,
+                                                            const double* __restrict__ U, double* __restrict__ fe,
+                                                            const int* flags, int guard) {
+  if (guard && flags[0]) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= M) return;
+  double u[10][3], fo[10][3];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const int n = conn[(size_t)e * 10 + i];
+    const int m = fix ? fix[n] : 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      u[i][c] = (m >> c) & 1 ? 0.0 : U[(size_t)n * 3 + c];
+      fo[i][c] = 0.0;
+    }
+  }
+  const double Ev = E[e], nv = nu[e];
+  const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
+  const double c11 = f * (1.0 - nv), c12 = f * nv, c44 = Ev / (2.0 * (1.0 + nv));
+  for (int g = 0; g < 4; ++g) {
+    double inv[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) inv[k] = qd[((size_t)g * 10 + k) * M + e];
+    const double wd = qd[((size_t)g * 10 + 9) * M + e];
+    double eps[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      const double d0 = c_dndxi[g][0][i], d1 = c_dndxi[g][1][i], d2 = c_dndxi[g][2][i];
+      const double gx = inv[0] * d0 + inv[1] * d1 + inv[2] * d2;
+      const double gy = inv[3] * d0 + inv[4] * d1 + inv[5] * d2;
+      const double gz = inv[6] * d0 + inv[7] * d1 + inv[8] * d2;
+      eps[0] += gx * u[i][0];
+      eps[1] += gy * u[i][1];
+      eps[2] += gz * u[i][2];
+      eps[3] += gy * u[i][0] + gx * u[i][1];
+      eps[4] += gz * u[i][1] + gy * u[i][2];
+      eps[5] += gz * u[i][0] + gx * u[i][2];
+    }
+    const double s0 = wd * (c11 * eps[0] + c12 * (eps[1] + eps[2]));
+    const double s1 = wd * (c11 * eps[1] + c12 * (eps[0] + eps[2]));
+    const double s2 = wd * (c11 * eps[2] + c12 * (eps[0] + eps[1]));
+    const double s3 = wd * c44 * eps[3], s4 = wd * c44 * eps[4], s5 = wd * c44 * eps[5];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      const double d0 = c_dndxi[g][0][i], d1 = c_dndxi[g][1][i], d2 = c_dndxi[g][2][i];
+      const double gx = inv[0] * d0 + inv[1] * d1 + inv[2] * d2;
+      const double gy = inv[3] * d0 + inv[4] * d1 + inv[5] * d2;
+      const double gz = inv[6] * d0 + inv[7] * d1 + inv[8] * d2;
+      fo[i][0] += gx * s0 + gy * s3 + gz * s5;
+      fo[i][1] += gy * s1 + gx * s3 + gz * s4;
+      fo[i][2] += gz * s2 + gy * s4 + gx * s5;
+    }
+  }
+  double* o = fe + (size_t)e * 30;
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[3 * i + c] = fo[i][c];
+}
+
+// Node-wise deterministic gather with the assembled operator's Dirichlet rule
+// (a fixed DOF has a unit diagonal and no coupling): for every DOF d of node n
+//   mode 0: y_d = (K u)_d;   mode 1: y_d = (K u)_d and partial u.y;
+//   mode 2: y_d = f_d - (K u)_d and partial y.y,
+// where (K u)_d = u_d for a fixed DOF, else the sum of the node's element
+// contributions in element order.
+__global__ __launch_bounds__(256) void mf_gather_kernel(int n_nodes, const int* __restrict__ n2e_ptr,
+                                                        const int* __restrict__ n2e, const double* __restrict__ fe,
+                                                        const unsigned char* __restrict__ fix,
+                                                        const double* __restrict__ u, const double* __restrict__ f,
+                                                        double* __restrict__ y, double* partials, int mode,
+                                                        const int* flags, int guard) {
+  if (guard && flags[0]) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double dot = 0.0;
+  if (n < n_nodes) {
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; ++k) {
+      const int code = n2e[k];  // e * 10 + local
+      const double* p = fe + (size_t)(code / 10) * 30 + 3 * (code % 10);
+      s[0] += p[0];
+      s[1] += p[1];
+      s[2] += p[2];
+    }
+    const int m = fix ? fix[n] : 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t d = (size_t)n * 3 + c;
+      const double ku = (m >> c) & 1 ? u[d] : s[c];
+      if (mode == 2) {
+        const double r = f[d] - ku;
+        y[d] = r;
+        dot += r * r;
+      } else {
+        y[d] = ku;
+        if (mode == 1) dot += u[d] * ku;
+      }
+    }
+  }
+  if (mode != 0) {
+    __shared__ double sm[4];
+    double v[1] = {block_sum(dot, sm)};
+    store_block_partials<1>(v, partials);
+  }
+}
+
 // ---------------------------------------------------------------- host wrappers
 int launch_ke(int M, const int* conn, const double* nodes, const double* E, const double* nu, double* Ke, int* bad,
               hipStream_t s) {
@@ -383,6 +545,37 @@ int launch_gather_node(int n_nodes, const int* n2e_ptr, const int* n2e, const do
   if (n_nodes <= 0) return kOk;
   hipLaunchKernelGGL(gather_node_kernel, dim3(ceil_div(n_nodes, 256)), dim3(256), 0, s, n_nodes, n2e_ptr, n2e, fe,
                      out, stride, accumulate);
+  return static_cast<int>(hipGetLastError());
+}
+
+int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s) {
+  mf_release(op);
+  if (op.M <= 0) return kOk;
+  int r = upload_element_consts(s);
+  if (r) return r;
+  WN_HIP(hipMalloc(&op.qd, sizeof(double) * 40 * (size_t)op.M));
+  WN_HIP(hipMalloc(&op.fe, sizeof(double) * 30 * (size_t)op.M));
+  hipLaunchKernelGGL(c3d10_qdata_kernel, dim3(ceil_div(op.M, 128)), dim3(128), 0, s, op.M, op.conn, nodes, op.qd, bad);
+  return static_cast<int>(hipGetLastError());
+}
+
+void mf_release(MfOp& op) {
+  if (op.qd) (void)hipFree(op.qd);
+  if (op.fe) (void)hipFree(op.fe);
+  op.qd = op.fe = nullptr;
+}
+
+int mf_grid(const MfOp& op) { return std::max(1, ceil_div(op.n_nodes, 256)); }
+
+int mf_apply(const MfOp& op, const double* u, const double* f, double* y, double* partials, int mode,
+             const int* flags, int guard, hipStream_t s) {
+  if (op.M > 0) {
+    if (!op.qd || !op.fe) return kNotReady;
+    hipLaunchKernelGGL(c3d10_pa_elem_kernel, dim3(ceil_div(op.M, 128)), dim3(128), 0, s, op.M, op.conn, op.qd, op.E,
+                       op.nu, op.fix, u, op.fe, flags, guard);
+  }
+  hipLaunchKernelGGL(mf_gather_kernel, dim3(mf_grid(op)), dim3(256), 0, s, op.n_nodes, op.n2e_ptr, op.n2e, op.fe,
+                     op.fix, u, f, y, partials, mode, flags, guard);
   return static_cast<int>(hipGetLastError());
 }
 

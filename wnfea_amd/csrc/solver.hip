@@ -384,13 +384,15 @@ static int graph_for(Solver& S, const std::vector<uint64_t>& key, F&& enqueue) {
 template <typename T>
 int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, int amg_vt, const double* scale,
             const T* b, T* x, double rtol, int max_iter, int check_every, const SpmvTune& tune0, hipStream_t s,
-            int* iters, double* rel) {
+            int* iters, double* rel, const MfOp* mf = nullptr) {
   KVec<T>& k = *reinterpret_cast<KVec<T>*>(std::is_same<T, double>::value ? (void*)&S.k64 : (void*)&S.k32);
-  const long long n = (long long)A.n_rows * A.br;
+  // with mf the operator is matrix-free (FP64 vectors only) and A is unused
+  if (mf && !std::is_same<T, double>::value) return kBadMode;
+  const long long n = mf ? 3LL * mf->n_nodes : (long long)A.n_rows * A.br;
   int r;
   if ((r = S.init())) return r;
   if ((r = k.ensure(n))) return r;
-  if ((r = S.ws.alloc(A.n_slices + 8))) return r;
+  if ((r = S.ws.alloc((mf ? mf_grid(*mf) : A.n_slices) + 8))) return r;
   SpmvTune t = tune0;
   t.flags = S.flags;
   t.guard = 1;
@@ -435,7 +437,16 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
       if ((e = launch_rho<T, T>(n, k.r, k.r, nullptr, S.ws, first, S.flags, st))) return e;
       if ((e = launch_update_p<T, T>(n, k.r, nullptr, k.p, S.ws, first, S.flags, st))) return e;
     }
-    if ((e = spmv_dot<T>(A, k.p, k.q, S.ws, 1, t, st))) return e;
+    if (mf) {
+      const double* pp = reinterpret_cast<const double*>(k.p);
+      double* qq = reinterpret_cast<double*>(k.q);
+      if ((e = mf_apply(*mf, pp, nullptr, qq, S.ws.partials, 1, S.flags, 1, st))) return e;
+      if ((e = launch_finalize(S.ws.partials, mf_grid(*mf), 1, kFinGamma, S.ws.sc + kScGamma, S.ws.sc, nullptr, 0,
+                               S.flags, st)))
+        return e;
+    } else if ((e = spmv_dot<T>(A, k.p, k.q, S.ws, 1, t, st))) {
+      return e;
+    }
     if (sep32) return launch_update_xr<T, float>(n, x, k.r, k.p, k.q, rp32, scale, S.ws, S.flags, st);
     if (sep64) return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, k.z, scale, S.ws, S.flags, st);
     return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, nullptr, nullptr, S.ws, S.flags, st);
@@ -449,7 +460,13 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
                           (const void*)S.ws.partials, (const void*)S.ws.sc, (const void*)S.flags})
       key_push(key, p);
     key_push(key, n);
-    key_matrix(key, A);
+    if (mf) {
+      for (const void* p : {(const void*)mf->qd, (const void*)mf->fe, (const void*)mf->conn, (const void*)mf->fix,
+                            (const void*)mf->E, (const void*)mf->nu, (const void*)mf->n2e})
+        key_push(key, p);
+    } else {
+      key_matrix(key, A);
+    }
     key_tune(key, t);
     key_amg(key, pkind == 2 ? amg : nullptr);
   }
@@ -483,9 +500,9 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
   return kOk;
 }
 template int pcg_run<double>(Solver&, const SellMatrix&, int, const double*, Amg*, int, const double*, const double*,
-                             double*, double, int, int, const SpmvTune&, hipStream_t, int*, double*);
+                             double*, double, int, int, const SpmvTune&, hipStream_t, int*, double*, const MfOp*);
 template int pcg_run<float>(Solver&, const SellMatrix&, int, const float*, Amg*, int, const double*, const float*,
-                            float*, double, int, int, const SpmvTune&, hipStream_t, int*, double*);
+                            float*, double, int, int, const SpmvTune&, hipStream_t, int*, double*, const MfOp*);
 
 // ------------------------------------------------------------------ refinement
 // FP64 iterative refinement around an FP32 (equilibrated) inner PCG:

@@ -77,6 +77,25 @@ struct SpmvTune {
 int upload_element_consts(hipStream_t s);
 int launch_ke(int M, const int* conn, const double* nodes, const double* E, const double* nu, double* Ke, int* bad,
               hipStream_t s);
+// Matrix-free C3D10 operator with the assembled matrix's Dirichlet rule.
+// Mesh arrays are borrowed (device); quadrature data and the element work
+// buffer are owned.
+struct MfOp {
+  int M = 0, n_nodes = 0;
+  const int* conn = nullptr;
+  const double* E = nullptr;
+  const double* nu = nullptr;
+  const unsigned char* fix = nullptr;  // per node, bit c: DOF c fixed (may be null)
+  const int* n2e_ptr = nullptr;
+  const int* n2e = nullptr;
+  double* qd = nullptr;  // [4][10][M] inverse Jacobians and det/24
+  double* fe = nullptr;  // [M][30]
+};
+int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s);
+void mf_release(MfOp& op);
+int mf_grid(const MfOp& op);  // blocks of the gather (= partial count of modes 1 and 2)
+int mf_apply(const MfOp& op, const double* u, const double* f, double* y, double* partials, int mode,
+             const int* flags, int guard, hipStream_t s);
 int launch_fint_elem(int M, const int* conn, const double* nodes, const double* E, const double* nu, const double* U,
                      int stride, double* fe, hipStream_t s);
 int launch_gather_node(int n_nodes, const int* n2e_ptr, const int* n2e, const double* fe, double* out, int stride,

@@ -354,8 +354,68 @@ def pcg_device(system: SolidSystem, b=None, rtol: float = 1e-8, max_iter: int = 
     return x, it.value, rel.value, dt_s
 
 
+class MatrixFreeOperator:
+    """K of a C3D10 system applied without the assembled matrix: per Gauss
+    point inverse Jacobians and weights (320 B per element, "partial
+    assembly") and the element contraction B^T D B on the fly, with the
+    assembled matrix's Dirichlet rule (fixed DOF: unit diagonal, no
+    coupling).  The reference's matrix-free f_int kernel used as a solve
+    operator (``hip_kernels.cpp:283-425``): C1 needs 67 MB of quadrature data
+    instead of 635 MB of FP64 blocks."""
+
+    def __init__(self, system: SolidSystem):
+        import torch
+        if system.dpn != 3 or not system.mesh:
+            raise ValueError("the matrix-free operator needs a C3D10 system with its device mesh")
+        self.lib = _lib.load()
+        self.system = system
+        m = system.mesh
+        bits = np.asarray(system.fixed_mask, dtype=np.uint8).reshape(-1, 3) * np.array([1, 2, 4], dtype=np.uint8)
+        self.fix = torch.as_tensor(bits.sum(axis=1).astype(np.uint8), device=m["nodes"].device)
+        bad = torch.zeros(1, dtype=torch.int32, device=m["nodes"].device)
+        h = _lib.ctypes.c_void_p()
+        _lib.check(self.lib.wnfea_mf_create(m["n_elements"], m["n_nodes"], _lib.dptr(m["conn"]), _lib.dptr(m["nodes"]),
+                                            _lib.dptr(m["E"]), _lib.dptr(m["nu"]), _lib.dptr(self.fix),
+                                            _lib.dptr(m["n2e_ptr"]), _lib.dptr(m["n2e"]), _lib.dptr(bad),
+                                            _lib.ctypes.byref(h), _lib.stream_of()), "wnfea_mf_create")
+        self.handle = h.value
+        if int(bad.item()):
+            self.close()
+            raise ValueError(f"Non-positive Jacobian determinant in C3D10 element {int(bad.item()) - 1}")
+        self.n = 3 * m["n_nodes"]
+
+    def apply(self, u, y=None):
+        import torch
+        y = torch.empty_like(u) if y is None else y
+        _lib.check(self.lib.wnfea_mf_apply(self.handle, _lib.dptr(u), _lib.dptr(y), _lib.stream_of()),
+                   "wnfea_mf_apply")
+        return y
+
+    def residual(self, u, f=None) -> float:
+        """||f - K u|| / ||f|| (f defaults to the system's F)."""
+        import torch
+        f = self.system.F if f is None else f
+        r = torch.empty_like(f)
+        out = (_lib.ctypes.c_double * 2)()
+        _lib.check(self.lib.wnfea_mf_residual(_Solver.get(), self.handle, _lib.dptr(u), _lib.dptr(f), _lib.dptr(r),
+                                              out, _lib.stream_of()), "wnfea_mf_residual")
+        return out[0] / out[1] if out[1] > 0 else 0.0
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.wnfea_mf_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def solve_mixed(system: SolidSystem, amg, tol: float = 1e-8, amg_precision: str = "fp32", max_iter: int = 2000,
-                check_every: int = 16, margin: float = 0.5, max_restarts: int = 3):
+                check_every: int = 16, margin: float = 0.5, max_restarts: int = 3,
+                operator: MatrixFreeOperator | None = None):
     """Mixed-precision solve of the resident system to ||F - K u|| <= tol ||F||.
 
     FP64 CG on K (FP64 SpMV, FP64 alpha/beta, FP64 solution update) with the
@@ -364,7 +424,9 @@ def solve_mixed(system: SolidSystem, amg, tol: float = 1e-8, amg_precision: str 
     true residual is verified afterwards; if it is above tol (recursion drift)
     CG is restarted on the true residual (a refinement step).  Restarts stop
     early once a step no longer reduces the true residual by half: the FP64
-    residual has then reached its round-off floor.
+    residual has then reached its round-off floor.  With ``operator`` (a
+    MatrixFreeOperator of the system) CG's products, the true residual and
+    the restarts use the matrix-free K; the V-cycle is unchanged.
     Returns (u, iterations, CG runs, ||F-Ku||/||F||, seconds)."""
     t0 = time.perf_counter()
     u = None
@@ -373,18 +435,35 @@ def solve_mixed(system: SolidSystem, amg, tol: float = 1e-8, amg_precision: str 
     b = system.F
     rel = float("inf")
     for steps in range(1, max_restarts + 2):
-        x, it, _, _ = pcg_device(system, b=b, rtol=margin * tol * (1.0 if u is None else
-                                                                    float(system.F.norm() / b.norm())),
-                                 max_iter=max_iter, preconditioner="amg", precision="fp64", amg=amg,
-                                 amg_precision=amg_precision, check_every=check_every)
+        rt = margin * tol * (1.0 if u is None else float(system.F.norm() / b.norm()))
+        if operator is None:
+            x, it, _, _ = pcg_device(system, b=b, rtol=rt, max_iter=max_iter, preconditioner="amg", precision="fp64",
+                                     amg=amg, amg_precision=amg_precision, check_every=check_every)
+        else:
+            x, it = _pcg_mf(system, operator, amg, b, rt, max_iter, amg_precision, check_every)
         iters += it
         u = x if u is None else u + x
-        prev, rel = rel, true_residual(system, u)
+        prev, rel = rel, (true_residual(system, u) if operator is None else operator.residual(u))
         if rel <= tol or steps > max_restarts or rel > 0.5 * prev:
             break
-        b = system.F - system.K.spmv(u)
+        b = system.F - (system.K.spmv(u) if operator is None else operator.apply(u))
     _sync()
     return u, iters, steps, rel, time.perf_counter() - t0
+
+
+def _pcg_mf(system: SolidSystem, operator: MatrixFreeOperator, amg, b, rtol: float, max_iter: int,
+            amg_precision: str, check_every: int):
+    """FP64 AMG-PCG from x = 0 on the matrix-free operator; (x, iterations)."""
+    import torch
+    L = _lib.load()
+    x = torch.empty_like(b)
+    it = _lib.ctypes.c_int(0)
+    rel = _lib.ctypes.c_double(0.0)
+    _lib.check(L.wnfea_pcg_mf(_Solver.get(), operator.handle, 2, None, amg.handle, amg.VT_CODE[amg_precision],
+                              _lib.dptr(system.scale), _lib.dptr(b), _lib.dptr(x), float(rtol), int(max_iter),
+                              int(check_every), _lib.ctypes.byref(it), _lib.ctypes.byref(rel), _lib.stream_of()),
+               "wnfea_pcg_mf")
+    return x, it.value
 
 
 def solve_multi(system: SolidSystem, amg, B, tol: float = 1e-8, group: int = 8, max_iter: int = 2000,
