@@ -426,6 +426,8 @@ def main() -> int:
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine", "fp16_coarse"], default="fp32")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
+    ap.add_argument("--operator", choices=["matrix_free", "assembled"], default="matrix_free",
+                    help="CG's operator K: matrix-free (quadrature data) or the assembled FP64 BSR matrix")
     ap.add_argument("--spmv-shape", default=None,
                     help="waves,unroll,xcd_remap of the SpMV instead of autotuning, for both value precisions "
                          "(e.g. 4,1,0) or as fp32:fp64 (e.g. 2,2,1:4,1,1)")
@@ -489,11 +491,19 @@ def main() -> int:
     log(f"[bench] amg setup {t_amg:.1f}s levels {[l['n'] for l in st['levels']]} "
         f"op-complexity {st['operator_complexity']:.2f}")
 
+    # CG's operator: matrix-free K (default) or the assembled FP64 BSR K;
+    # every result is verified with the assembled FP64 K
+    from wnfea_amd.solver.solid_solver import MatrixFreeOperator
+    mf_op = MatrixFreeOperator(system)
+    main_op = mf_op if args.operator == "matrix_free" else None
+    alt_op = None if args.operator == "matrix_free" else mf_op
+    alt_name = "assembled" if args.operator == "matrix_free" else "matrix_free"
+
     def mixed_solve():
-        return solve_mixed(system, amg, tol=1e-8, amg_precision=args.amg_precision)
+        return solve_mixed(system, amg, tol=1e-8, amg_precision=args.amg_precision, operator=main_op)
 
     def fp64_solve():
-        return solve_mixed(system, amg, tol=1e-8, amg_precision="fp64")
+        return solve_mixed(system, amg, tol=1e-8, amg_precision="fp64", operator=main_op)
 
     for _ in range(args.warmup):
         mixed_solve()
@@ -504,8 +514,8 @@ def main() -> int:
     t_mixed = (time.perf_counter() - t0) / args.steps
     u, iters, steps, rel, _ = res[-1]
     rel_true = true_residual(system, u)
-    log(f"[bench] mixed (FP64 CG + {args.amg_precision} V-cycle): {t_mixed * 1e3:.1f} ms/solve iterations {iters} "
-        f"restarts {steps - 1} rel {rel_true:.2e}")
+    log(f"[bench] mixed (FP64 CG on {args.operator} K + {args.amg_precision} V-cycle): {t_mixed * 1e3:.1f} ms/solve "
+        f"iterations {iters} restarts {steps - 1} rel {rel_true:.2e}")
     # all-FP64 AMG-PCG on the same hierarchy
     fp64_solve()
     torch.cuda.synchronize()
@@ -515,20 +525,35 @@ def main() -> int:
     torch.cuda.synchronize()
     t_fp64 = (time.perf_counter() - t0) / max(1, args.steps // 2)
     rel64 = true_residual(system, x64)
-    log(f"[bench] fp64 AMG-PCG: {t_fp64 * 1e3:.1f} ms/solve iters {it64} rel {rel64:.2e}")
+    log(f"[bench] fp64 AMG-PCG ({args.operator} K): {t_fp64 * 1e3:.1f} ms/solve iters {it64} rel {rel64:.2e}")
     err_u = float(torch.linalg.norm(u - x64) / torch.linalg.norm(x64))
     others = {}
     for other in ("fp32", "fp16", "fp16_fine", "fp16_coarse"):
         if other == args.amg_precision:
             continue
-        solve_mixed(system, amg, tol=1e-8, amg_precision=other)
+        solve_mixed(system, amg, tol=1e-8, amg_precision=other, operator=main_op)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        uo, ito, _, relo, _ = solve_mixed(system, amg, tol=1e-8, amg_precision=other)
+        uo, ito, _, relo, _ = solve_mixed(system, amg, tol=1e-8, amg_precision=other, operator=main_op)
         torch.cuda.synchronize()
         t_other = time.perf_counter() - t0
         others[f"mixed_{other}_vcycle"] = {"seconds": round(t_other, 5), "iterations": ito, "rel_res": relo}
         log(f"[bench] mixed with {other} V-cycle: {t_other * 1e3:.1f} ms iters {ito} rel {relo:.2e}")
+    # the same two solves with the other operator for CG
+    alt = {"operator": alt_name}
+    for name, prec in (("mixed", args.amg_precision), ("fp64", "fp64")):
+        solve_mixed(system, amg, tol=1e-8, amg_precision=prec, operator=alt_op)
+        torch.cuda.synchronize()
+        n_rep = max(1, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(n_rep):
+            um, itm, _, _, _ = solve_mixed(system, amg, tol=1e-8, amg_precision=prec, operator=alt_op)
+        torch.cuda.synchronize()
+        tm = (time.perf_counter() - t0) / n_rep
+        relm = true_residual(system, um)
+        alt[name] = {"seconds": round(tm, 5), "iterations": itm, "rel_res": relm}
+        log(f"[bench] {name} with the {alt_name} CG operator: {tm * 1e3:.1f} ms iters {itm} rel {relm:.2e}")
+    alt["speedup_mixed_vs_fp64"] = round(alt["fp64"]["seconds"] / alt["mixed"]["seconds"], 3)
     # the same two solves with every kernel launched from the host
     set_graphs(args.no_graphs)
     eager = {}
@@ -582,13 +607,15 @@ def main() -> int:
         "data": "synthetic (structured C3D10 mesh generated in-process, deterministic loads)",
         "config": {"workload": workload, "n_dofs": meta["n_dofs"], "n_elements": meta["n_elements"],
                    "nnzb": nnzb, "tol": 1e-8, "solver": "FP64 CG + low-precision SA-AMG V-cycle",
-                   "parallelism": "single-gpu"},
+                   "cg_operator": args.operator, "parallelism": "single-gpu"},
         "dof_per_s": round(meta["n_dofs"] / t_mixed, 1),
         "rel_res_fp64_verified": rel_true,
         "iterations": iters,
         "cg_restarts": steps - 1,
         "vcycle_storage": args.amg_precision,
         "fp64_amg_pcg": {"seconds": round(t_fp64, 5), "iterations": it64, "rel_res": rel64},
+        "cg_operator": args.operator,
+        "other_cg_operator": alt,
         "speedup_mixed_vs_fp64_amg_pcg": round(t_fp64 / t_mixed, 3),
         "fp64_jacobi_pcg": jac,
         "restart_ir_fp32_krylov": ir,

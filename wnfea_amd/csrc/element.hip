@@ -403,11 +403,13 @@ __global__ __launch_bounds__(128) void c3d10_qdata_kernel(int M, const int* __re
   }
 }
 
-// fe[e] = K_e (m .* u)_e from the quadrature data, where m zeroes the DOFs
-// marked in node_fix (bit c: DOF c of the node fixed).  One thread per
-// element; the B matrix rows are formed per node on the fly, twice per Gauss
-// point (strain, then B^T sigma), so only u_e, the 30 outputs and one Gauss
-// point's data are live.
+// fe = K_e (m .* u)_e from the quadrature data, where m zeroes the DOFs
+// marked in node_fix (bit c: DOF c of the node fixed), stored [i][e][3] for
+// local node i.  One thread per
+// element, in two sweeps over its nodes so that only one node's data is live
+// at a time: (1) strains at the 4 Gauss points, eps_g = sum_i B_gi u_i, from
+// the preloaded J_g^-1; then T_g = (det J_g / 24) S_g J_g^-1 with S_g the
+// stress tensor D eps_g; (2) f_i = sum_g T_g dN_i/dxi(g), written per node.
 __global__ __launch_bounds__(128) void c3d10_pa_elem_kernel(int M, const int* __restrict__ conn,
                                                             const double* __restrict__ qd,
                                                             const double* __restrict__ E,
@@ -418,59 +420,73 @@ __global__ __launch_bounds__(128) void c3d10_pa_elem_kernel(int M, const int* __
   if (guard && flags[0]) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= M) return;
-  double u[10][3], fo[10][3];
+  double T[4][9];  // J_g^-1 first, then T_g
+  double eps[4][6];
 #pragma unroll
+  for (int g = 0; g < 4; ++g) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) T[g][k] = qd[((size_t)g * 10 + k) * M + e];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) eps[g][k] = 0.0;
+  }
+#pragma unroll 2
   for (int i = 0; i < 10; ++i) {
     const int n = conn[(size_t)e * 10 + i];
     const int m = fix ? fix[n] : 0;
+    const double ux = m & 1 ? 0.0 : U[(size_t)n * 3 + 0];
+    const double uy = m & 2 ? 0.0 : U[(size_t)n * 3 + 1];
+    const double uz = m & 4 ? 0.0 : U[(size_t)n * 3 + 2];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      u[i][c] = (m >> c) & 1 ? 0.0 : U[(size_t)n * 3 + c];
-      fo[i][c] = 0.0;
+    for (int g = 0; g < 4; ++g) {
+      const double d0 = c_dndxi[g][0][i], d1 = c_dndxi[g][1][i], d2 = c_dndxi[g][2][i];
+      const double gx = T[g][0] * d0 + T[g][1] * d1 + T[g][2] * d2;
+      const double gy = T[g][3] * d0 + T[g][4] * d1 + T[g][5] * d2;
+      const double gz = T[g][6] * d0 + T[g][7] * d1 + T[g][8] * d2;
+      eps[g][0] += gx * ux;
+      eps[g][1] += gy * uy;
+      eps[g][2] += gz * uz;
+      eps[g][3] += gy * ux + gx * uy;
+      eps[g][4] += gz * uy + gy * uz;
+      eps[g][5] += gz * ux + gx * uz;
     }
   }
   const double Ev = E[e], nv = nu[e];
   const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
   const double c11 = f * (1.0 - nv), c12 = f * nv, c44 = Ev / (2.0 * (1.0 + nv));
+#pragma unroll
   for (int g = 0; g < 4; ++g) {
-    double inv[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) inv[k] = qd[((size_t)g * 10 + k) * M + e];
     const double wd = qd[((size_t)g * 10 + 9) * M + e];
-    double eps[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double s0 = wd * (c11 * eps[g][0] + c12 * (eps[g][1] + eps[g][2]));
+    const double s1 = wd * (c11 * eps[g][1] + c12 * (eps[g][0] + eps[g][2]));
+    const double s2 = wd * (c11 * eps[g][2] + c12 * (eps[g][0] + eps[g][1]));
+    const double s3 = wd * c44 * eps[g][3], s4 = wd * c44 * eps[g][4], s5 = wd * c44 * eps[g][5];
+    // S = [[s0 s3 s5] [s3 s1 s4] [s5 s4 s2]];  T_g = S J^-1 (J^-1 rows in T[g])
+    double t[9];
 #pragma unroll
-    for (int i = 0; i < 10; ++i) {
-      const double d0 = c_dndxi[g][0][i], d1 = c_dndxi[g][1][i], d2 = c_dndxi[g][2][i];
-      const double gx = inv[0] * d0 + inv[1] * d1 + inv[2] * d2;
-      const double gy = inv[3] * d0 + inv[4] * d1 + inv[5] * d2;
-      const double gz = inv[6] * d0 + inv[7] * d1 + inv[8] * d2;
-      eps[0] += gx * u[i][0];
-      eps[1] += gy * u[i][1];
-      eps[2] += gz * u[i][2];
-      eps[3] += gy * u[i][0] + gx * u[i][1];
-      eps[4] += gz * u[i][1] + gy * u[i][2];
-      eps[5] += gz * u[i][0] + gx * u[i][2];
+    for (int c = 0; c < 3; ++c) {
+      t[0 + c] = s0 * T[g][c] + s3 * T[g][3 + c] + s5 * T[g][6 + c];
+      t[3 + c] = s3 * T[g][c] + s1 * T[g][3 + c] + s4 * T[g][6 + c];
+      t[6 + c] = s5 * T[g][c] + s4 * T[g][3 + c] + s2 * T[g][6 + c];
     }
-    const double s0 = wd * (c11 * eps[0] + c12 * (eps[1] + eps[2]));
-    const double s1 = wd * (c11 * eps[1] + c12 * (eps[0] + eps[2]));
-    const double s2 = wd * (c11 * eps[2] + c12 * (eps[0] + eps[1]));
-    const double s3 = wd * c44 * eps[3], s4 = wd * c44 * eps[4], s5 = wd * c44 * eps[5];
 #pragma unroll
-    for (int i = 0; i < 10; ++i) {
-      const double d0 = c_dndxi[g][0][i], d1 = c_dndxi[g][1][i], d2 = c_dndxi[g][2][i];
-      const double gx = inv[0] * d0 + inv[1] * d1 + inv[2] * d2;
-      const double gy = inv[3] * d0 + inv[4] * d1 + inv[5] * d2;
-      const double gz = inv[6] * d0 + inv[7] * d1 + inv[8] * d2;
-      fo[i][0] += gx * s0 + gy * s3 + gz * s5;
-      fo[i][1] += gy * s1 + gx * s3 + gz * s4;
-      fo[i][2] += gz * s2 + gy * s4 + gx * s5;
-    }
+    for (int k = 0; k < 9; ++k) T[g][k] = t[k];
   }
-  double* o = fe + (size_t)e * 30;
+#pragma unroll 2
+  for (int i = 0; i < 10; ++i) {
+    double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
-  for (int i = 0; i < 10; ++i)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) o[3 * i + c] = fo[i][c];
+    for (int g = 0; g < 4; ++g) {
+      const double d0 = c_dndxi[g][0][i], d1 = c_dndxi[g][1][i], d2 = c_dndxi[g][2][i];
+      fx += T[g][0] * d0 + T[g][1] * d1 + T[g][2] * d2;
+      fy += T[g][3] * d0 + T[g][4] * d1 + T[g][5] * d2;
+      fz += T[g][6] * d0 + T[g][7] * d1 + T[g][8] * d2;
+    }
+    // layout [local node][element][3]: the block's stores are contiguous
+    double* o = fe + ((size_t)i * M + e) * 3;
+    o[0] = fx;
+    o[1] = fy;
+    o[2] = fz;
+  }
 }
 
 // Node-wise deterministic gather with the assembled operator's Dirichlet rule
@@ -478,42 +494,92 @@ __global__ __launch_bounds__(128) void c3d10_pa_elem_kernel(int M, const int* __
 //   mode 0: y_d = (K u)_d;   mode 1: y_d = (K u)_d and partial u.y;
 //   mode 2: y_d = f_d - (K u)_d and partial y.y,
 // where (K u)_d = u_d for a fixed DOF, else the sum of the node's element
-// contributions in element order.
-__global__ __launch_bounds__(256) void mf_gather_kernel(int n_nodes, const int* __restrict__ n2e_ptr,
-                                                        const int* __restrict__ n2e, const double* __restrict__ fe,
-                                                        const unsigned char* __restrict__ fix,
-                                                        const double* __restrict__ u, const double* __restrict__ f,
-                                                        double* __restrict__ y, double* partials, int mode,
-                                                        const int* flags, int guard) {
+// contributions in element order.  Each wavefront takes one chunk of whole
+// nodes, at most 64 of them with at most 64 incidences in all (chunk
+// boundaries fixed at set-up): lane l loads incidence l of the chunk (one
+// round of independent loads, issued together with the node's own operands),
+// the values meet in LDS, and one lane per node adds its incidences in
+// order.  A node with more than 64 incidences is a chunk of its own, summed
+// by its first lane from memory.
+constexpr int kGatherWaves = 4;
+
+__global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_kernel(
+    int n_chunks, int M, const int2* __restrict__ chunk, const int* __restrict__ n2e_ptr,
+    const int* __restrict__ n2e, const double* __restrict__ fe, const unsigned char* __restrict__ fix,
+    const double* __restrict__ u, const double* __restrict__ f, double* __restrict__ y, double* partials, int mode,
+    const int* flags, int guard) {
   if (guard && flags[0]) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  double dot = 0.0;
-  if (n < n_nodes) {
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int k = n2e_ptr[n]; k < n2e_ptr[n + 1]; ++k) {
-      const int code = n2e[k];  // e * 10 + local
-      const double* p = fe + (size_t)(code / 10) * 30 + 3 * (code % 10);
-      s[0] += p[0];
-      s[1] += p[1];
-      s[2] += p[2];
-    }
-    const int m = fix ? fix[n] : 0;
+  __shared__ double sv[kGatherWaves][64][3];
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int c = blockIdx.x * kGatherWaves + w;
+  const bool live = c < n_chunks;
+  // chunk c: nodes [a.x, b.x), incidences [a.y, b.y)
+  const int2 a = live ? chunk[c] : make_int2(0, 0);
+  const int2 b = live ? chunk[c + 1] : make_int2(0, 0);
+  const int k0 = a.y, k1 = b.y;
+  const bool big = k1 - k0 > 64;  // then the chunk is one node
+  // this lane's node and its operands, loaded before the incidences
+  const int n = a.x + l;
+  const bool has = n < b.x;
+  int j0 = 0, j1 = 0, m = 0;
+  double uu[3] = {0.0, 0.0, 0.0}, ff[3] = {0.0, 0.0, 0.0};
+  if (has) {
+    j0 = n2e_ptr[n] - k0;
+    j1 = n2e_ptr[n + 1] - k0;
+    m = fix ? fix[n] : 0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const size_t d = (size_t)n * 3 + c;
-      const double ku = (m >> c) & 1 ? u[d] : s[c];
+    for (int cc = 0; cc < 3; ++cc) {
+      if (mode == 1 || ((m >> cc) & 1)) uu[cc] = u[(size_t)n * 3 + cc];
+      if (mode == 2) ff[cc] = f[(size_t)n * 3 + cc];
+    }
+  }
+  const int k = k0 + l;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  if (!big && k < k1) {
+    const int code = n2e[k];  // e * 10 + local
+    const double* p = fe + ((size_t)(code % 10) * M + code / 10) * 3;
+    v0 = p[0];
+    v1 = p[1];
+    v2 = p[2];
+  }
+  sv[w][l][0] = v0;
+  sv[w][l][1] = v1;
+  sv[w][l][2] = v2;
+  __syncthreads();
+  double dot = 0.0;
+  if (has) {
+    double s[3] = {0.0, 0.0, 0.0};
+    if (big) {
+      for (int j = k0; j < k1; ++j) {
+        const int code = n2e[j];
+        const double* p = fe + ((size_t)(code % 10) * M + code / 10) * 3;
+        s[0] += p[0];
+        s[1] += p[1];
+        s[2] += p[2];
+      }
+    } else {
+      for (int j = j0; j < j1; ++j) {
+        s[0] += sv[w][j][0];
+        s[1] += sv[w][j][1];
+        s[2] += sv[w][j][2];
+      }
+    }
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const size_t d = (size_t)n * 3 + cc;
+      const double ku = (m >> cc) & 1 ? uu[cc] : s[cc];
       if (mode == 2) {
-        const double r = f[d] - ku;
+        const double r = ff[cc] - ku;
         y[d] = r;
         dot += r * r;
       } else {
         y[d] = ku;
-        if (mode == 1) dot += u[d] * ku;
+        if (mode == 1) dot += uu[cc] * ku;
       }
     }
   }
   if (mode != 0) {
-    __shared__ double sm[4];
+    __shared__ double sm[kGatherWaves];
     double v[1] = {block_sum(dot, sm)};
     store_block_partials<1>(v, partials);
   }
@@ -548,8 +614,36 @@ int launch_gather_node(int n_nodes, const int* n2e_ptr, const int* n2e, const do
   return static_cast<int>(hipGetLastError());
 }
 
+// Gather chunks: consecutive nodes, at most 64 per chunk and at most 64
+// incidences in all, or one node alone when it has more.
+std::vector<int> mf_chunks(const std::vector<int>& ptr) {  // first node of each chunk, then n
+  const int n = (int)ptr.size() - 1;
+  std::vector<int> c{0};
+  int start = 0;
+  while (start < n) {
+    int end = start + 1;
+    if (ptr[end] - ptr[start] <= 64)
+      while (end < n && end - start < 64 && ptr[end + 1] - ptr[start] <= 64) ++end;
+    c.push_back(end);
+    start = end;
+  }
+  return c;
+}
+
 int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s) {
   mf_release(op);
+  if (op.n_nodes <= 0) return kBadArg;
+  std::vector<int> ptr((size_t)op.n_nodes + 1);
+  WN_HIP(hipMemcpy(ptr.data(), op.n2e_ptr, sizeof(int) * ptr.size(), hipMemcpyDeviceToHost));
+  if (ptr[0] != 0 || ptr[op.n_nodes] != 10LL * op.M) return kBadArg;
+  for (int i = 0; i < op.n_nodes; ++i)
+    if (ptr[i + 1] < ptr[i]) return kBadArg;
+  const std::vector<int> c = mf_chunks(ptr);
+  op.n_chunks = (int)c.size() - 1;
+  std::vector<int2> c2(c.size());
+  for (size_t i = 0; i < c.size(); ++i) c2[i] = make_int2(c[i], ptr[(size_t)c[i]]);
+  WN_HIP(hipMalloc(&op.chunk, sizeof(int2) * c2.size()));
+  WN_HIP(hipMemcpy(op.chunk, c2.data(), sizeof(int2) * c2.size(), hipMemcpyHostToDevice));
   if (op.M <= 0) return kOk;
   int r = upload_element_consts(s);
   if (r) return r;
@@ -560,22 +654,25 @@ int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s) {
 }
 
 void mf_release(MfOp& op) {
-  if (op.qd) (void)hipFree(op.qd);
-  if (op.fe) (void)hipFree(op.fe);
+  for (void* p : {(void*)op.qd, (void*)op.fe, (void*)op.chunk})
+    if (p) (void)hipFree(p);
   op.qd = op.fe = nullptr;
+  op.chunk = nullptr;
+  op.n_chunks = 0;
 }
 
-int mf_grid(const MfOp& op) { return std::max(1, ceil_div(op.n_nodes, 256)); }
+int mf_grid(const MfOp& op) { return std::max(1, ceil_div(op.n_chunks, kGatherWaves)); }
 
 int mf_apply(const MfOp& op, const double* u, const double* f, double* y, double* partials, int mode,
              const int* flags, int guard, hipStream_t s) {
+  if (!op.chunk) return kNotReady;
   if (op.M > 0) {
     if (!op.qd || !op.fe) return kNotReady;
     hipLaunchKernelGGL(c3d10_pa_elem_kernel, dim3(ceil_div(op.M, 128)), dim3(128), 0, s, op.M, op.conn, op.qd, op.E,
                        op.nu, op.fix, u, op.fe, flags, guard);
   }
-  hipLaunchKernelGGL(mf_gather_kernel, dim3(mf_grid(op)), dim3(256), 0, s, op.n_nodes, op.n2e_ptr, op.n2e, op.fe,
-                     op.fix, u, f, y, partials, mode, flags, guard);
+  hipLaunchKernelGGL(mf_gather_kernel, dim3(mf_grid(op)), dim3(64 * kGatherWaves), 0, s, op.n_chunks, op.M, op.chunk,
+                     op.n2e_ptr, op.n2e, op.fe, op.fix, u, f, y, partials, mode, flags, guard);
   return static_cast<int>(hipGetLastError());
 }
 

@@ -89,8 +89,11 @@ struct MfOp {
   const int* n2e_ptr = nullptr;
   const int* n2e = nullptr;
   double* qd = nullptr;  // [4][10][M] inverse Jacobians and det/24
-  double* fe = nullptr;  // [M][30]
+  double* fe = nullptr;  // [10][M][3] element contributions per local node
+  int2* chunk = nullptr;  // [n_chunks+1] {first node, first incidence} of each gather chunk
+  int n_chunks = 0;
 };
+std::vector<int> mf_chunks(const std::vector<int>& n2e_ptr);
 int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s);
 void mf_release(MfOp& op);
 int mf_grid(const MfOp& op);  // blocks of the gather (= partial count of modes 1 and 2)
