@@ -424,7 +424,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
-    ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine", "fp16_coarse"], default="fp32")
+    ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine", "fp16_coarse", "fp16_smooth"],
+                    default="fp32")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
     ap.add_argument("--operator", choices=["matrix_free", "assembled"], default="matrix_free",
                     help="CG's operator K: matrix-free (quadrature data) or the assembled FP64 BSR matrix")
@@ -528,7 +529,7 @@ def main() -> int:
     log(f"[bench] fp64 AMG-PCG ({args.operator} K): {t_fp64 * 1e3:.1f} ms/solve iters {it64} rel {rel64:.2e}")
     err_u = float(torch.linalg.norm(u - x64) / torch.linalg.norm(x64))
     others = {}
-    for other in ("fp32", "fp16", "fp16_fine", "fp16_coarse"):
+    for other in ("fp32", "fp16", "fp16_fine", "fp16_coarse", "fp16_smooth"):
         if other == args.amg_precision:
             continue
         solve_mixed(system, amg, tol=1e-8, amg_precision=other, operator=main_op)

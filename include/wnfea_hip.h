@@ -279,8 +279,10 @@ int wnfea_amg_tune_precision(void *h, int prec, int waves, int xcd_remap, int un
 
 /* z = M r, one V-cycle (device vectors).  vt: 0 FP64 values and vectors;
  * 1 FP32 values, 2 FP16 values on every level, 3 FP16 values on the finest
- * level and FP32 below, 4 FP32 on the finest level and FP16 below (vt >= 1:
- * FP32 vectors and arithmetic). */
+ * level and FP32 below, 4 FP32 on the finest level and FP16 below, 5 FP16
+ * values in the smoothing steps and FP32 in the defects and transfers (not
+ * symmetric; wnfea_pcg then uses the flexible beta) (vt >= 1: FP32 vectors
+ * and arithmetic). */
 int wnfea_amg_vcycle(void *h, int vt, const void *r, void *z, void *stream);
 
 /* ---------------------------------------------------------------- solvers */

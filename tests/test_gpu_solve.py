@@ -43,9 +43,10 @@ def test_amg_pcg_fp64(c0_system, c0_amg, c0_oracle, precision):
     assert _rel(x.cpu().numpy(), u_ref) <= 1e-8
 
 
-@pytest.mark.parametrize("amg_precision", ["fp32", "fp16", "fp16_fine", "fp16_coarse"])
+@pytest.mark.parametrize("amg_precision", ["fp32", "fp16", "fp16_fine", "fp16_coarse", "fp16_smooth"])
 def test_mixed_precision_preconditioner(c0_system, c0_amg, c0_oracle, amg_precision):
-    """FP64 CG with an FP32/FP16 V-cycle: FP64-verified 1e-8 and the 1e-6 mixed-mode displacement bar."""
+    """FP64 CG with an FP32/FP16 V-cycle: FP64-verified 1e-8 and the 1e-6 mixed-mode displacement bar
+    (fp16_smooth: FP16 smoothing steps, FP32 defects, the flexible beta)."""
     from wnfea_amd.solver.solid_solver import pcg_device, solve_mixed, true_residual
     _, _, u_ref = c0_oracle
     u, it, steps, rel, _ = solve_mixed(c0_system, c0_amg, tol=1e-8, amg_precision=amg_precision)
@@ -359,3 +360,33 @@ def test_matrix_free_solve_matches_oracle(c0_system, c0_amg, c0_oracle, amg_prec
     u2, it2, _, _, _ = solve_mixed(c0_system, c0_amg, tol=1e-8, amg_precision=amg_precision)
     assert abs(it - it2) <= 2
     op.close()
+
+
+def test_fp16_smoothing_cycle_is_the_fp32_cycle_with_rounded_smoother(c0_amg):
+    """The nonsymmetric cycle (FP16 smoothing steps, FP32 defects and
+    transfers) equals the numpy cycle whose smoother operators are rounded to
+    FP16 while defects use the FP32-rounded operators; it differs from the
+    symmetric FP32 cycle by O(FP16 rounding)."""
+    import copy
+    import scipy.sparse as sp
+    import torch
+    from vcycle_mirror import vcycle
+    levels = copy.deepcopy(c0_amg.levels)
+
+    def rnd(M, dt):
+        M = sp.csr_matrix(M).copy()
+        M.data = M.data.astype(dt).astype(np.float64)
+        return M
+
+    for lv in levels[:-1]:
+        lv.A_smooth = rnd(lv.A, np.float16)
+        lv.A = rnd(lv.A, np.float32)
+        lv.P, lv.R = rnd(lv.P, np.float32), rnd(lv.R, np.float32)
+    n = levels[0].A.shape[0]
+    r = np.random.default_rng(9).standard_normal(n)
+    z_ref = vcycle(levels, c0_amg.coarse_inv, r, deg=c0_amg.level_degrees, ratio=c0_amg.ratio)
+    rt = torch.as_tensor(r, device="cuda").float()
+    z = c0_amg.vcycle(rt, storage="fp16_smooth").double().cpu().numpy()
+    assert _rel(z, z_ref) <= 2e-5
+    z32 = c0_amg.vcycle(rt, storage="fp32").double().cpu().numpy()
+    assert 1e-6 <= _rel(z, z32) <= 2e-2

@@ -20,23 +20,26 @@ def cheb_coefs(lmax, ratio, deg):
 
 
 def vcycle(levels, inv, b, l=0, deg=2, ratio=30.0):
-    """deg: Chebyshev degree, one for all levels or a list per level."""
+    """deg: Chebyshev degree, one for all levels or a list per level.  A level
+    may carry ``A_smooth``, the operator of its smoothing steps (the defect
+    always uses ``A``)."""
     lv = levels[l]
     if l == len(levels) - 1:
         return inv @ b
     A, dinv = lv.A, lv.dinv
+    As = getattr(lv, "A_smooth", A)
     dl = deg[l] if isinstance(deg, (list, tuple)) else deg
     c = cheb_coefs(lv.lmax, ratio, dl)
     d = c[0][1] * dinv * b
     x = d.copy()
     for k in range(1, dl):
-        d = c[k][0] * d + c[k][1] * dinv * (b - A @ x)
+        d = c[k][0] * d + c[k][1] * dinv * (b - As @ x)
         x = x + d
     res = b - A @ x
     e = vcycle(levels, inv, lv.R @ res, l + 1, deg, ratio)
     x = x + lv.P @ e
     for k in range(dl):
-        d = c[k][0] * d + c[k][1] * dinv * (b - A @ x)
+        d = c[k][0] * d + c[k][1] * dinv * (b - As @ x)
         x = x + d
     return x
 

@@ -54,18 +54,29 @@ __global__ __launch_bounds__(kVecBlock) void jacobi_dot_kernel(long long n, cons
 
 // partials of rho = r . (S z) (finalised with kFinRho); z may be in a lower
 // precision than r (preconditioner output) and S (per-DOF scale) may be null.
+// With q (flexible PCG) also the partials of (S z) . q, q = A p of the
+// previous iteration (finalised with kFinRhoFlex).
 template <typename T, typename ZT>
 __global__ __launch_bounds__(kVecBlock) void rho_kernel(long long n, const T* __restrict__ r, const ZT* __restrict__ z,
-                                                        const double* __restrict__ scale, double* partials,
-                                                        const int* skip) {
+                                                        const double* __restrict__ scale, const T* __restrict__ q,
+                                                        double* partials, const int* skip) {
   if (skip && *skip) return;
-  double acc = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    acc += double(r[i]) * double(z[i]) * (scale ? scale[i] : 1.0);
+  double acc = 0.0, acq = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double zs = double(z[i]) * (scale ? scale[i] : 1.0);
+    acc += double(r[i]) * zs;
+    if (q) acq += zs * double(q[i]);
+  }
   __shared__ double sm[kVecBlock / 64];
-  const double bs = block_sum(acc, sm);
-  double v[1] = {bs};
-  store_block_partials<1>(v, partials);
+  double v[2];
+  v[0] = block_sum(acc, sm);
+  if (q) {
+    v[1] = block_sum(acq, sm);
+    store_block_partials<2>(v, partials);
+  } else {
+    double v1[1] = {v[0]};
+    store_block_partials<1>(v1, partials);
+  }
 }
 
 // p = S z + beta p  (p = S z on the first iteration).  first < 0: the first
@@ -175,6 +186,15 @@ __global__ __launch_bounds__(1024) void dot_finalize_kernel(const double* __rest
       sc[kScRhoPrev] = v[0];
       break;
     }
+    case kFinRhoFlex: {
+      // beta = z_new . (r_new - r_old) / rho_old = -alpha_old (z_new . q_old) / rho_old
+      sc[kScRho] = v[0];
+      const double prev = sc[kScRhoPrev];
+      const bool f = first >= 0 ? first != 0 : skip[1] == 0;
+      sc[kScBeta] = f ? 0.0 : (prev != 0.0 ? -sc[kScAlpha] * v[1] / prev : 0.0);
+      sc[kScRhoPrev] = v[0];
+      break;
+    }
     case kFinGamma:
       sc[kScGamma] = v[0];
       sc[kScAlpha] = v[0] != 0.0 ? sc[kScRho] / v[0] : 0.0;
@@ -233,11 +253,13 @@ int launch_jacobi_dot(long long n, const T* r, const T* dinv, T* z, Workspace& w
 }
 template <typename T, typename ZT>
 int launch_rho(long long n, const T* r, const ZT* z, const double* scale, Workspace& w, int first, const int* skip,
-               hipStream_t s) {
+               hipStream_t s, const T* q) {
   const int g = vec_grid(n);
-  hipLaunchKernelGGL((rho_kernel<T, ZT>), dim3(g), dim3(kVecBlock), 0, s, n, r, z, scale, w.partials, skip);
+  hipLaunchKernelGGL((rho_kernel<T, ZT>), dim3(g), dim3(kVecBlock), 0, s, n, r, z, scale, q, w.partials, skip);
   int e = static_cast<int>(hipGetLastError());
-  return e ? e : launch_finalize(w.partials, g, 1, kFinRho, nullptr, w.sc, nullptr, first, skip, s);
+  return e ? e
+           : launch_finalize(w.partials, g, q ? 2 : 1, q ? kFinRhoFlex : kFinRho, nullptr, w.sc, nullptr, first, skip,
+                             s);
 }
 template <typename T, typename ZT>
 int launch_update_p(long long n, const ZT* z, const double* scale, T* p, Workspace& w, int first, const int* skip,
@@ -282,7 +304,7 @@ WN_INST(double)
 
 #define WN_INST2(T, U)                                                                                           \
   template int launch_rho<T, U>(long long, const T*, const U*, const double*, Workspace&, int, const int*,      \
-                                hipStream_t);                                                                  \
+                                hipStream_t, const T*);                                                        \
   template int launch_update_p<T, U>(long long, const U*, const double*, T*, Workspace&, int, const int*,      \
                                      hipStream_t);                                                             \
   template int launch_update_xr<T, U>(long long, T*, T*, const T*, const T*, U*, const double*, Workspace&,    \

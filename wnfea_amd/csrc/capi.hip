@@ -391,12 +391,13 @@ int wnfea_solver_set_graphs(void* h, int enable) {
 // prec: 0 FP64 primary values, 1 FP32 values (FP32 vectors, FP64 dots),
 // 2 FP64 equilibrated values.  pkind: 0 none, 1 Jacobi (dinv, same precision
 // as the vectors), 2 AMG (amg handle; amg_vt 0 FP64 / 1 FP32 / 2 FP16 /
-// 3 FP16-finest-level cycle;
+// 3 FP16-finest-level cycle / 4 FP16-below-finest cycle / 5 FP16 smoothing
+// steps, the nonsymmetric cycle run with the flexible beta;
 // scale, when given, makes the preconditioner z = S M(S r)).
 int wnfea_pcg(void* solver, void* bsr, int prec, int pkind, const void* dinv, void* amg, int amg_vt,
               const double* scale, const void* b, void* x, double rtol, int max_iter, int check_every, int* iters,
               double* rel, void* stream) {
-  if (!solver || !bsr || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2 || amg_vt < 0 || amg_vt > 4)
+  if (!solver || !bsr || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2 || amg_vt < 0 || amg_vt > 5)
     return kBadArg;
   if ((pkind == 1 && !dinv) || (pkind == 2 && !amg)) return kBadArg;
   auto* B = static_cast<WnBsr*>(bsr);
@@ -513,7 +514,7 @@ int wnfea_mf_residual(void* solver, void* h, const double* u, const double* f, d
 int wnfea_pcg_mf(void* solver, void* h, int pkind, const double* dinv, void* amg, int amg_vt, const double* scale,
                  const double* b, double* x, double rtol, int max_iter, int check_every, int* iters, double* rel,
                  void* stream) {
-  if (!solver || !h || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2 || amg_vt < 0 || amg_vt > 4)
+  if (!solver || !h || !b || !x || max_iter < 0 || pkind < 0 || pkind > 2 || amg_vt < 0 || amg_vt > 5)
     return kBadArg;
   if ((pkind == 1 && !dinv) || (pkind == 2 && !amg)) return kBadArg;
   static const SellMatrix none;

@@ -93,6 +93,7 @@ enum FinMode : int {
   kFinRho = 1,    // rho: sc[Rho] = sum; sc[Beta] = first ? 0 : rho / rho_prev; sc[RhoPrev] = rho
   kFinGamma = 2,  // gamma: sc[Gamma] = sum; sc[Alpha] = rho / gamma
   kFinRR = 3,     // ||r||^2: sc[RR] = sum; count the iteration; done flag when sc[RR] <= sc[Tol2]
+  kFinRhoFlex = 4,  // rho and z.q_old: flexible (Polak-Ribiere) beta = -alpha (z.q_old) / rho_prev
 };
 
 }  // namespace wn

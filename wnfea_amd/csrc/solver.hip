@@ -215,8 +215,11 @@ static int coarse_apply(const Amg& g, const T* b, T* x, const int* skip, hipStre
 }
 
 // One V-cycle from level l.  VT: values of this level; VTC: values of every
-// coarser level; T: vector and arithmetic type.
-template <typename VT, typename VTC, typename T>
+// coarser level; T: vector and arithmetic type; VS / VSC: values of the
+// smoothing steps on this level / the coarser ones (the defect, restriction
+// and prolongation use VT / VTC).  With VS != VT the cycle is not symmetric:
+// it serves flexible PCG.
+template <typename VT, typename VTC, typename T, typename VS = VT, typename VSC = VTC>
 static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t s) {
   AmgLevel& L = *g.L[l];
   SpmvTune t = g.tune;
@@ -247,7 +250,7 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   hipLaunchKernelGGL(cheb_first_kernel<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, b, dinv, d, cur, T(c.c2[0]), skip);
   if ((r = static_cast<int>(hipGetLastError()))) return r;
   for (int k = 1; k < deg; ++k) {
-    if ((r = spmv_cheb<VT, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
+    if ((r = spmv_cheb<VS, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
     std::swap(cur, oth);
   }
   // defect and restriction
@@ -255,11 +258,11 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   AmgLevel& N = *g.L[l + 1];
   T** nv = level_vecs<T>(N);
   if ((r = spmv_restrict<VT, T>(L.R, res, nv[0], t, s))) return r;
-  if ((r = vcycle<VTC, VTC, T>(g, l + 1, nv[0], nv[1], skip, s))) return r;
+  if ((r = vcycle<VTC, VTC, T, VSC, VSC>(g, l + 1, nv[0], nv[1], skip, s))) return r;
   if ((r = spmv_addto<VT, T>(L.P, nv[1], cur, t, s))) return r;  // cur == post here
   oth = (cur == x) ? tmp : x;
   for (int k = 0; k < deg; ++k) {
-    if ((r = spmv_cheb<VT, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
+    if ((r = spmv_cheb<VS, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
     std::swap(cur, oth);
   }
   return kOk;
@@ -291,6 +294,9 @@ int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream
     case 4:
       if ((g.store & 6) != 6) return kNotReady;
       return vcycle<float, _Float16, float>(g, 0, r32, z32, skip, s);
+    case 5:  // FP16 smoothing steps, FP32 defects and transfers: nonsymmetric
+      if ((g.store & 6) != 6) return kNotReady;
+      return vcycle<float, float, float, _Float16, _Float16>(g, 0, r32, z32, skip, s);
 
     default:
       return kBadMode;
@@ -402,11 +408,15 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
   const bool amg32 = (pkind == 2) && amg_vt != 0;
   if (pkind == 2 && !amg32 && !std::is_same<T, double>::value) return kBadMode;
   const bool sep32 = amg32 && (!std::is_same<T, float>::value || scale != nullptr);
+  // a nonsymmetric cycle needs the flexible (Polak-Ribiere) beta, which reads
+  // q = A p of the previous iteration in the rho kernel
+  const bool flexible = pkind == 2 && amg_vt == 5;
   const bool sep64 = (pkind == 2) && !amg32 && scale != nullptr;
   if (sep32 && (r = S.ensure_p(n))) return r;
   float* rp32 = sep32 ? S.rp32 : nullptr;
   // x = 0, r = b, rr = b.b
   if ((r = launch_fill<T>(n, x, T(0), s))) return r;
+  if (flexible && (r = launch_fill<T>(n, k.q, T(0), s))) return r;
   if ((r = launch_cast_scale<T, T>(n, b, nullptr, 1.0, k.r, s))) return r;
   if ((r = launch_dot<T>(n, k.r, k.r, S.ws, S.ws.sc + kScRR, s))) return r;
   hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(64), 0, s, S.ws.sc, S.flags, rtol * rtol);
@@ -424,7 +434,8 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
       const float* vin = sep32 ? rp32 : reinterpret_cast<const float*>(k.r);
       float* vout = sep32 ? S.zp32 : reinterpret_cast<float*>(k.z);
       if ((e = amg_apply(*amg, amg_vt, vin, vout, S.flags, st))) return e;
-      if ((e = launch_rho<T, float>(n, k.r, vout, scale, S.ws, first, S.flags, st))) return e;
+      if ((e = launch_rho<T, float>(n, k.r, vout, scale, S.ws, first, S.flags, st, flexible ? k.q : nullptr)))
+        return e;
       if ((e = launch_update_p<T, float>(n, vout, scale, k.p, S.ws, first, S.flags, st))) return e;
     } else if (pkind == 2) {
       // FP64 cycle: in-place scaled input in k.z when S is given

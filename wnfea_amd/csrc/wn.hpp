@@ -128,10 +128,11 @@ int launch_cast_scale(long long n, const A* x, const double* sc, double f, B* y,
 template <typename T>
 int launch_jacobi_dot(long long n, const T* r, const T* dinv, T* z, Workspace& w, int first, const int* skip,
                       hipStream_t s);
-// rho = r . (S z) (S may be null); finalise beta = rho / rho_prev
+// rho = r . (S z) (S may be null); finalise beta = rho / rho_prev, or with q
+// (= A p of the previous iteration) the flexible beta = -alpha (S z . q) / rho_prev
 template <typename T, typename ZT>
 int launch_rho(long long n, const T* r, const ZT* z, const double* scale, Workspace& w, int first, const int* skip,
-               hipStream_t s);
+               hipStream_t s, const T* q = nullptr);
 // p = S z + beta p (p = S z when first)
 template <typename T, typename ZT>
 int launch_update_p(long long n, const ZT* z, const double* scale, T* p, Workspace& w, int first, const int* skip,

@@ -207,7 +207,7 @@ class SmoothedAggregationAMG:
     STORAGE_BITS = {"fp64": 1, "fp32": 2, "fp16": 4}
     # "fp16_fine": FP16 values on the finest level, FP32 on the coarser ones;
     # "fp16_coarse": FP32 on the finest level, FP16 on the coarser ones
-    VT_CODE = {"fp64": 0, "fp32": 1, "fp16": 2, "fp16_fine": 3, "fp16_coarse": 4}
+    VT_CODE = {"fp64": 0, "fp32": 1, "fp16": 2, "fp16_fine": 3, "fp16_coarse": 4, "fp16_smooth": 5}
 
     def __init__(self, fine, A_hat: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None = None,
                  dof_scale: np.ndarray | None = None, theta: float = 0.08, theta_decay: float = 1.0,

@@ -305,7 +305,7 @@ def _as_system(K, F, want_fp64_equilibrated=False) -> SolidSystem:
     return system_from_csr(K, F, want_fp64_equilibrated=want_fp64_equilibrated)
 
 
-_VT = {"fp64": 0, "fp32": 1, "fp16": 2, "fp16_fine": 3, "fp16_coarse": 4}
+_VT = {"fp64": 0, "fp32": 1, "fp16": 2, "fp16_fine": 3, "fp16_coarse": 4, "fp16_smooth": 5}
 
 
 def pcg_device(system: SolidSystem, b=None, rtol: float = 1e-8, max_iter: int = 2000, preconditioner: str = "jacobi",
