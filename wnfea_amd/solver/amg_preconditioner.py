@@ -25,6 +25,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 import scipy.sparse as sp
+from scipy.sparse.linalg import LinearOperator
 
 from .. import _lib
 
@@ -316,3 +317,133 @@ class SmoothedAggregationAMG:
             self.close()
         except Exception:
             pass
+
+
+class _LevelView:
+    """Host view of one level in the working precision (A, inv_diag, P, R),
+    as the reference's ``AMGLevel`` (``amg_preconditioner.py:128-145``)."""
+
+    def __init__(self, lv, dt):
+        self.A = sp.csr_matrix(lv.A).astype(dt)
+        self.inv_diag = np.asarray(lv.dinv, dtype=dt)
+        self.P = sp.csr_matrix(lv.P).astype(dt) if lv.P is not None else None
+        self.R = sp.csr_matrix(lv.R).astype(dt) if lv.R is not None else None
+
+
+class BlockBeamAMGPreconditioner(LinearOperator):
+    """Drop-in for ``wnfea/solver/amg_preconditioner.py:147-661``: M^-1 as a
+    scipy LinearOperator (``matvec`` takes and returns FP64 numpy vectors),
+    ``solve_pcg``, the tri-precision switch (``enable_tri_precision``,
+    ``apply_adaptive``, ``last_precision_used``) and ``levels`` in the working
+    precision.  Underneath it is this package's device SA-AMG on the
+    symmetrically equilibrated operator (``z = S M(S r)``, as the reference's
+    ``inv_sqrt_d`` scaling); every V-cycle runs on the GPU whatever
+    ``device`` says (``"cpu"`` is accepted for API parity).  ``A`` (scipy, 6
+    or 3 DOF per node) defaults to ``assemble_global_system_sparse(model)``;
+    supported DOFs are recognised as unit-diagonal rows without couplings.
+    ``coarse_size`` and ``max_levels`` bound the hierarchy as in the
+    reference (the coarsest level is solved exactly; the tuned solver path,
+    ``solid_solver.build_amg``, uses a 12 000-row coarsest level instead)."""
+
+    _PREC = {0: "fp64", 1: "fp32", 2: "fp16"}
+
+    def __init__(self, model, A=None, max_levels: int = 6, coarse_size: int = 36,
+                 working_dtype=np.float64, device: str = "cpu"):
+        import torch
+        from .solid_solver import _sync, build_amg, system_from_csr
+        if A is None:
+            from .beam_solver import assemble_global_system_sparse
+            A, _ = assemble_global_system_sparse(model)
+        A = sp.csr_matrix(A, dtype=np.float64)
+        n = A.shape[0]
+        super().__init__(dtype=np.float64, shape=(n, n))
+        nn = len(model.mesh_nodes)
+        dpn = n // nn if nn and n % nn == 0 and n // nn in (3, 6) else 6
+        self.model = model
+        self.max_levels, self.coarse_size = int(max_levels), int(coarse_size)
+        self.working_dtype = np.dtype(working_dtype)
+        self.device = str(device).lower()
+        self.use_tri_precision = False
+        self.switch_tol = 1e-2
+        self.last_precision_used = None
+        self.fp16_vcycle_count = 0
+        self.fp32_vcycle_count = 0
+        # supported DOFs: a unit diagonal and nothing else in the row
+        nnz = np.diff(A.indptr)
+        fixed = (nnz == 1) & (A.diagonal() == 1.0)
+        self.system = system_from_csr(A, np.zeros(n), fixed_mask=fixed,
+                                      coords=np.asarray(model.mesh_nodes, dtype=np.float64), dpn=dpn)
+        S = self.system.scale.cpu().numpy()
+        self.inv_sqrt_d = S
+        self.amg = build_amg(self.system, storage=("fp64", "fp32", "fp16"), coarse_size=self.coarse_size,
+                             max_levels=self.max_levels)
+        self.gpu_handle = self.amg.handle
+        self._A = A
+        self._levels = None
+        self._S = torch.as_tensor(S, device=self.system.K.device)
+        _sync()
+
+    @property
+    def levels(self):
+        if self._levels is None:
+            self._levels = [_LevelView(lv, self.working_dtype) for lv in self.amg.levels]
+        return self._levels
+
+    def enable_tri_precision(self, switch_tol: float = 1e-2) -> None:
+        """FP16 V-cycles while the relative residual is above switch_tol
+        (``amg_preconditioner.py:442-445``)."""
+        self.use_tri_precision = True
+        self.switch_tol = float(switch_tol)
+
+    def _apply(self, r, storage: str) -> np.ndarray:
+        import torch
+        rt = torch.as_tensor(np.ascontiguousarray(r, dtype=np.float64), device=self._S.device) * self._S
+        if storage != "fp64":
+            rt = rt.float()
+        z = self.amg.vcycle(rt, storage=storage).double() * self._S
+        return z.cpu().numpy()
+
+    def _matvec(self, r):
+        storage = "fp32" if self.working_dtype == np.float32 else "fp64"
+        self.last_precision_used = self.working_dtype.name
+        if storage == "fp32":
+            self.fp32_vcycle_count += 1
+        return self._apply(np.ravel(r), storage)
+
+    def _rmatvec(self, r):
+        return self._matvec(r)  # the FP64/FP32 cycles are symmetric
+
+    def apply_adaptive(self, r, current_res_rel: float | None = None) -> np.ndarray:
+        """``amg_preconditioner.py:538-584``: an FP16-stored cycle above the
+        switch tolerance, the working precision below it."""
+        if self.use_tri_precision and current_res_rel is not None and current_res_rel > self.switch_tol:
+            self.last_precision_used = "float16"
+            self.fp16_vcycle_count += 1
+            return self._apply(np.ravel(r), "fp16")
+        return self._matvec(r)
+
+    def solve_pcg(self, b, rtol: float = 1e-6, max_iter: int = 500, precision_mode: int | None = None,
+                  check_interval: int = 1, coarse_sweeps: int | None = None):
+        """``amg_preconditioner.py:608-661``: PCG on the device with the
+        Krylov vectors, the products with A and alpha/beta in FP64 and the
+        V-cycle in precision_mode (0 FP64, 1 FP32, 2 FP16-stored; default: the
+        working precision).  The coarsest level is solved exactly, so
+        coarse_sweeps has no effect.  Returns (u, iterations, ||b - A u|| / ||b||)."""
+        import torch
+        from .solid_solver import pcg_device
+        if precision_mode is None:
+            precision_mode = 1 if self.working_dtype == np.float32 else 0
+        bt = torch.as_tensor(np.ascontiguousarray(b, dtype=np.float64), device=self._S.device)
+        x, it, _, _ = pcg_device(self.system, b=bt, rtol=rtol, max_iter=max_iter, preconditioner="amg",
+                                 precision="fp64", amg=self.amg, amg_precision=self._PREC[int(precision_mode)],
+                                 check_every=max(1, int(check_interval)))
+        u = x.cpu().numpy()
+        nb = float(np.linalg.norm(b))
+        rel = float(np.linalg.norm(b - self._A @ u) / nb) if nb > 0 else 0.0
+        return u, it, rel
+
+    def close(self):
+        if getattr(self, "amg", None) is not None:
+            self.amg.close()
+            self.amg = None
+            self.gpu_handle = None

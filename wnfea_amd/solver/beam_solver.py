@@ -266,3 +266,43 @@ def solve_nonlinear_jfnk(model: FEAModel, n_load_steps: int = 5, max_newton_iter
         bs.close()
     model.displacements = out
     return out
+
+
+def assemble_global_system_sparse(model: FEAModel, device: str = "auto"):
+    """(K csr FP64, F_full) with 6 DOF per node, the reference's
+    ``assembler.py:285-450``: linear beam stiffness (and C3D10 solids in the
+    translational DOFs of their nodes), supports as zeroed rows/columns with a
+    unit diagonal, F = 0 there, zeros eliminated.  Assembled on the device
+    whatever ``device`` says (kept for API parity).  Models mixing beams and
+    solids are not supported here."""
+    import scipy.sparse as sp
+    has_beams = model.mesh_elements is not None and len(model.mesh_elements) > 0
+    has_solids = getattr(model, "solid_elements", None) is not None and len(model.solid_elements) > 0
+    if has_beams and has_solids:
+        raise NotImplementedError("mixed beam-solid models are not supported by this assembly")
+    n = len(model.mesh_nodes)
+    if has_beams:
+        bs = BeamSystem(model)
+        bs.assemble_linear()
+        K = bs.system.K.to_scipy_bsr().tocsr()
+        F = bs.system.F.cpu().numpy()
+        bs.close()
+    elif has_solids:
+        from .solid_solver import assemble_c3d10_sparse_3dof
+        K3, F3, _ = assemble_c3d10_sparse_3dof(model)
+        K3 = K3.tocoo()
+        m3 = (np.arange(3 * n) // 3) * 6 + np.arange(3 * n) % 3  # translational DOF of the 6-DOF layout
+        K = sp.csr_matrix((K3.data, (m3[K3.row], m3[K3.col])), shape=(6 * n, 6 * n))
+        F = np.zeros(6 * n)
+        F[m3] = F3
+        # the reference's rule also fixes the rotational DOFs of supported nodes
+        fixed = model.fixed_dof_mask(6)
+        rot = fixed & (np.arange(6 * n) % 6 >= 3)
+        if rot.any():
+            K = (K + sp.csr_matrix((np.ones(int(rot.sum())), (np.flatnonzero(rot), np.flatnonzero(rot))),
+                                   shape=K.shape)).tocsr()
+    else:
+        return sp.csr_matrix((6 * n, 6 * n)), np.zeros(6 * n)
+    K.eliminate_zeros()
+    K.sort_indices()
+    return K, F

@@ -248,16 +248,19 @@ def refresh_scaling(system: SolidSystem) -> SolidSystem:
 
 
 def system_from_csr(K_csr, F, fixed_mask: np.ndarray | None = None, coords=None,
-                    want_fp64_equilibrated: bool = False, sigma: int = 4096) -> SolidSystem:
-    """Device system from a host scalar CSR (3 DOF per node), e.g. a drop-in caller's K."""
+                    want_fp64_equilibrated: bool = False, sigma: int = 4096, dpn: int = 3) -> SolidSystem:
+    """Device system from a host scalar CSR (``dpn`` = 3 or 6 DOF per node),
+    e.g. a drop-in caller's K."""
     import torch
     from .bsr_matrix import csr_to_bsr
-    Kb = csr_to_bsr(K_csr, 3, np.float64)
+    if dpn not in (3, 6):
+        raise ValueError("dpn must be 3 or 6")
+    Kb = csr_to_bsr(K_csr, dpn, np.float64)
     n = Kb.n_nodes
-    K = DeviceBSR(3, 3, n, n, Kb.block_row_ptr, Kb.block_col_idx, sigma)
+    K = DeviceBSR(dpn, dpn, n, n, Kb.block_row_ptr, Kb.block_col_idx, sigma)
     K.set_values(torch.as_tensor(Kb.values, device=K.device), which=1)
     if fixed_mask is None:
-        fixed_mask = np.zeros(3 * n, dtype=bool)
+        fixed_mask = np.zeros(dpn * n, dtype=bool)
     return _finish_system(K, np.asarray(F, dtype=np.float64), fixed_mask, list(np.flatnonzero(fixed_mask)), n,
                           want_fp64_equilibrated, coords=coords)
 
