@@ -37,6 +37,8 @@ class FEAModel:
     solid_materials: dict[int, str] = field(default_factory=dict)
     supports: list[SupportDef] = field(default_factory=list)
     loads: list[LoadDef] = field(default_factory=list)
+    couplings: list = field(default_factory=list)               # RigidCoupling (mixed beam-solid models)
+    geometry_to_mesh_node_map: dict = field(default_factory=dict)
     displacements: np.ndarray | None = None
     element_results: dict | None = None
     reaction_forces: np.ndarray | None = None
@@ -69,6 +71,8 @@ class FEAModel:
         self.solid_elements = None
         self.solid_materials.clear()
         self.element_properties.clear()
+        self.couplings.clear()
+        self.geometry_to_mesh_node_map.clear()
         self.supports = [s for s in self.supports if s.is_geometry_node]
         self.loads = [ld for ld in self.loads if ld.is_geometry_node]
         self.clear_results()

@@ -229,22 +229,17 @@ class BeamSystem:
 
 def compute_internal_forces_fast(model: FEAModel, U_full, F_int=None, device: str = "hip", block_size: int = 0,
                                  grid_size: int = 0):
-    """Reference-compatible F_int of the beams (``fast_kernels.py:359-448``):
-    U_full (N*6) numpy or device tensor -> numpy (or device tensor for device
-    input).  The launch-shape arguments are accepted and ignored."""
-    import torch
-    bs = BeamSystem(model)
-    try:
-        on_dev = isinstance(U_full, torch.Tensor)
-        U = U_full if on_dev else torch.as_tensor(np.asarray(U_full, dtype=np.float64), device="cuda")
-        F = bs.internal_forces(U.to(torch.float64))
-        out = F if on_dev else F.cpu().numpy()
-        if F_int is not None and not on_dev:
-            F_int[:] += out
-            return F_int
-        return out
-    finally:
-        bs.close()
+    """Reference-compatible F_int (``fast_kernels.py:359-448``) of the beams
+    and C3D10 solids of a model (6 DOF per node): U_full (N*6) numpy or
+    device tensor -> numpy (or device tensor for device input); with F_int
+    (numpy) the forces are added to it.  The device copies of the mesh are
+    kept between calls; the launch-shape arguments are accepted and ignored."""
+    from .residual import compute_internal_forces
+    out = compute_internal_forces(model, U_full)
+    if F_int is not None and isinstance(out, np.ndarray):
+        F_int[:] += out
+        return F_int
+    return out
 
 
 def solve_nonlinear_jfnk(model: FEAModel, n_load_steps: int = 5, max_newton_iter: int = 25, tol_rel: float = 1e-5,
@@ -270,39 +265,46 @@ def solve_nonlinear_jfnk(model: FEAModel, n_load_steps: int = 5, max_newton_iter
 
 def assemble_global_system_sparse(model: FEAModel, device: str = "auto"):
     """(K csr FP64, F_full) with 6 DOF per node, the reference's
-    ``assembler.py:285-450``: linear beam stiffness (and C3D10 solids in the
-    translational DOFs of their nodes), supports as zeroed rows/columns with a
-    unit diagonal, F = 0 there, zeros eliminated.  Assembled on the device
-    whatever ``device`` says (kept for API parity).  Models mixing beams and
-    solids are not supported here."""
+    ``assembler.py:285-450``: linear beam stiffness and C3D10 solids (in the
+    translational DOFs of their nodes) summed, then supports as zeroed
+    rows/columns with a unit diagonal, F = 0 there, zeros eliminated.  The
+    element matrices and their sums are computed on the device whatever
+    ``device`` says (kept for API parity)."""
+    import copy
     import scipy.sparse as sp
     has_beams = model.mesh_elements is not None and len(model.mesh_elements) > 0
     has_solids = getattr(model, "solid_elements", None) is not None and len(model.solid_elements) > 0
-    if has_beams and has_solids:
-        raise NotImplementedError("mixed beam-solid models are not supported by this assembly")
     n = len(model.mesh_nodes)
-    if has_beams:
-        bs = BeamSystem(model)
+    fixed = model.fixed_dof_mask(6)
+    F = model.load_vector(6)
+    if has_beams and not has_solids:
+        bs = BeamSystem(model)  # the device assembly applies the support rule itself
         bs.assemble_linear()
         K = bs.system.K.to_scipy_bsr().tocsr()
-        F = bs.system.F.cpu().numpy()
         bs.close()
-    elif has_solids:
-        from .solid_solver import assemble_c3d10_sparse_3dof
-        K3, F3, _ = assemble_c3d10_sparse_3dof(model)
-        K3 = K3.tocoo()
-        m3 = (np.arange(3 * n) // 3) * 6 + np.arange(3 * n) % 3  # translational DOF of the 6-DOF layout
-        K = sp.csr_matrix((K3.data, (m3[K3.row], m3[K3.col])), shape=(6 * n, 6 * n))
-        F = np.zeros(6 * n)
-        F[m3] = F3
-        # the reference's rule also fixes the rotational DOFs of supported nodes
-        fixed = model.fixed_dof_mask(6)
-        rot = fixed & (np.arange(6 * n) % 6 >= 3)
-        if rot.any():
-            K = (K + sp.csr_matrix((np.ones(int(rot.sum())), (np.flatnonzero(rot), np.flatnonzero(rot))),
-                                   shape=K.shape)).tocsr()
     else:
-        return sp.csr_matrix((6 * n, 6 * n)), np.zeros(6 * n)
+        bare = copy.copy(model)
+        bare.supports = []
+        K = sp.csr_matrix((6 * n, 6 * n))
+        if has_beams:
+            bs = BeamSystem(bare)
+            bs.assemble_linear()
+            K = K + bs.system.K.to_scipy_bsr().tocsr()
+            bs.close()
+        if has_solids:
+            from .solid_solver import assemble_c3d10_bsr
+            s = assemble_c3d10_bsr(bare, apply_bcs=False)
+            K3 = s.K.to_scipy_bsr().tocoo()
+            s.K.close()
+            m3 = (np.arange(3 * n) // 3) * 6 + np.arange(3 * n) % 3  # translational DOF in the 6-DOF layout
+            K = K + sp.csr_matrix((K3.data, (m3[K3.row], m3[K3.col])), shape=(6 * n, 6 * n))
+        K = sp.csr_matrix(K)
+        if fixed.any():
+            K.sort_indices()
+            rows = np.repeat(np.arange(6 * n), np.diff(K.indptr))
+            K.data[(fixed[rows] | fixed[K.indices]) & (rows != K.indices)] = 0.0
+            K.data[fixed[rows] & (rows == K.indices)] = 1.0  # stored diagonals only, as the reference
+    F[fixed] = 0.0
     K.eliminate_zeros()
     K.sort_indices()
     return K, F
