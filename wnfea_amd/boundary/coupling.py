@@ -31,5 +31,5 @@ class RigidCoupling:
     label: str = ""
 
     def compute_coupling_matrices(self, master_coord, slave_coords) -> list:
-        """List of the 3x6 T_i (``coupling.py:36-69``)."""
+        """List of the 3x6 T_i (``coupling.py:33-69``)."""
         return list(coupling_matrices(master_coord, slave_coords))
