@@ -429,6 +429,8 @@ def main() -> int:
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
     ap.add_argument("--operator", choices=["matrix_free", "assembled"], default="matrix_free",
                     help="CG's operator K: matrix-free (quadrature data) or the assembled FP64 BSR matrix")
+    ap.add_argument("--smoother-operator", choices=["matrix_free", "assembled"], default="matrix_free",
+                    help="operator of the finest level's smoothing steps and defect in the V-cycle")
     ap.add_argument("--spmv-shape", default=None,
                     help="waves,unroll,xcd_remap of the SpMV instead of autotuning, for both value precisions "
                          "(e.g. 4,1,0) or as fp32:fp64 (e.g. 2,2,1:4,1,1)")
@@ -499,6 +501,8 @@ def main() -> int:
     main_op = mf_op if args.operator == "matrix_free" else None
     alt_op = None if args.operator == "matrix_free" else mf_op
     alt_name = "assembled" if args.operator == "matrix_free" else "matrix_free"
+    if args.smoother_operator == "matrix_free":
+        mf_op.attach_smoother(amg)
 
     def mixed_solve():
         return solve_mixed(system, amg, tol=1e-8, amg_precision=args.amg_precision, operator=main_op)
@@ -515,7 +519,8 @@ def main() -> int:
     t_mixed = (time.perf_counter() - t0) / args.steps
     u, iters, steps, rel, _ = res[-1]
     rel_true = true_residual(system, u)
-    log(f"[bench] mixed (FP64 CG on {args.operator} K + {args.amg_precision} V-cycle): {t_mixed * 1e3:.1f} ms/solve "
+    log(f"[bench] mixed (FP64 CG on {args.operator} K + {args.amg_precision} V-cycle, {args.smoother_operator} "
+        f"finest smoother): {t_mixed * 1e3:.1f} ms/solve "
         f"iterations {iters} restarts {steps - 1} rel {rel_true:.2e}")
     # all-FP64 AMG-PCG on the same hierarchy
     fp64_solve()
@@ -555,6 +560,25 @@ def main() -> int:
         alt[name] = {"seconds": round(tm, 5), "iterations": itm, "rel_res": relm}
         log(f"[bench] {name} with the {alt_name} CG operator: {tm * 1e3:.1f} ms iters {itm} rel {relm:.2e}")
     alt["speedup_mixed_vs_fp64"] = round(alt["fp64"]["seconds"] / alt["mixed"]["seconds"], 3)
+    # the same two solves with the other operator in the finest-level smoother
+    alt_sm_name = "assembled" if args.smoother_operator == "matrix_free" else "matrix_free"
+    amg.attach_matrix_free(mf_op if alt_sm_name == "matrix_free" else None)
+    alt_sm = {"operator": alt_sm_name}
+    for name, fn in (("mixed", mixed_solve), ("fp64", fp64_solve)):
+        fn()
+        torch.cuda.synchronize()
+        n_rep = max(1, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(n_rep):
+            um, itm, _, _, _ = fn()
+        torch.cuda.synchronize()
+        tm = (time.perf_counter() - t0) / n_rep
+        relm = true_residual(system, um)
+        alt_sm[name] = {"seconds": round(tm, 5), "iterations": itm, "rel_res": relm}
+        log(f"[bench] {name} with the {alt_sm_name} smoother operator: {tm * 1e3:.1f} ms iters {itm} "
+            f"rel {relm:.2e}")
+    alt_sm["speedup_mixed_vs_fp64"] = round(alt_sm["fp64"]["seconds"] / alt_sm["mixed"]["seconds"], 3)
+    amg.attach_matrix_free(mf_op if args.smoother_operator == "matrix_free" else None)
     # the same two solves with every kernel launched from the host
     set_graphs(args.no_graphs)
     eager = {}
@@ -608,7 +632,8 @@ def main() -> int:
         "data": "synthetic (structured C3D10 mesh generated in-process, deterministic loads)",
         "config": {"workload": workload, "n_dofs": meta["n_dofs"], "n_elements": meta["n_elements"],
                    "nnzb": nnzb, "tol": 1e-8, "solver": "FP64 CG + low-precision SA-AMG V-cycle",
-                   "cg_operator": args.operator, "parallelism": "single-gpu"},
+                   "cg_operator": args.operator, "smoother_operator": args.smoother_operator,
+                   "parallelism": "single-gpu"},
         "dof_per_s": round(meta["n_dofs"] / t_mixed, 1),
         "rel_res_fp64_verified": rel_true,
         "iterations": iters,
@@ -617,6 +642,8 @@ def main() -> int:
         "fp64_amg_pcg": {"seconds": round(t_fp64, 5), "iterations": it64, "rel_res": rel64},
         "cg_operator": args.operator,
         "other_cg_operator": alt,
+        "smoother_operator": args.smoother_operator,
+        "other_smoother_operator": alt_sm,
         "speedup_mixed_vs_fp64_amg_pcg": round(t_fp64 / t_mixed, 3),
         "fp64_jacobi_pcg": jac,
         "restart_ir_fp32_krylov": ir,

@@ -84,6 +84,18 @@ int wnfea_pcg_mf(void *solver, void *handle, int pkind, const double *dinv, void
                  const double *scale, const double *b, double *x, double rtol, int max_iter, int check_every,
                  int *iters, double *rel, void *stream);
 
+/* Make the finest level's smoothing steps and defect of the AMG V-cycles
+ * with FP64 or FP32 finest values (amg_vt 0, 1, 4, 5) matrix-free: each
+ * product is diag(scale) K diag(scale) x with K this matrix-free operator
+ * (FP32 cycles use a rounded copy of its quadrature data) and scale the
+ * device FP64 equilibration of the fine matrix (3 * n_nodes), followed by the
+ * step's own update in node order.  Replaces the finest-level smoother and
+ * defect products of the reference's V-cycle (wnfea/solver/native/
+ * hip_kernels.cpp:525-649 jacobi_smooth_csr_*, used at 2106-2160 and
+ * 2219-2240).  handle = NULL detaches; the operator and scale must outlive
+ * the attachment. */
+int wnfea_amg_attach_mf(void *amg, void *handle, const double *scale, void *stream);
+
 /* Gauss-point stresses (optional gp_sig[M][4][6], gp_vm[M][4]), cell means
  * cell_sig[M][6], cell_vm[M], and (if nodal_vm) the nodal average of cell_vm.
  * U has 3 doubles per node.  Replaces element_stresses_c3d10 and

@@ -390,3 +390,44 @@ def test_fp16_smoothing_cycle_is_the_fp32_cycle_with_rounded_smoother(c0_amg):
     assert _rel(z, z_ref) <= 2e-5
     z32 = c0_amg.vcycle(rt, storage="fp32").double().cpu().numpy()
     assert 1e-6 <= _rel(z, z32) <= 2e-2
+
+
+def test_matrix_free_smoothing_cycle_and_solve(c0_system, c0_oracle):
+    """Finest-level smoothing steps and defect on the matrix-free operator
+    (diag(s) K diag(s), the same exact operator as the assembled S K S): the
+    FP64 cycle is still the numpy mirror's cycle, the FP32 one within FP32
+    arithmetic of it, solves keep their accuracy and iteration counts, and
+    detaching restores the assembled cycle bit for bit."""
+    import torch
+    from vcycle_mirror import vcycle
+    from wnfea_amd.solver.solid_solver import MatrixFreeOperator, build_amg, solve_mixed, true_residual
+    _, _, u_ref = c0_oracle
+    amg = build_amg(c0_system, storage=("fp64", "fp32", "fp16"))
+    n = amg.levels[0].A.shape[0]
+    r = np.random.default_rng(12).standard_normal(n)
+    z_ref = vcycle(amg.levels, amg.coarse_inv, r, deg=amg.level_degrees, ratio=amg.ratio)
+    rd = torch.as_tensor(r, device="cuda")
+    z64_asm = amg.vcycle(rd).cpu().numpy()
+    z32_asm = amg.vcycle(rd.float()).double().cpu().numpy()
+    op = MatrixFreeOperator(c0_system)
+    op.attach_smoother(amg)
+    assert amg.matrix_free_smoothing
+    z64 = amg.vcycle(rd).cpu().numpy()
+    assert _rel(z64, z_ref) <= 1e-12 and not np.array_equal(z64, z64_asm)
+    z32 = amg.vcycle(rd.float()).double().cpu().numpy()
+    assert _rel(z32, z_ref) <= 1e-4 and _rel(z32, z32_asm) <= 1e-4
+    z16s = amg.vcycle(rd.float(), storage="fp16_smooth").double().cpu().numpy()
+    assert _rel(z16s, z_ref) <= 2e-2
+    its = {}
+    for prec, tol_u in (("fp64", 1e-8), ("fp32", 1e-6), ("fp16_smooth", 1e-6)):
+        u, it, _, rel, _ = solve_mixed(c0_system, amg, tol=1e-8, amg_precision=prec, operator=op)
+        assert rel <= 1e-8 and true_residual(c0_system, u) <= 1e-8, prec
+        assert _rel(u.cpu().numpy(), u_ref) <= tol_u, prec
+        its[prec] = it
+    op.close()  # detaches
+    assert not amg.matrix_free_smoothing
+    np.testing.assert_array_equal(amg.vcycle(rd).cpu().numpy(), z64_asm)
+    for prec in ("fp64", "fp32"):
+        _, it2, _, _, _ = solve_mixed(c0_system, amg, tol=1e-8, amg_precision=prec)
+        assert abs(its[prec] - it2) <= 2, (prec, its[prec], it2)
+    amg.close()

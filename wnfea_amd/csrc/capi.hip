@@ -51,6 +51,7 @@ int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const i
                   const int* r_rowptr, const int* r_colidx, const double* r_vals, int sigma);
 int amg_set_coarse(Amg& g, int n, const double* inv);
 int amg_set_tune(Amg& g, int prec, int waves, int xcd_remap, int unroll);
+int amg_attach_mf(Amg& g, const MfOp* op, const double* scale, hipStream_t s);
 int amg_finalize(Amg& g, const SpmvTune& t);
 }  // namespace wn
 
@@ -508,6 +509,15 @@ int wnfea_mf_residual(void* solver, void* h, const double* u, const double* f, d
   out[0] = std::sqrt(hs[kScTrue]);
   out[1] = std::sqrt(hs[kScF2]);
   return kOk;
+}
+
+// Run the finest level's smoothing and defect products of the FP32 and FP64
+// V-cycles matrix-free on diag(scale) K diag(scale) (scale: the device FP64
+// equilibration of the fine matrix, 3 * n_nodes); mf = null detaches.  The
+// operator and the scale must outlive the attachment.
+int wnfea_amg_attach_mf(void* amg, void* mf, const double* scale, void* stream) {
+  if (!amg || (mf && !scale)) return kBadArg;
+  return amg_attach_mf(*static_cast<Amg*>(amg), mf ? &static_cast<WnMf*>(mf)->op : nullptr, scale, S(stream));
 }
 
 // FP64 PCG with the matrix-free operator (pkind and preconditioner as in wnfea_pcg)

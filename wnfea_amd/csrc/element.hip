@@ -361,8 +361,9 @@ __global__ void adjoint_dot_kernel(int M, const int* __restrict__ conn, const do
 // ---------------------------------------------------------------- matrix-free K u
 // Quadrature data of the matrix-free operator ("partial assembly"): for each
 // element and Gauss point g the inverse Jacobian J^-1 (row-major) and
-// det J / 24, stored qd[(g*10 + k)*M + e] so that the threads of a block read
-// them coalesced.  A non-positive det J flags the element (bad = e + 1).
+// det J / 24, stored qd[(k*M + e)*4 + g] so that the four lanes of an
+// element and the elements of a wavefront read one contiguous run.  A
+// non-positive det J flags the element (bad = e + 1).
 __global__ __launch_bounds__(128) void c3d10_qdata_kernel(int M, const int* __restrict__ conn,
                                                           const double* __restrict__ nodes, double* __restrict__ qd,
                                                           int* bad) {
@@ -398,94 +399,136 @@ __global__ __launch_bounds__(128) void c3d10_qdata_kernel(int M, const int* __re
                            c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id,
                            (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id};
 #pragma unroll
-    for (int k = 0; k < 9; ++k) qd[((size_t)g * 10 + k) * M + e] = inv[k];
-    qd[((size_t)g * 10 + 9) * M + e] = det / 24.0;
+    for (int k = 0; k < 9; ++k) qd[((size_t)k * M + e) * 4 + g] = inv[k];
+    qd[((size_t)9 * M + e) * 4 + g] = det / 24.0;
   }
 }
 
-// fe = K_e (m .* u)_e from the quadrature data, where m zeroes the DOFs
-// marked in node_fix (bit c: DOF c of the node fixed), stored [i][e][3] for
-// local node i.  One thread per
-// element, in two sweeps over its nodes so that only one node's data is live
-// at a time: (1) strains at the 4 Gauss points, eps_g = sum_i B_gi u_i, from
-// the preloaded J_g^-1; then T_g = (det J_g / 24) S_g J_g^-1 with S_g the
-// stress tensor D eps_g; (2) f_i = sum_g T_g dN_i/dxi(g), written per node.
-__global__ __launch_bounds__(128) void c3d10_pa_elem_kernel(int M, const int* __restrict__ conn,
-                                                            const double* __restrict__ qd,
+// Natural derivatives dN_i/dxi at a Gauss point with barycentric coordinates
+// L (L[0] = 1 - xi - eta - zeta): dN_i/dxi_m = dN_i/dL_{m+1} - dN_i/dL_1,
+// corners N = L(2L - 1), midsides N = 4 L_p L_q in Abaqus order (edges 1-2,
+// 2-3, 3-1, 1-4, 2-4, 3-4).  i is a compile-time constant after unrolling.
+template <typename T>
+__device__ __forceinline__ void natural_grad(int i, const T (&L)[4], T& d0, T& d1, T& d2) {
+  T dl[4] = {T(0), T(0), T(0), T(0)};
+  switch (i) {
+    case 0: dl[0] = T(4) * L[0] - T(1); break;
+    case 1: dl[1] = T(4) * L[1] - T(1); break;
+    case 2: dl[2] = T(4) * L[2] - T(1); break;
+    case 3: dl[3] = T(4) * L[3] - T(1); break;
+    case 4: dl[0] = T(4) * L[1]; dl[1] = T(4) * L[0]; break;
+    case 5: dl[1] = T(4) * L[2]; dl[2] = T(4) * L[1]; break;
+    case 6: dl[0] = T(4) * L[2]; dl[2] = T(4) * L[0]; break;
+    case 7: dl[0] = T(4) * L[3]; dl[3] = T(4) * L[0]; break;
+    case 8: dl[1] = T(4) * L[3]; dl[3] = T(4) * L[1]; break;
+    default: dl[2] = T(4) * L[3]; dl[3] = T(4) * L[2]; break;
+  }
+  d0 = dl[1] - dl[0];
+  d1 = dl[2] - dl[0];
+  d2 = dl[3] - dl[0];
+}
+
+// v from the lane given by a quad permutation (DPP, within groups of 4 lanes)
+template <int Ctrl>
+__device__ __forceinline__ float quad_perm(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), Ctrl, 0xF, 0xF, false));
+}
+template <int Ctrl>
+__device__ __forceinline__ double quad_perm(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(u & 0xffffffffu), Ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), Ctrl, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+// Sum over the 4 lanes of a quad; every lane gets the same bits,
+// (v0 + v1) + (v2 + v3) up to commutation.
+template <typename T>
+__device__ __forceinline__ T quad_sum(T v) {
+  v += quad_perm<0xB1>(v);  // lanes (1, 0, 3, 2): xor 1
+  v += quad_perm<0x4E>(v);  // lanes (2, 3, 0, 1): xor 2
+  return v;
+}
+
+// fe = K_e (sc .* u)_e from the quadrature data, stored [i][e][3] for local
+// node i, where sc is a per-DOF scaling that also carries the Dirichlet rule
+// (0 on a fixed DOF: its column of the operator is zero).  Four lanes per
+// element, lane g for Gauss point g: the strain eps_g = sum_i B_gi u_i from
+// the preloaded J_g^-1, T_g = (det J_g / 24) S_g J_g^-1 with S_g = D eps_g,
+// then each node's contribution T_g dN_i/dxi(g) summed over the quad, lane g
+// storing the nodes i = g (mod 4).  The four-fold split keeps enough waves in
+// flight to hide the latency of the connectivity and nodal-value loads.
+// T is the type of the quadrature data, the vectors and the arithmetic.
+template <typename T>
+__global__ __launch_bounds__(256) void c3d10_pa_elem_kernel(int M, const int* __restrict__ conn,
+                                                            const T* __restrict__ qd,
                                                             const double* __restrict__ E,
                                                             const double* __restrict__ nu,
-                                                            const unsigned char* __restrict__ fix,
-                                                            const double* __restrict__ U, double* __restrict__ fe,
+                                                            const T* __restrict__ sc,
+                                                            const T* __restrict__ U, T* __restrict__ fe,
                                                             const int* flags, int guard) {
   if (guard && flags[0]) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= M) return;
-  double T[4][9];  // J_g^-1 first, then T_g
-  double eps[4][6];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = (int)(t >> 2), g = (int)(t & 3);
+  if (e >= M) return;  // whole quads leave together
+  // barycentric coordinates of Gauss point g: (a, a, a) and its permutations with one b
+  const double ga = (5.0 - sqrt(5.0)) / 20.0, gb = (5.0 + 3.0 * sqrt(5.0)) / 20.0;
+  const double p0 = g == 1 ? gb : ga, p1 = g == 2 ? gb : ga, p2 = g == 3 ? gb : ga;
+  const T L[4] = {T(1.0 - p0 - p1 - p2), T(p0), T(p1), T(p2)};
+  T Ji[9];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
+  for (int k = 0; k < 9; ++k) Ji[k] = qd[((size_t)k * M + e) * 4 + g];
+  const T wd = qd[((size_t)9 * M + e) * 4 + g];
+  T eps[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  int nn[10];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) T[g][k] = qd[((size_t)g * 10 + k) * M + e];
+  for (int i = 0; i < 10; ++i) nn[i] = conn[(size_t)e * 10 + i];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) eps[g][k] = 0.0;
-  }
-#pragma unroll 2
   for (int i = 0; i < 10; ++i) {
-    const int n = conn[(size_t)e * 10 + i];
-    const int m = fix ? fix[n] : 0;
-    const double ux = m & 1 ? 0.0 : U[(size_t)n * 3 + 0];
-    const double uy = m & 2 ? 0.0 : U[(size_t)n * 3 + 1];
-    const double uz = m & 4 ? 0.0 : U[(size_t)n * 3 + 2];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const double d0 = c_dndxi[g][0][i], d1 = c_dndxi[g][1][i], d2 = c_dndxi[g][2][i];
-      const double gx = T[g][0] * d0 + T[g][1] * d1 + T[g][2] * d2;
-      const double gy = T[g][3] * d0 + T[g][4] * d1 + T[g][5] * d2;
-      const double gz = T[g][6] * d0 + T[g][7] * d1 + T[g][8] * d2;
-      eps[g][0] += gx * ux;
-      eps[g][1] += gy * uy;
-      eps[g][2] += gz * uz;
-      eps[g][3] += gy * ux + gx * uy;
-      eps[g][4] += gz * uy + gy * uz;
-      eps[g][5] += gz * ux + gx * uz;
-    }
+    const int n = nn[i];
+    const T ux = U[(size_t)n * 3 + 0] * sc[(size_t)n * 3 + 0];
+    const T uy = U[(size_t)n * 3 + 1] * sc[(size_t)n * 3 + 1];
+    const T uz = U[(size_t)n * 3 + 2] * sc[(size_t)n * 3 + 2];
+    T d0, d1, d2;
+    natural_grad<T>(i, L, d0, d1, d2);
+    const T gx = Ji[0] * d0 + Ji[1] * d1 + Ji[2] * d2;
+    const T gy = Ji[3] * d0 + Ji[4] * d1 + Ji[5] * d2;
+    const T gz = Ji[6] * d0 + Ji[7] * d1 + Ji[8] * d2;
+    eps[0] += gx * ux;
+    eps[1] += gy * uy;
+    eps[2] += gz * uz;
+    eps[3] += gy * ux + gx * uy;
+    eps[4] += gz * uy + gy * uz;
+    eps[5] += gz * ux + gx * uz;
   }
   const double Ev = E[e], nv = nu[e];
   const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
-  const double c11 = f * (1.0 - nv), c12 = f * nv, c44 = Ev / (2.0 * (1.0 + nv));
+  const T c11 = T(f * (1.0 - nv)), c12 = T(f * nv), c44 = T(Ev / (2.0 * (1.0 + nv)));
+  const T s0 = wd * (c11 * eps[0] + c12 * (eps[1] + eps[2]));
+  const T s1 = wd * (c11 * eps[1] + c12 * (eps[0] + eps[2]));
+  const T s2 = wd * (c11 * eps[2] + c12 * (eps[0] + eps[1]));
+  const T s3 = wd * c44 * eps[3], s4 = wd * c44 * eps[4], s5 = wd * c44 * eps[5];
+  // S = [[s0 s3 s5] [s3 s1 s4] [s5 s4 s2]];  Tg = S J^-1 (J^-1 rows in Ji)
+  T Tg[9];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    const double wd = qd[((size_t)g * 10 + 9) * M + e];
-    const double s0 = wd * (c11 * eps[g][0] + c12 * (eps[g][1] + eps[g][2]));
-    const double s1 = wd * (c11 * eps[g][1] + c12 * (eps[g][0] + eps[g][2]));
-    const double s2 = wd * (c11 * eps[g][2] + c12 * (eps[g][0] + eps[g][1]));
-    const double s3 = wd * c44 * eps[g][3], s4 = wd * c44 * eps[g][4], s5 = wd * c44 * eps[g][5];
-    // S = [[s0 s3 s5] [s3 s1 s4] [s5 s4 s2]];  T_g = S J^-1 (J^-1 rows in T[g])
-    double t[9];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      t[0 + c] = s0 * T[g][c] + s3 * T[g][3 + c] + s5 * T[g][6 + c];
-      t[3 + c] = s3 * T[g][c] + s1 * T[g][3 + c] + s4 * T[g][6 + c];
-      t[6 + c] = s5 * T[g][c] + s4 * T[g][3 + c] + s2 * T[g][6 + c];
-    }
-#pragma unroll
-    for (int k = 0; k < 9; ++k) T[g][k] = t[k];
+  for (int c = 0; c < 3; ++c) {
+    Tg[0 + c] = s0 * Ji[c] + s3 * Ji[3 + c] + s5 * Ji[6 + c];
+    Tg[3 + c] = s3 * Ji[c] + s1 * Ji[3 + c] + s4 * Ji[6 + c];
+    Tg[6 + c] = s5 * Ji[c] + s4 * Ji[3 + c] + s2 * Ji[6 + c];
   }
-#pragma unroll 2
-  for (int i = 0; i < 10; ++i) {
-    double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const double d0 = c_dndxi[g][0][i], d1 = c_dndxi[g][1][i], d2 = c_dndxi[g][2][i];
-      fx += T[g][0] * d0 + T[g][1] * d1 + T[g][2] * d2;
-      fy += T[g][3] * d0 + T[g][4] * d1 + T[g][5] * d2;
-      fz += T[g][6] * d0 + T[g][7] * d1 + T[g][8] * d2;
+  for (int i = 0; i < 10; ++i) {
+    T d0, d1, d2;
+    natural_grad<T>(i, L, d0, d1, d2);
+    const T fx = quad_sum(Tg[0] * d0 + Tg[1] * d1 + Tg[2] * d2);
+    const T fy = quad_sum(Tg[3] * d0 + Tg[4] * d1 + Tg[5] * d2);
+    const T fz = quad_sum(Tg[6] * d0 + Tg[7] * d1 + Tg[8] * d2);
+    if ((i & 3) == g) {
+      // layout [local node][element][3]
+      T* o = fe + ((size_t)i * M + e) * 3;
+      o[0] = fx;
+      o[1] = fy;
+      o[2] = fz;
     }
-    // layout [local node][element][3]: the block's stores are contiguous
-    double* o = fe + ((size_t)i * M + e) * 3;
-    o[0] = fx;
-    o[1] = fy;
-    o[2] = fz;
   }
 }
 
@@ -495,13 +538,15 @@ __global__ __launch_bounds__(128) void c3d10_pa_elem_kernel(int M, const int* __
 //   mode 2: y_d = f_d - (K u)_d and partial y.y,
 // where (K u)_d = u_d for a fixed DOF, else the sum of the node's element
 // contributions in element order.  Each wavefront takes one chunk of whole
-// nodes, at most 64 of them with at most 64 incidences in all (chunk
-// boundaries fixed at set-up): lane l loads incidence l of the chunk (one
-// round of independent loads, issued together with the node's own operands),
-// the values meet in LDS, and one lane per node adds its incidences in
-// order.  A node with more than 64 incidences is a chunk of its own, summed
-// by its first lane from memory.
+// nodes, at most 64 of them with at most kChunkInc incidences in all (chunk
+// boundaries fixed at set-up): lane l loads incidences l, l + 64, ... of the
+// chunk (one round of independent loads, issued together with the node's own
+// operands), the values meet in LDS, and one lane per node adds its
+// incidences in order.  A node with more than kChunkInc incidences is a chunk
+// of its own, summed by its first lane from memory.
 constexpr int kGatherWaves = 4;
+constexpr int kIncPerLane = 4;
+constexpr int kChunkInc = 64 * kIncPerLane;
 
 __global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_kernel(
     int n_chunks, int M, const int2* __restrict__ chunk, const int* __restrict__ n2e_ptr,
@@ -509,7 +554,7 @@ __global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_kernel(
     const double* __restrict__ u, const double* __restrict__ f, double* __restrict__ y, double* partials, int mode,
     const int* flags, int guard) {
   if (guard && flags[0]) return;
-  __shared__ double sv[kGatherWaves][64][3];
+  __shared__ double sv[kGatherWaves][kChunkInc][3];
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
   const int c = blockIdx.x * kGatherWaves + w;
   const bool live = c < n_chunks;
@@ -517,7 +562,7 @@ __global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_kernel(
   const int2 a = live ? chunk[c] : make_int2(0, 0);
   const int2 b = live ? chunk[c + 1] : make_int2(0, 0);
   const int k0 = a.y, k1 = b.y;
-  const bool big = k1 - k0 > 64;  // then the chunk is one node
+  const bool big = k1 - k0 > kChunkInc;  // then the chunk is one node
   // this lane's node and its operands, loaded before the incidences
   const int n = a.x + l;
   const bool has = n < b.x;
@@ -533,18 +578,21 @@ __global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_kernel(
       if (mode == 2) ff[cc] = f[(size_t)n * 3 + cc];
     }
   }
-  const int k = k0 + l;
-  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-  if (!big && k < k1) {
-    const int code = n2e[k];  // e * 10 + local
-    const double* p = fe + ((size_t)(code % 10) * M + code / 10) * 3;
-    v0 = p[0];
-    v1 = p[1];
-    v2 = p[2];
+#pragma unroll
+  for (int q = 0; q < kIncPerLane; ++q) {
+    const int k = k0 + l + 64 * q;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    if (!big && k < k1) {
+      const int code = n2e[k];  // e * 10 + local
+      const double* p = fe + ((size_t)(code % 10) * M + code / 10) * 3;
+      v0 = p[0];
+      v1 = p[1];
+      v2 = p[2];
+    }
+    sv[w][l + 64 * q][0] = v0;
+    sv[w][l + 64 * q][1] = v1;
+    sv[w][l + 64 * q][2] = v2;
   }
-  sv[w][l][0] = v0;
-  sv[w][l][1] = v1;
-  sv[w][l][2] = v2;
   __syncthreads();
   double dot = 0.0;
   if (has) {
@@ -585,6 +633,100 @@ __global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_kernel(
   }
 }
 
+// The same gather for the V-cycle's finest level, on the scaled operator
+// A = diag(sc) K diag(sc): for DOF k of node n, a_k = sc_k (K (sc .* x))_k
+// ((K u)_k = u_k for a fixed DOF), then the epilogue Op (a Chebyshev step or
+// the defect, as in the sliced-ELL products) consumes a_k with its operands
+// loaded before the incidences.  Output order is node order, so every
+// operand and result stream is contiguous.
+template <typename T, typename Op>
+__global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_op_kernel(
+    int n_chunks, int M, const int2* __restrict__ chunk, const int* __restrict__ n2e_ptr,
+    const int* __restrict__ n2e, const T* __restrict__ fe, const unsigned char* __restrict__ fix,
+    const T* __restrict__ sc, const T* __restrict__ x, Op op, const int* skip) {
+  if (skip && *skip) return;
+  __shared__ T sv[kGatherWaves][kChunkInc][3];
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int c = blockIdx.x * kGatherWaves + w;
+  const bool live = c < n_chunks;
+  const int2 a = live ? chunk[c] : make_int2(0, 0);
+  const int2 b = live ? chunk[c + 1] : make_int2(0, 0);
+  const int k0 = a.y, k1 = b.y;
+  const bool big = k1 - k0 > kChunkInc;
+  const int n = a.x + l;
+  const bool has = n < b.x;
+  int j0 = 0, j1 = 0, m = 0;
+  T s3[3] = {T(1), T(1), T(1)}, xf[3] = {T(0), T(0), T(0)};
+  typename Op::Pre pre[3];
+  if (has) {
+    j0 = n2e_ptr[n] - k0;
+    j1 = n2e_ptr[n + 1] - k0;
+    m = fix ? fix[n] : 0;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const long long k = (long long)n * 3 + cc;
+      op.load(k, pre[cc]);
+      if (sc) s3[cc] = sc[k];
+      if ((m >> cc) & 1) xf[cc] = x[k];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kIncPerLane; ++q) {
+    const int k = k0 + l + 64 * q;
+    T v0 = T(0), v1 = T(0), v2 = T(0);
+    if (!big && k < k1) {
+      const int code = n2e[k];  // e * 10 + local
+      const T* p = fe + ((size_t)(code % 10) * M + code / 10) * 3;
+      v0 = p[0];
+      v1 = p[1];
+      v2 = p[2];
+    }
+    sv[w][l + 64 * q][0] = v0;
+    sv[w][l + 64 * q][1] = v1;
+    sv[w][l + 64 * q][2] = v2;
+  }
+  __syncthreads();
+  if (!has) return;
+  T acc[3] = {T(0), T(0), T(0)};
+  if (big) {
+    for (int j = k0; j < k1; ++j) {
+      const int code = n2e[j];
+      const T* p = fe + ((size_t)(code % 10) * M + code / 10) * 3;
+      acc[0] += p[0];
+      acc[1] += p[1];
+      acc[2] += p[2];
+    }
+  } else {
+    for (int j = j0; j < j1; ++j) {
+      acc[0] += sv[w][j][0];
+      acc[1] += sv[w][j][1];
+      acc[2] += sv[w][j][2];
+    }
+  }
+  double dots[2] = {0.0, 0.0};
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    const T ku = (m >> cc) & 1 ? s3[cc] * xf[cc] : acc[cc];
+    op.out((long long)n * 3 + cc, s3[cc] * ku, dots, pre[cc]);
+  }
+}
+
+// out_k = 0 for a DOF fixed in node_fix (bit k % 3 of node k / 3), else
+// scale_k (1 without a scale): the element kernel's input scaling.
+template <typename T>
+__global__ void dof_scale_kernel(long long n, const double* __restrict__ scale, const unsigned char* __restrict__ fix,
+                                 T* __restrict__ out) {
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+    const bool fixed = fix && ((fix[k / 3] >> (k % 3)) & 1);
+    out[k] = fixed ? T(0) : (scale ? T(scale[k]) : T(1));
+  }
+}
+
+__global__ void round_to_float_kernel(long long n, const double* __restrict__ src, float* __restrict__ dst) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
 // ---------------------------------------------------------------- host wrappers
 int launch_ke(int M, const int* conn, const double* nodes, const double* E, const double* nu, double* Ke, int* bad,
               hipStream_t s) {
@@ -614,16 +756,16 @@ int launch_gather_node(int n_nodes, const int* n2e_ptr, const int* n2e, const do
   return static_cast<int>(hipGetLastError());
 }
 
-// Gather chunks: consecutive nodes, at most 64 per chunk and at most 64
-// incidences in all, or one node alone when it has more.
+// Gather chunks: consecutive nodes, at most 64 per chunk and at most
+// kChunkInc incidences in all, or one node alone when it has more.
 std::vector<int> mf_chunks(const std::vector<int>& ptr) {  // first node of each chunk, then n
   const int n = (int)ptr.size() - 1;
   std::vector<int> c{0};
   int start = 0;
   while (start < n) {
     int end = start + 1;
-    if (ptr[end] - ptr[start] <= 64)
-      while (end < n && end - start < 64 && ptr[end + 1] - ptr[start] <= 64) ++end;
+    if (ptr[end] - ptr[start] <= kChunkInc)
+      while (end < n && end - start < 64 && ptr[end + 1] - ptr[start] <= kChunkInc) ++end;
     c.push_back(end);
     start = end;
   }
@@ -647,16 +789,19 @@ int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s) {
   if (op.M <= 0) return kOk;
   int r = upload_element_consts(s);
   if (r) return r;
+  const long long n = 3LL * op.n_nodes;
   WN_HIP(hipMalloc(&op.qd, sizeof(double) * 40 * (size_t)op.M));
   WN_HIP(hipMalloc(&op.fe, sizeof(double) * 30 * (size_t)op.M));
+  WN_HIP(hipMalloc(&op.mask, sizeof(double) * n));
   hipLaunchKernelGGL(c3d10_qdata_kernel, dim3(ceil_div(op.M, 128)), dim3(128), 0, s, op.M, op.conn, nodes, op.qd, bad);
+  hipLaunchKernelGGL(dof_scale_kernel<double>, dim3(vec_grid(n)), dim3(256), 0, s, n, nullptr, op.fix, op.mask);
   return static_cast<int>(hipGetLastError());
 }
 
 void mf_release(MfOp& op) {
-  for (void* p : {(void*)op.qd, (void*)op.fe, (void*)op.chunk})
+  for (void* p : {(void*)op.qd, (void*)op.fe, (void*)op.mask, (void*)op.chunk})
     if (p) (void)hipFree(p);
-  op.qd = op.fe = nullptr;
+  op.qd = op.fe = op.mask = nullptr;
   op.chunk = nullptr;
   op.n_chunks = 0;
 }
@@ -667,9 +812,9 @@ int mf_apply(const MfOp& op, const double* u, const double* f, double* y, double
              const int* flags, int guard, hipStream_t s) {
   if (!op.chunk) return kNotReady;
   if (op.M > 0) {
-    if (!op.qd || !op.fe) return kNotReady;
-    hipLaunchKernelGGL(c3d10_pa_elem_kernel, dim3(ceil_div(op.M, 128)), dim3(128), 0, s, op.M, op.conn, op.qd, op.E,
-                       op.nu, op.fix, u, op.fe, flags, guard);
+    if (!op.qd || !op.fe || !op.mask) return kNotReady;
+    hipLaunchKernelGGL(c3d10_pa_elem_kernel<double>, dim3(ceil_div(4L * op.M, 256)), dim3(256), 0, s, op.M, op.conn,
+                       op.qd, op.E, op.nu, op.mask, u, op.fe, flags, guard);
   }
   hipLaunchKernelGGL(mf_gather_kernel, dim3(mf_grid(op)), dim3(64 * kGatherWaves), 0, s, op.n_chunks, op.M, op.chunk,
                      op.n2e_ptr, op.n2e, op.fe, op.fix, u, f, y, partials, mode, flags, guard);
@@ -700,5 +845,88 @@ int launch_adjoint_dot(int M, const int* conn, const double* E, const double* fe
   hipLaunchKernelGGL(adjoint_dot_kernel, dim3(ceil_div(M, 256)), dim3(256), 0, s, M, conn, E, fe, lam, grad);
   return static_cast<int>(hipGetLastError());
 }
+
+// ---------------------------------------------------------------- V-cycle smoothing on the matrix-free operator
+void mf_smooth_release(MfSmooth& m) {
+  for (void* p : {(void*)m.s64e, (void*)m.s32, (void*)m.s32e, (void*)m.qd32, (void*)m.fe32})
+    if (p) (void)hipFree(p);
+  m = MfSmooth{};
+}
+
+int mf_smooth_attach(MfSmooth& m, const MfOp* op, const double* scale, int want32, hipStream_t s) {
+  mf_smooth_release(m);
+  if (!op) return kOk;  // detach
+  if (!op->chunk || (op->M > 0 && (!op->qd || !op->fe)) || !scale) return kBadArg;
+  const long long n = 3LL * op->n_nodes;
+  const size_t nv = (size_t)std::max<long long>(n, 1);
+  WN_HIP(hipMalloc(&m.s64e, sizeof(double) * nv));
+  hipLaunchKernelGGL(dof_scale_kernel<double>, dim3(vec_grid(n)), dim3(256), 0, s, n, scale, op->fix, m.s64e);
+  if (want32) {
+    WN_HIP(hipMalloc(&m.s32, sizeof(float) * nv));
+    WN_HIP(hipMalloc(&m.s32e, sizeof(float) * nv));
+    WN_HIP(hipMalloc(&m.qd32, sizeof(float) * 40 * std::max<size_t>((size_t)op->M, 1)));
+    WN_HIP(hipMalloc(&m.fe32, sizeof(float) * 30 * std::max<size_t>((size_t)op->M, 1)));
+    hipLaunchKernelGGL(round_to_float_kernel, dim3(vec_grid(n)), dim3(256), 0, s, n, scale, m.s32);
+    hipLaunchKernelGGL(dof_scale_kernel<float>, dim3(vec_grid(n)), dim3(256), 0, s, n, scale, op->fix, m.s32e);
+    if (op->M > 0)
+      hipLaunchKernelGGL(round_to_float_kernel, dim3(vec_grid(40LL * op->M)), dim3(256), 0, s, 40LL * op->M, op->qd,
+                         m.qd32);
+  }
+  WN_HIP(hipGetLastError());
+  m.op = op;
+  m.s64 = scale;
+  return kOk;
+}
+
+template <> bool mf_smooth_ready<double>(const MfSmooth& m) { return m.op && m.s64 && m.s64e; }
+template <> bool mf_smooth_ready<float>(const MfSmooth& m) { return m.op && m.s32 && m.s32e && m.qd32 && m.fe32; }
+
+// scaling of the gathered result (the true s) and of the element input (s with fixed DOFs zeroed)
+template <typename T> static const T* mf_scale(const MfSmooth& m);
+template <> const double* mf_scale<double>(const MfSmooth& m) { return m.s64; }
+template <> const float* mf_scale<float>(const MfSmooth& m) { return m.s32; }
+template <typename T> static const T* mf_scale_in(const MfSmooth& m);
+template <> const double* mf_scale_in<double>(const MfSmooth& m) { return m.s64e; }
+template <> const float* mf_scale_in<float>(const MfSmooth& m) { return m.s32e; }
+template <typename T> static const T* mf_qd(const MfSmooth& m);
+template <> const double* mf_qd<double>(const MfSmooth& m) { return m.op->qd; }
+template <> const float* mf_qd<float>(const MfSmooth& m) { return m.qd32; }
+template <typename T> static T* mf_fe(const MfSmooth& m);
+template <> double* mf_fe<double>(const MfSmooth& m) { return m.op->fe; }
+template <> float* mf_fe<float>(const MfSmooth& m) { return m.fe32; }
+
+template <typename T, typename Op>
+static int mf_smooth_run(const MfSmooth& m, const T* x, const Op& o, const int* skip, hipStream_t s) {
+  if (!mf_smooth_ready<T>(m)) return kNotReady;
+  const MfOp& op = *m.op;
+  const T* sc = mf_scale<T>(m);
+  if (op.M > 0)
+    hipLaunchKernelGGL(c3d10_pa_elem_kernel<T>, dim3(ceil_div(4L * op.M, 256)), dim3(256), 0, s, op.M, op.conn,
+                       mf_qd<T>(m), op.E, op.nu, mf_scale_in<T>(m), x, mf_fe<T>(m), skip, skip ? 1 : 0);
+  hipLaunchKernelGGL((mf_gather_op_kernel<T, Op>), dim3(mf_grid(op)), dim3(64 * kGatherWaves), 0, s, op.n_chunks,
+                     op.M, op.chunk, op.n2e_ptr, op.n2e, mf_fe<T>(m), op.fix, sc, x, o, skip);
+  return static_cast<int>(hipGetLastError());
+}
+
+template <typename T>
+int mf_smooth_cheb(const MfSmooth& m, const T* x_in, T* x_out, const T* b, const T* dinv, T* d, T c1, T c2,
+                   const int* skip, hipStream_t s) {
+  OpCheb<3, T> o{b, dinv, x_in, x_out, d, c1, c2};
+  return mf_smooth_run<T>(m, x_in, o, skip, s);
+}
+
+template <typename T>
+int mf_smooth_defect(const MfSmooth& m, const T* x, const T* b, T* res, const int* skip, hipStream_t s) {
+  OpDefect<3, T> o{b, res};
+  return mf_smooth_run<T>(m, x, o, skip, s);
+}
+
+template int mf_smooth_cheb<double>(const MfSmooth&, const double*, double*, const double*, const double*, double*,
+                                    double, double, const int*, hipStream_t);
+template int mf_smooth_cheb<float>(const MfSmooth&, const float*, float*, const float*, const float*, float*, float,
+                                   float, const int*, hipStream_t);
+template int mf_smooth_defect<double>(const MfSmooth&, const double*, const double*, double*, const int*,
+                                      hipStream_t);
+template int mf_smooth_defect<float>(const MfSmooth&, const float*, const float*, float*, const int*, hipStream_t);
 
 }  // namespace wn

@@ -640,8 +640,13 @@ static int pcg_multi_run(Solver& S, const SellMatrix& A, Amg* g, int vt, const d
   if ((r = mr_fin<NR>(M, gv, kMrStart, rtol * rtol, s))) return r;
   const int gs = spmm_grid64(A, t);
   int done_all = 0;
+  // chunks sized from each open column's residual reduction rate, as in the
+  // single-column PCG (idle iterations after convergence are costly)
+  int chunk = check_every, it_prev = 0;
+  double rr_prev[NR] = {};
   for (int it = 0; it < max_iter && !done_all;) {
-    const int chunk = std::min(check_every, max_iter - it);
+    chunk = std::max(1, std::min(chunk, max_iter - it));
+    const int it0 = it;
     for (int c = 0; c < chunk; ++c, ++it) {
       if ((r = vcycle_mr<float, float, float, NR>(*g, M, 0, M.rp32, M.zp32, M.flags, s))) return r;
       hipLaunchKernelGGL((mr_rho_kernel<float, NR>), dim3(gv), dim3(kMrBlock), 0, s, n, M.r, M.zp32, scale,
@@ -659,8 +664,25 @@ static int pcg_multi_run(Solver& S, const SellMatrix& A, Amg* g, int vt, const d
       if ((r = mr_fin<NR>(M, gv, kMrRR, 0.0, s))) return r;
     }
     WN_HIP(hipMemcpyAsync(M.h_flags, M.flags, sizeof(int) * (2 + 2 * NR), hipMemcpyDeviceToHost, s));
+    WN_HIP(hipMemcpyAsync(M.h_sc, M.sc, sizeof(double) * kMsCount * NR, hipMemcpyDeviceToHost, s));
     WN_HIP(hipStreamSynchronize(s));
     done_all = M.h_flags[0];
+    if (done_all) break;
+    double need_max = 0.0;
+    bool known = true;
+    for (int k = 0; k < NR; ++k) {
+      if (M.h_flags[2 + k]) continue;
+      const double rr = M.h_sc[kMsRR * NR + k], tol2 = M.h_sc[kMsTol2 * NR + k];
+      const double prev = it_prev == 0 ? M.h_sc[kMsRR0 * NR + k] : rr_prev[k];
+      rr_prev[k] = rr;
+      if (!(rr > 0.0 && rr < prev && tol2 > 0.0 && tol2 < rr)) {
+        known = false;
+        continue;
+      }
+      need_max = std::max(need_max, std::log(tol2 / rr) / (std::log(rr / prev) / (it - it0)));
+    }
+    it_prev = it;
+    chunk = (known && need_max < check_every) ? std::max(1, (int)need_max) : check_every;
   }
   WN_HIP(hipMemcpyAsync(M.h_sc, M.sc, sizeof(double) * kMsCount * NR, hipMemcpyDeviceToHost, s));
   WN_HIP(hipMemcpyAsync(M.h_flags, M.flags, sizeof(int) * (2 + 2 * NR), hipMemcpyDeviceToHost, s));

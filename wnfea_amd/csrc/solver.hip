@@ -247,14 +247,20 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   T* post = (deg % 2 == 0) ? x : tmp;
   T* cur = ((deg - 1) % 2 == 0) ? post : (post == x ? tmp : x);
   T* oth = (cur == x) ? tmp : x;
+  // the finest level of an FP32 / FP64 cycle may run its products matrix-free
+  const bool mf = l == 0 && std::is_same<VT, T>::value && mf_smooth_ready<T>(g.mfs);
+  auto cheb = [&](const T* xi, T* xo, int k) -> int {
+    if (mf) return mf_smooth_cheb<T>(g.mfs, xi, xo, b, dinv, d, T(c.c1[k]), T(c.c2[k]), skip, s);
+    return spmv_cheb<VS, T>(A, xi, xo, b, dinv, d, c.c1[k], c.c2[k], t, s);
+  };
   hipLaunchKernelGGL(cheb_first_kernel<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, b, dinv, d, cur, T(c.c2[0]), skip);
   if ((r = static_cast<int>(hipGetLastError()))) return r;
   for (int k = 1; k < deg; ++k) {
-    if ((r = spmv_cheb<VS, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
+    if ((r = cheb(cur, oth, k))) return r;
     std::swap(cur, oth);
   }
   // defect and restriction
-  if ((r = spmv_defect<VT, T>(A, cur, b, res, t, s))) return r;
+  if ((r = mf ? mf_smooth_defect<T>(g.mfs, cur, b, res, skip, s) : spmv_defect<VT, T>(A, cur, b, res, t, s))) return r;
   AmgLevel& N = *g.L[l + 1];
   T** nv = level_vecs<T>(N);
   if ((r = spmv_restrict<VT, T>(L.R, res, nv[0], t, s))) return r;
@@ -262,7 +268,7 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   if ((r = spmv_addto<VT, T>(L.P, nv[1], cur, t, s))) return r;  // cur == post here
   oth = (cur == x) ? tmp : x;
   for (int k = 0; k < deg; ++k) {
-    if ((r = spmv_cheb<VS, T>(A, cur, oth, b, dinv, d, c.c1[k], c.c2[k], t, s))) return r;
+    if ((r = cheb(cur, oth, k))) return r;
     std::swap(cur, oth);
   }
   return kOk;
@@ -340,7 +346,9 @@ static void key_amg(std::vector<uint64_t>& k, const Amg* g) {
   if (!g) return;
   for (int v : {g->store, g->n_coarse, (int)g->L.size()}) key_push(k, v);
   for (const void* p : {(const void*)g->inv, (const void*)g->inv32, (const void*)g->tiles64, (const void*)g->tiles32,
-                        (const void*)g->tij, (const void*)g->tpart})
+                        (const void*)g->tij, (const void*)g->tpart, (const void*)g->mfs.op,
+                        (const void*)g->mfs.s64, (const void*)g->mfs.s32, (const void*)g->mfs.qd32,
+                        (const void*)g->mfs.fe32})
     key_push(k, p);
   key_tune(k, g->tune);
   for (const auto& lp : g->L) {
@@ -482,8 +490,16 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
     key_amg(key, pkind == 2 ? amg : nullptr);
   }
   int done_iters = 0;
+  // Iterations enqueued after convergence still launch every kernel (each
+  // returns at once, but a grid of thousands of workgroups costs several us
+  // to retire), so a chunk is sized from the residual reduction rate of the
+  // previous one to end near convergence: rounded down, since one more small
+  // chunk costs a host round trip, far less than idle iterations.
+  int chunk = std::min(check_every, max_iter);
+  int it_prev = 0;
+  double rr_prev = 0.0;
   for (int it = 0; it < max_iter;) {
-    const int chunk = std::min(check_every, max_iter - it);
+    chunk = std::max(1, std::min(chunk, max_iter - it));
     if (S.use_graphs && chunk == check_every) {
       r = graph_for(S, key, [&](hipStream_t st) -> int {
         for (int c = 0; c < chunk; ++c) {
@@ -500,9 +516,19 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
     }
     it += chunk;
     WN_HIP(hipMemcpyAsync(S.h_flags, S.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    WN_HIP(hipMemcpyAsync(S.h_sc, S.ws.sc, sizeof(double) * kScCount, hipMemcpyDeviceToHost, s));
     WN_HIP(hipStreamSynchronize(s));
     done_iters = S.h_flags[1];
     if (S.h_flags[0]) break;
+    const double rr = S.h_sc[kScRR], tol2 = S.h_sc[kScTol2];
+    if (it_prev == 0) rr_prev = S.h_sc[kScRR0];
+    chunk = check_every;
+    if (rr > 0.0 && rr < rr_prev && tol2 > 0.0 && tol2 < rr) {
+      const double need = std::log(tol2 / rr) / (std::log(rr / rr_prev) / (it - it_prev));
+      if (need < check_every) chunk = std::max(1, (int)need);
+    }
+    it_prev = it;
+    rr_prev = rr;
   }
   WN_HIP(hipMemcpyAsync(S.h_sc, S.ws.sc, sizeof(double) * kScCount, hipMemcpyDeviceToHost, s));
   WN_HIP(hipStreamSynchronize(s));
@@ -694,6 +720,11 @@ int amg_set_coarse(Amg& g, int n, const double* inv) {
   WN_HIP(hipDeviceSynchronize());
   g.n_coarse = n;
   return kOk;
+}
+
+int amg_attach_mf(Amg& g, const MfOp* op, const double* scale, hipStream_t s) {
+  if (op && (g.L.empty() || 3LL * op->n_nodes != g.L[0]->n)) return kBadArg;
+  return mf_smooth_attach(g.mfs, op, scale, (g.store & 2) ? 1 : 0, s);
 }
 
 int amg_set_tune(Amg& g, int prec, int waves, int xcd_remap, int unroll) {

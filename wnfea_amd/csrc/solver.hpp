@@ -39,8 +39,11 @@ struct Amg {
   int2* tij = nullptr;
   double* tpart = nullptr;
   SpmvTune tune;
+  // optional: the finest level's smoothing and defect products matrix-free
+  MfSmooth mfs;
   ~Amg() { release(); }
   void release() {
+    mf_smooth_release(mfs);
     for (auto& lp : L) {
       AmgLevel& l = *lp;
       if (l.own_A && l.A) {

@@ -88,11 +88,38 @@ struct MfOp {
   const unsigned char* fix = nullptr;  // per node, bit c: DOF c fixed (may be null)
   const int* n2e_ptr = nullptr;
   const int* n2e = nullptr;
-  double* qd = nullptr;  // [4][10][M] inverse Jacobians and det/24
-  double* fe = nullptr;  // [10][M][3] element contributions per local node
+  double* qd = nullptr;    // [10][M][4] inverse Jacobian entries and det/24 per Gauss point
+  double* fe = nullptr;    // [10][M][3] element contributions per local node
+  double* mask = nullptr;  // [3 n_nodes] 0 for a fixed DOF, else 1
   int2* chunk = nullptr;  // [n_chunks+1] {first node, first incidence} of each gather chunk
   int n_chunks = 0;
 };
+// The finest level's smoothing and defect products of a V-cycle run
+// matrix-free on the equilibrated operator: y = s .* K (s .* x), with K the
+// matrix-free C3D10 operator (assembled Dirichlet rule) and s the diagonal
+// scaling.  FP64 uses the operator's own quadrature data and element buffer;
+// FP32 keeps rounded copies of the quadrature data and the scaling.
+struct MfSmooth {
+  const MfOp* op = nullptr;     // not owned: it must outlive the attachment
+  const double* s64 = nullptr;  // not owned
+  double* s64e = nullptr;       // s with fixed DOFs zeroed (the element kernel's input scaling)
+  float* s32 = nullptr;
+  float* s32e = nullptr;
+  float* qd32 = nullptr;  // [10][M][4]
+  float* fe32 = nullptr;  // [10][M][3]
+};
+int mf_smooth_attach(MfSmooth& m, const MfOp* op, const double* scale, int want32, hipStream_t s);
+void mf_smooth_release(MfSmooth& m);
+template <typename T> bool mf_smooth_ready(const MfSmooth& m);
+template <> bool mf_smooth_ready<double>(const MfSmooth& m);
+template <> bool mf_smooth_ready<float>(const MfSmooth& m);
+// x_out = x_in + d',  d' = c1 d + c2 dinv (b - A x_in)
+template <typename T>
+int mf_smooth_cheb(const MfSmooth& m, const T* x_in, T* x_out, const T* b, const T* dinv, T* d, T c1, T c2,
+                   const int* skip, hipStream_t s);
+// res = b - A x
+template <typename T>
+int mf_smooth_defect(const MfSmooth& m, const T* x, const T* b, T* res, const int* skip, hipStream_t s);
 std::vector<int> mf_chunks(const std::vector<int>& n2e_ptr);
 int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s);
 void mf_release(MfOp& op);

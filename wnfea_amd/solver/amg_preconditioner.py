@@ -307,10 +307,38 @@ class SmoothedAggregationAMG:
                                              _lib.stream_of()), "wnfea_amg_vcycle")
         return z
 
+    def attach_matrix_free(self, op, scale=None):
+        """Run the finest level's smoothing steps and defect of the FP64 and
+        FP32 cycles on the matrix-free operator ``op`` (a MatrixFreeOperator
+        of the fine system) scaled as the fine level, diag(scale) K
+        diag(scale); ``scale`` defaults to the system's equilibration.  None
+        detaches.  Attach again after the system's values change (the FP32
+        cycle keeps rounded copies).  The hierarchy keeps ``op`` alive while
+        attached."""
+        prev = getattr(self, "_mf", None)
+        if op is not None and scale is None:
+            scale = op.system.scale
+        _lib.check(self.lib.wnfea_amg_attach_mf(self.handle, op.handle if op is not None else None,
+                                                _lib.dptr(scale) if op is not None else None, _lib.stream_of()),
+                   "wnfea_amg_attach_mf")
+        if prev is not None and prev is not op:
+            prev._attached.discard(self)
+        self._mf, self._mf_scale = op, scale
+        if op is not None:
+            op._attached.add(self)
+
+    @property
+    def matrix_free_smoothing(self) -> bool:
+        return getattr(self, "_mf", None) is not None
+
     def close(self):
         if getattr(self, "handle", None):
             self.lib.wnfea_amg_destroy(self.handle)
             self.handle = None
+        mf = getattr(self, "_mf", None)
+        if mf is not None:
+            mf._attached.discard(self)
+            self._mf = self._mf_scale = None
 
     def __del__(self):
         try:

@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import os
 import time
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -382,10 +383,17 @@ class MatrixFreeOperator:
                                             _lib.dptr(m["n2e_ptr"]), _lib.dptr(m["n2e"]), _lib.dptr(bad),
                                             _lib.ctypes.byref(h), _lib.stream_of()), "wnfea_mf_create")
         self.handle = h.value
+        self._attached = weakref.WeakSet()  # hierarchies smoothing on this operator
         if int(bad.item()):
             self.close()
             raise ValueError(f"Non-positive Jacobian determinant in C3D10 element {int(bad.item()) - 1}")
         self.n = 3 * m["n_nodes"]
+
+    def attach_smoother(self, amg):
+        """Make ``amg``'s finest-level smoothing and defect products use this
+        operator (SmoothedAggregationAMG.attach_matrix_free)."""
+        amg.attach_matrix_free(self)
+        return amg
 
     def apply(self, u, y=None):
         import torch
@@ -405,6 +413,9 @@ class MatrixFreeOperator:
         return out[0] / out[1] if out[1] > 0 else 0.0
 
     def close(self):
+        for amg in list(getattr(self, "_attached", ())):
+            if getattr(amg, "handle", None):
+                amg.attach_matrix_free(None)
         if getattr(self, "handle", None):
             self.lib.wnfea_mf_destroy(self.handle)
             self.handle = None
