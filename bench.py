@@ -123,6 +123,68 @@ def spmv_only(system, info, reps: int, log) -> int:
     return 0
 
 
+def mf_bytes(n_elements: int, n_nodes: int, s: int) -> dict:
+    """Algorithmic bytes of the matrix-free kernels on an affine C3D10 mesh
+    with s-byte values: the element kernel reads the connectivity (40 B), the
+    quadrature data (10 values), E and nu (16 B) per element and every node's
+    value and scaling once (6 values), and writes 30 values per element; the
+    Chebyshev-step gather reads the incidence codes (40 B) and the 30 element
+    values per element, and per node its incidence pointer, Dirichlet byte
+    and five operands (b, D^-1, x, d, s), and writes two (x', d')."""
+    elem = n_elements * (40 + 10 * s + 16 + 30 * s) + n_nodes * 6 * s
+    gather = n_elements * (40 + 30 * s) + n_nodes * (4 + 1 + 15 * s + 6 * s)
+    return {"elem": elem, "gather_cheb": gather}
+
+
+def mf_only(system, model, reps: int, log) -> int:
+    """Matrix-free kernels alone on the resident I-beam: `reps` FP64 operator
+    applications (element kernel + gather), then `reps` FP32 V-cycles with the
+    finest level's smoothing and defect matrix-free and `reps` with them on
+    the assembled matrix, then 5 copies of a 256 MiB FP32 buffer to calibrate
+    the HBM counters."""
+    import torch
+    from wnfea_amd.solver.solid_solver import MatrixFreeOperator, build_amg
+    op = MatrixFreeOperator(system)
+    amg = build_amg(system, storage=("fp64", "fp32"))
+    u = torch.randn(system.n_dofs, dtype=torch.float64, device=system.K.device)
+    r = torch.randn(system.n_dofs, dtype=torch.float32, device=system.K.device)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        st.record()
+        for _ in range(reps):
+            fn()
+        en.record()
+        torch.cuda.synchronize()
+        return st.elapsed_time(en) / 1e3 / reps
+
+    y = torch.empty_like(u)
+    t_apply = timed(lambda: op.apply(u, y))
+    z = torch.empty_like(r)
+    t_asm = timed(lambda: amg.vcycle(r, z))
+    op.attach_smoother(amg)
+    t_mf = timed(lambda: amg.vcycle(r, z))
+    n = 64 * 2 ** 20
+    a = torch.ones(n, dtype=torch.float32, device=system.K.device)
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        c.copy_(a)
+    torch.cuda.synchronize()
+    ne, nn = system.mesh["n_elements"], system.n_nodes
+    print(json.dumps({"mf64_apply_us": t_apply * 1e6, "vcycle32_matrix_free_us": t_mf * 1e6,
+                      "vcycle32_assembled_us": t_asm * 1e6, "affine": op.affine,
+                      "bytes_fp32": mf_bytes(ne, nn, 4), "bytes_fp64": mf_bytes(ne, nn, 8),
+                      "copy_bytes_each_way": 4 * n}), flush=True)
+    op.close()
+    amg.close()
+    log("[bench] mf-only done")
+    return 0
+
+
 def run_batch(args, log) -> int:
     """64 seeded uneven cantilever boxes (nx in [10,60], ny = nz in [3,8],
     2k-60k DOF each) solved as one block-diagonal system to 1e-8 per mesh,
@@ -426,7 +488,8 @@ def main() -> int:
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--amg-precision", choices=["fp16", "fp32", "fp16_fine", "fp16_coarse", "fp16_smooth"],
                     default="fp32")
-    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel from the host")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay PCG iteration chunks as HIP graphs (default: host launches, measured faster here)")
     ap.add_argument("--operator", choices=["matrix_free", "assembled"], default="matrix_free",
                     help="CG's operator K: matrix-free (quadrature data) or the assembled FP64 BSR matrix")
     ap.add_argument("--smoother-operator", choices=["matrix_free", "assembled"], default="matrix_free",
@@ -434,6 +497,8 @@ def main() -> int:
     ap.add_argument("--spmv-shape", default=None,
                     help="waves,unroll,xcd_remap of the SpMV instead of autotuning, for both value precisions "
                          "(e.g. 4,1,0) or as fp32:fp64 (e.g. 2,2,1:4,1,1)")
+    ap.add_argument("--mf-only", action="store_true",
+                    help="time the matrix-free operator and V-cycle kernels alone (for counter runs)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV launches (and a 256 MiB copy for counter calibration), for rocprofv3 --pmc")
     ap.add_argument("--ir", action="store_true", help="also time the restart-IR (FP32 Krylov) scheme")
@@ -452,7 +517,7 @@ def main() -> int:
 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     torch.cuda.set_device(0)
-    set_graphs(not args.no_graphs)
+    set_graphs(args.graphs)
     if args.config == "batch":
         return run_batch(args, log)
     if args.config == "adjoint":
@@ -486,6 +551,8 @@ def main() -> int:
     tune = tunes["fp32"]
     if args.spmv_only:
         return spmv_only(system, info, args.spmv_reps, log)
+    if args.mf_only:
+        return mf_only(system, model, args.spmv_reps, log)
     t0 = time.perf_counter()
     amg = build_amg(system, waves=tune["waves"], unroll=tune["unroll"], xcd_remap=tune["xcd_remap"])
     apply_tune(amg, tunes)
@@ -579,8 +646,8 @@ def main() -> int:
             f"rel {relm:.2e}")
     alt_sm["speedup_mixed_vs_fp64"] = round(alt_sm["fp64"]["seconds"] / alt_sm["mixed"]["seconds"], 3)
     amg.attach_matrix_free(mf_op if args.smoother_operator == "matrix_free" else None)
-    # the same two solves with every kernel launched from the host
-    set_graphs(args.no_graphs)
+    # the same two solves with the other launch mode (graph replay or host launches)
+    set_graphs(not args.graphs)
     eager = {}
     for name, fn in (("mixed", mixed_solve), ("fp64", fp64_solve)):
         fn()
@@ -589,8 +656,8 @@ def main() -> int:
         fn()
         torch.cuda.synchronize()
         eager[name] = round(time.perf_counter() - t0, 5)
-    set_graphs(not args.no_graphs)
-    log(f"[bench] {'graphs' if args.no_graphs else 'host launches'}: {eager}")
+    set_graphs(args.graphs)
+    log(f"[bench] {'host launches' if args.graphs else 'graphs'}: {eager}")
     ir = None
     if args.ir:
         _, ir_outer, ir_inner, ir_rel, ir_t = ir_device(system, tol=1e-8, amg=amg, max_outer=30)
@@ -612,6 +679,23 @@ def main() -> int:
     b64 = spmv_bytes(nnzb, nb, 3, 8, 8)
     gbs32, gbs64 = b32 / t32 / 1e9, b64 / t64 / 1e9
     log(f"[bench] spmv fp32 {t32 * 1e6:.1f} us {gbs32:.0f} GB/s | fp64 {t64 * 1e6:.1f} us {gbs64:.0f} GB/s")
+    # the matrix-free FP64 operator (CG's product): element kernel + gather
+    y_mf = torch.empty_like(u)
+    for _ in range(3):
+        mf_op.apply(u, y_mf)
+    st_ev, en_ev = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    st_ev.record()
+    for _ in range(args.spmv_reps):
+        mf_op.apply(u, y_mf)
+    en_ev.record()
+    torch.cuda.synchronize()
+    t_mf = st_ev.elapsed_time(en_ev) / 1e3 / args.spmv_reps
+    ne_mf = system.mesh["n_elements"]
+    b_mf = mf_bytes(ne_mf, nb, 8)["elem"] + ne_mf * (40 + 30 * 8) + nb * (4 + 1 + 3 * 8)
+    mf_info = {"apply_fp64_us": round(t_mf * 1e6, 2), "algorithmic_bytes": b_mf,
+               "gbs": round(b_mf / t_mf / 1e9, 1), "affine": mf_op.affine}
+    log(f"[bench] matrix-free FP64 apply {t_mf * 1e6:.1f} us {b_mf / t_mf / 1e9:.0f} GB/s algorithmic")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
@@ -648,8 +732,8 @@ def main() -> int:
         "fp64_jacobi_pcg": jac,
         "restart_ir_fp32_krylov": ir,
         **others,
-        ("graph_replay_seconds" if args.no_graphs else "host_launch_seconds"): eager,
-        "graphs": not args.no_graphs,
+        ("host_launch_seconds" if args.graphs else "graph_replay_seconds"): eager,
+        "graphs": args.graphs,
         "speedup_mixed_vs_fp64_jacobi_pcg": round(jac["seconds"] / t_mixed, 2) if jac else None,
         "displacement_rel_err_mixed_vs_fp64": err_u,
         "spmv_gbs": {"fp32": round(gbs32, 1), "fp64": round(gbs64, 1), "fp32_us": round(t32 * 1e6, 2),
@@ -660,6 +744,7 @@ def main() -> int:
                      "kernel": f"sell_spmv_kernel<3,3,float,float,{tune['waves']},{tune['unroll']},OpAxpby> "
                                f"(FP32 3x3 BSR SpMV, {info['lanes']} lanes per block row)"},
         "spmv_tune": tunes,
+        "matrix_free_operator": mf_info,
         "setup": {"mesh_sec": round(t_mesh, 2), "rcm_sec": round(t_rcm, 2), "assembly_sec": round(t_asm, 2),
                   "amg_setup_sec": round(t_amg, 2), "amg": st},
         "cpu_baseline": cpu,

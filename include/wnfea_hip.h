@@ -69,6 +69,12 @@ int wnfea_mf_create(int M, int n_nodes, const int *conn, const double *nodes, co
                     void *stream);
 void wnfea_mf_destroy(void *handle);
 
+/* 1 when every element is affine (straight edges, midside nodes at the edge
+ * midpoints: J is constant, so one inverse Jacobian per element is kept, 80
+ * B instead of 320 B), 0 for a mesh with curved elements, < 0 on a bad
+ * handle.  No reference counterpart. */
+int wnfea_mf_affine(void *handle);
+
 /* y = K u, device FP64 vectors of 3 * n_nodes (matrix-free, deterministic). */
 int wnfea_mf_apply(void *handle, const double *u, double *y, void *stream);
 
@@ -305,9 +311,10 @@ int wnfea_amg_vcycle(void *h, int vt, const void *r, void *z, void *stream);
 void *wnfea_solver_create(void);
 void wnfea_solver_destroy(void *h);
 
-/* enable != 0 (default): full chunks of check_every PCG iterations are
- * captured once as a HIP graph and replayed (bitwise the same kernels as host
- * launches).  No reference counterpart (its loop launches every kernel). */
+/* enable != 0: full chunks of check_every PCG iterations are captured once
+ * as a HIP graph and replayed (bitwise the same kernels as host launches);
+ * 0 (default): every kernel is launched from the host.  No reference
+ * counterpart (its loop launches every kernel). */
 int wnfea_solver_set_graphs(void *h, int enable);
 
 /* PCG from x = 0 until ||r|| <= rtol ||b|| (recursive residual); prec as

@@ -74,7 +74,7 @@ def test_graph_replay_matches_host_launches(c0_system, c0_amg, amg_precision):
         x2, it2, rel2, _ = pcg_device(c0_system, check_every=8, **kw)  # replays the cached graph
         x3, it3, _, _ = pcg_device(c0_system, check_every=5, **kw)       # new capture, ragged last chunk
     finally:
-        set_graphs(True)
+        set_graphs(False)  # the default
     assert it0 == it1 == it2 == it3 and rel0 == rel1 == rel2
     assert torch.equal(x0, x1) and torch.equal(x1, x2) and torch.equal(x1, x3)
     # a capped run stops at exactly max_iter with and without graphs
@@ -84,7 +84,6 @@ def test_graph_replay_matches_host_launches(c0_system, c0_amg, amg_precision):
     xa, ita, _, _ = pcg_device(c0_system, check_every=4, **kw)
     set_graphs(False)
     xb, itb, _, _ = pcg_device(c0_system, check_every=4, **kw)
-    set_graphs(True)
     assert ita == itb == 13 and torch.equal(xa, xb)
 
 
@@ -431,3 +430,46 @@ def test_matrix_free_smoothing_cycle_and_solve(c0_system, c0_oracle):
         _, it2, _, _, _ = solve_mixed(c0_system, amg, tol=1e-8, amg_precision=prec)
         assert abs(its[prec] - it2) <= 2, (prec, its[prec], it2)
     amg.close()
+
+
+def test_matrix_free_operator_curved_and_affine_elements():
+    """Generated meshes are affine (one inverse Jacobian per element); moving
+    the midside nodes off the edge midpoints makes the elements curved (J
+    varies inside them) and the operator keeps data per Gauss point.  Both
+    apply the oracle's assembled K, and the matrix-free smoother on the curved
+    mesh still gives the mirror's cycle."""
+    import torch
+    from oracle import oracle as O
+    from vcycle_mirror import vcycle
+    from wnfea_amd.mesh.rcm import apply_rcm_to_model
+    from wnfea_amd.mesh.solid_generator import generate_c3d10_structured_block
+    from wnfea_amd.solver.solid_solver import MatrixFreeOperator, assemble_c3d10_bsr, build_amg
+    rng = np.random.default_rng(13)
+    for curved in (False, True):
+        m, _ = generate_c3d10_structured_block(0.4, 0.1, 0.1, 8, 2, 2, tip_load_total=-1000.0, tets_per_hex=6,
+                                               load_direction="z")
+        if curved:
+            nodes = np.array(m.mesh_nodes, dtype=np.float64)
+            mid = np.unique(np.asarray(m.solid_elements)[:, 4:].ravel())
+            nodes[mid] += 0.001 * rng.standard_normal((len(mid), 3))  # 2 % of the 0.05 m element edge
+            m.mesh_nodes = nodes
+        mr, _, _ = apply_rcm_to_model(m)
+        system = assemble_c3d10_bsr(mr, want_fp64_equilibrated=True)
+        op = MatrixFreeOperator(system)
+        assert op.affine == (not curved)
+        E, nu = mr.element_moduli()
+        K, _ = O.assemble_csr(mr.mesh_nodes, mr.solid_elements, E, nu, mr.fixed_dof_mask(3), mr.load_vector(3))
+        for _ in range(2):
+            u = rng.standard_normal(system.n_dofs)
+            y = op.apply(torch.as_tensor(u, device="cuda")).cpu().numpy()
+            assert _rel(y, K @ u) <= 1e-13, curved
+        amg = build_amg(system, storage=("fp64", "fp32"))
+        op.attach_smoother(amg)
+        r = rng.standard_normal(system.n_dofs)
+        z_ref = vcycle(amg.levels, amg.coarse_inv, r, deg=amg.level_degrees, ratio=amg.ratio)
+        z = amg.vcycle(torch.as_tensor(r, device="cuda")).cpu().numpy()
+        assert _rel(z, z_ref) <= 1e-12, curved
+        z32 = amg.vcycle(torch.as_tensor(r, device="cuda").float()).double().cpu().numpy()
+        assert _rel(z32, z_ref) <= 1e-4, curved
+        op.close()
+        amg.close()

@@ -72,6 +72,7 @@ SIGNATURES = {
     "wnfea_mf_residual": (_i, [_p, _p, _p, _p, _p, _pd, _p]),
     "wnfea_pcg_mf": (_i, [_p, _p, _i, _p, _p, _i, _p, _p, _p, _d, _i, _i, _pi, _pd, _p]),
     "wnfea_amg_attach_mf": (_i, [_p, _p, _p, _p]),
+    "wnfea_mf_affine": (_i, [_p]),
     "wnfea_pcg_multi": (_i, [_p, _p, _p, _i, _p, _i, _p, _p, _d, _i, _i, _pi, _pd, _p]),
     "wnfea_bsr_spmm": (_i, [_p, _i, _i, _p, _p, _p]),
 }

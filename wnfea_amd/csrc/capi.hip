@@ -485,6 +485,13 @@ void wnfea_mf_destroy(void* h) {
   delete m;
 }
 
+// 1 when the mesh is affine (one set of quadrature data per element), 0 when
+// it holds curved elements (data per Gauss point)
+int wnfea_mf_affine(void* h) {
+  if (!h) return kBadArg;
+  return static_cast<WnMf*>(h)->op.affine ? 1 : 0;
+}
+
 // y = K u (device FP64 vectors of 3 * n_nodes)
 int wnfea_mf_apply(void* h, const double* u, double* y, void* stream) {
   if (!h || !u || !y) return kBadArg;

@@ -366,7 +366,7 @@ __global__ void adjoint_dot_kernel(int M, const int* __restrict__ conn, const do
 // non-positive det J flags the element (bad = e + 1).
 __global__ __launch_bounds__(128) void c3d10_qdata_kernel(int M, const int* __restrict__ conn,
                                                           const double* __restrict__ nodes, double* __restrict__ qd,
-                                                          int* bad) {
+                                                          int* bad, int* curved) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= M) return;
   double X[10][3];
@@ -376,6 +376,8 @@ __global__ __launch_bounds__(128) void c3d10_qdata_kernel(int M, const int* __re
 #pragma unroll
     for (int c = 0; c < 3; ++c) X[i][c] = nodes[(size_t)n * 3 + c];
   }
+  double J0[9] = {};  // J at the first Gauss point
+  bool differs = false;
   for (int g = 0; g < 4; ++g) {
     double J[3][3];
 #pragma unroll
@@ -387,6 +389,15 @@ __global__ __launch_bounds__(128) void c3d10_qdata_kernel(int M, const int* __re
         for (int i = 0; i < 10; ++i) s = fma(c_dndxi[g][r][i], X[i][c], s);
         J[r][c] = s;
       }
+    // affine element: J equal at all points up to rounding
+    double jmax = 0.0, dmax = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      if (g == 0) J0[k] = J[k / 3][k % 3];
+      jmax = fmax(jmax, fabs(J0[k]));
+      dmax = fmax(dmax, fabs(J[k / 3][k % 3] - J0[k]));
+    }
+    differs = differs || dmax > 1e-12 * jmax;
     const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
     const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
     const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
@@ -402,36 +413,22 @@ __global__ __launch_bounds__(128) void c3d10_qdata_kernel(int M, const int* __re
     for (int k = 0; k < 9; ++k) qd[((size_t)k * M + e) * 4 + g] = inv[k];
     qd[((size_t)9 * M + e) * 4 + g] = det / 24.0;
   }
+  if (differs && curved) atomicOr(curved, 1);
 }
 
-// Natural derivatives dN_i/dxi at a Gauss point with barycentric coordinates
-// L (L[0] = 1 - xi - eta - zeta): dN_i/dxi_m = dN_i/dL_{m+1} - dN_i/dL_1,
-// corners N = L(2L - 1), midsides N = 4 L_p L_q in Abaqus order (edges 1-2,
-// 2-3, 3-1, 1-4, 2-4, 3-4).  i is a compile-time constant after unrolling.
-template <typename T>
-__device__ __forceinline__ void natural_grad(int i, const T (&L)[4], T& d0, T& d1, T& d2) {
-  T dl[4] = {T(0), T(0), T(0), T(0)};
-  switch (i) {
-    case 0: dl[0] = T(4) * L[0] - T(1); break;
-    case 1: dl[1] = T(4) * L[1] - T(1); break;
-    case 2: dl[2] = T(4) * L[2] - T(1); break;
-    case 3: dl[3] = T(4) * L[3] - T(1); break;
-    case 4: dl[0] = T(4) * L[1]; dl[1] = T(4) * L[0]; break;
-    case 5: dl[1] = T(4) * L[2]; dl[2] = T(4) * L[1]; break;
-    case 6: dl[0] = T(4) * L[2]; dl[2] = T(4) * L[0]; break;
-    case 7: dl[0] = T(4) * L[3]; dl[3] = T(4) * L[0]; break;
-    case 8: dl[1] = T(4) * L[3]; dl[3] = T(4) * L[1]; break;
-    default: dl[2] = T(4) * L[3]; dl[3] = T(4) * L[2]; break;
-  }
-  d0 = dl[1] - dl[0];
-  d1 = dl[2] - dl[0];
-  d2 = dl[3] - dl[0];
+// Once-per-element quadrature data of an affine mesh: the first Gauss
+// point's entries, qa[k*M + e] = qd[(k*M + e)*4].
+__global__ void qdata_affine_kernel(long long n, const double* __restrict__ qd, double* __restrict__ qa) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    qa[i] = qd[i * 4];
 }
 
-// v from the lane given by a quad permutation (DPP, within groups of 4 lanes)
+// v from the lane given by a quad permutation (DPP, within groups of 4
+// lanes).  The FP32 form takes 0 for the old value, which lets the compiler
+// fold the permutation into the consuming add (v_add_f32_dpp).
 template <int Ctrl>
 __device__ __forceinline__ float quad_perm(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), Ctrl, 0xF, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), Ctrl, 0xF, 0xF, false));
 }
 template <int Ctrl>
 __device__ __forceinline__ double quad_perm(double v) {
@@ -441,24 +438,38 @@ __device__ __forceinline__ double quad_perm(double v) {
   return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 // Sum over the 4 lanes of a quad; every lane gets the same bits,
-// (v0 + v1) + (v2 + v3) up to commutation.
+// (v0 + v1) + (v2 + v3) up to commutation.  Plain adds (no contraction with
+// the product that made v), so each step is one v_add_f32_dpp in FP32.
 template <typename T>
 __device__ __forceinline__ T quad_sum(T v) {
-  v += quad_perm<0xB1>(v);  // lanes (1, 0, 3, 2): xor 1
-  v += quad_perm<0x4E>(v);  // lanes (2, 3, 0, 1): xor 2
+#pragma clang fp contract(off)
+  v = quad_perm<0xB1>(v) + v;  // lanes (1, 0, 3, 2): xor 1
+  v = quad_perm<0x4E>(v) + v;  // lanes (2, 3, 0, 1): xor 2
   return v;
 }
+
+// The 10-node tetrahedron in barycentric derivatives: node i's shape function
+// depends on L_k only through corner k (N = L_k (2 L_k - 1), dN/dL_k =
+// 4 L_k - 1) or through the two ends (p, q) of its edge (N = 4 L_p L_q,
+// dN/dL_p = 4 L_q), and d/dxi_a = d/dL_{a+1} - d/dL_0.  Midside nodes 4..9
+// sit on the edges 1-2, 2-3, 3-1, 1-4, 2-4, 3-4 (Abaqus order).
+__device__ constexpr int kEdgeP[6] = {0, 1, 2, 0, 1, 2};
+__device__ constexpr int kEdgeQ[6] = {1, 2, 0, 3, 3, 3};
 
 // fe = K_e (sc .* u)_e from the quadrature data, stored [i][e][3] for local
 // node i, where sc is a per-DOF scaling that also carries the Dirichlet rule
 // (0 on a fixed DOF: its column of the operator is zero).  Four lanes per
-// element, lane g for Gauss point g: the strain eps_g = sum_i B_gi u_i from
-// the preloaded J_g^-1, T_g = (det J_g / 24) S_g J_g^-1 with S_g = D eps_g,
-// then each node's contribution T_g dN_i/dxi(g) summed over the quad, lane g
-// storing the nodes i = g (mod 4).  The four-fold split keeps enough waves in
-// flight to hide the latency of the connectivity and nodal-value loads.
-// T is the type of the quadrature data, the vectors and the arithmetic.
-template <typename T>
+// element, lane g for Gauss point g.  Strain: Q_k = sum_i dN_i/dL_k u_i (only
+// the nonzero derivatives are visited), the natural displacement gradient
+// G_a = Q_{a+1} - Q_0 and H = J^-1 G, H[r][c] = du_c/dx_r.  Stress S = D eps,
+// T = (det J / 24) S J^-1, and node i's force T dN_i/dxi = sum_k R_k dN_i/dL_k
+// with R_k = T e_{k-1} (k >= 1), R_0 = -(R_1 + R_2 + R_3); the four points'
+// forces are summed over the quad and lane g stores the nodes i = g (mod 4).
+// The four-fold split keeps enough waves in flight to hide the latency of the
+// connectivity and nodal-value loads.  T is the type of the quadrature data,
+// the vectors and the arithmetic; Affine: one set of quadrature data per
+// element instead of one per Gauss point.
+template <typename T, bool Affine>
 __global__ __launch_bounds__(256) void c3d10_pa_elem_kernel(int M, const int* __restrict__ conn,
                                                             const T* __restrict__ qd,
                                                             const double* __restrict__ E,
@@ -467,67 +478,95 @@ __global__ __launch_bounds__(256) void c3d10_pa_elem_kernel(int M, const int* __
                                                             const T* __restrict__ U, T* __restrict__ fe,
                                                             const int* flags, int guard) {
   if (guard && flags[0]) return;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // XCD-contiguous element ranges (the grid is padded to a multiple of 8)
+  const int nblk = (int)((4LL * M + 255) / 256);
+  const int lb = slice_of_block(nblk, 1);
+  if (lb >= nblk) return;
+  const long long t = (long long)lb * blockDim.x + threadIdx.x;
   const int e = (int)(t >> 2), g = (int)(t & 3);
   if (e >= M) return;  // whole quads leave together
   // barycentric coordinates of Gauss point g: (a, a, a) and its permutations with one b
   const double ga = (5.0 - sqrt(5.0)) / 20.0, gb = (5.0 + 3.0 * sqrt(5.0)) / 20.0;
   const double p0 = g == 1 ? gb : ga, p1 = g == 2 ? gb : ga, p2 = g == 3 ? gb : ga;
   const T L[4] = {T(1.0 - p0 - p1 - p2), T(p0), T(p1), T(p2)};
+  T c4[4], cc[4];  // 4 L_k and 4 L_k - 1
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    c4[k] = T(4) * L[k];
+    cc[k] = c4[k] - T(1);
+  }
+  // quadrature data of this point (affine: of the element, shared by its 4 lanes)
   T Ji[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) Ji[k] = qd[((size_t)k * M + e) * 4 + g];
-  const T wd = qd[((size_t)9 * M + e) * 4 + g];
-  T eps[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  for (int k = 0; k < 9; ++k) Ji[k] = Affine ? qd[(size_t)k * M + e] : qd[((size_t)k * M + e) * 4 + g];
+  const T wd = Affine ? qd[(size_t)9 * M + e] : qd[((size_t)9 * M + e) * 4 + g];
   int nn[10];
 #pragma unroll
   for (int i = 0; i < 10; ++i) nn[i] = conn[(size_t)e * 10 + i];
+  T Q[4][3];
 #pragma unroll
   for (int i = 0; i < 10; ++i) {
-    const int n = nn[i];
-    const T ux = U[(size_t)n * 3 + 0] * sc[(size_t)n * 3 + 0];
-    const T uy = U[(size_t)n * 3 + 1] * sc[(size_t)n * 3 + 1];
-    const T uz = U[(size_t)n * 3 + 2] * sc[(size_t)n * 3 + 2];
-    T d0, d1, d2;
-    natural_grad<T>(i, L, d0, d1, d2);
-    const T gx = Ji[0] * d0 + Ji[1] * d1 + Ji[2] * d2;
-    const T gy = Ji[3] * d0 + Ji[4] * d1 + Ji[5] * d2;
-    const T gz = Ji[6] * d0 + Ji[7] * d1 + Ji[8] * d2;
-    eps[0] += gx * ux;
-    eps[1] += gy * uy;
-    eps[2] += gz * uz;
-    eps[3] += gy * ux + gx * uy;
-    eps[4] += gz * uy + gy * uz;
-    eps[5] += gz * ux + gx * uz;
+    const int n3 = 3 * nn[i];
+    T u[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[c] = U[n3 + c] * sc[n3 + c];
+    if (i < 4) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Q[i][c] = cc[i] * u[c];  // corners come first: initialise
+    } else {
+      const int p = kEdgeP[i - 4], q = kEdgeQ[i - 4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Q[p][c] += c4[q] * u[c];
+        Q[q][c] += c4[p] * u[c];
+      }
+    }
+  }
+  T H[3][3];  // H = J^-1 G,  G[a][c] = Q[a+1][c] - Q[0][c]
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T g0 = Q[1][c] - Q[0][c], g1 = Q[2][c] - Q[0][c], g2 = Q[3][c] - Q[0][c];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) H[r][c] = Ji[3 * r] * g0 + Ji[3 * r + 1] * g1 + Ji[3 * r + 2] * g2;
   }
   const double Ev = E[e], nv = nu[e];
   const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
   const T c11 = T(f * (1.0 - nv)), c12 = T(f * nv), c44 = T(Ev / (2.0 * (1.0 + nv)));
-  const T s0 = wd * (c11 * eps[0] + c12 * (eps[1] + eps[2]));
-  const T s1 = wd * (c11 * eps[1] + c12 * (eps[0] + eps[2]));
-  const T s2 = wd * (c11 * eps[2] + c12 * (eps[0] + eps[1]));
-  const T s3 = wd * c44 * eps[3], s4 = wd * c44 * eps[4], s5 = wd * c44 * eps[5];
-  // S = [[s0 s3 s5] [s3 s1 s4] [s5 s4 s2]];  Tg = S J^-1 (J^-1 rows in Ji)
-  T Tg[9];
+  const T e0 = H[0][0], e1 = H[1][1], e2 = H[2][2];
+  const T s0 = wd * (c11 * e0 + c12 * (e1 + e2));
+  const T s1 = wd * (c11 * e1 + c12 * (e0 + e2));
+  const T s2 = wd * (c11 * e2 + c12 * (e0 + e1));
+  const T s3 = wd * c44 * (H[1][0] + H[0][1]), s4 = wd * c44 * (H[2][1] + H[1][2]),
+          s5 = wd * c44 * (H[2][0] + H[0][2]);
+  // S = [[s0 s3 s5] [s3 s1 s4] [s5 s4 s2]];  R_{k+1} = column k of S J^-1, R_0 = -(R_1 + R_2 + R_3)
+  T R[4][3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    Tg[0 + c] = s0 * Ji[c] + s3 * Ji[3 + c] + s5 * Ji[6 + c];
-    Tg[3 + c] = s3 * Ji[c] + s1 * Ji[3 + c] + s4 * Ji[6 + c];
-    Tg[6 + c] = s5 * Ji[c] + s4 * Ji[3 + c] + s2 * Ji[6 + c];
+  for (int k = 0; k < 3; ++k) {
+    R[k + 1][0] = s0 * Ji[k] + s3 * Ji[3 + k] + s5 * Ji[6 + k];
+    R[k + 1][1] = s3 * Ji[k] + s1 * Ji[3 + k] + s4 * Ji[6 + k];
+    R[k + 1][2] = s5 * Ji[k] + s4 * Ji[3 + k] + s2 * Ji[6 + k];
   }
 #pragma unroll
+  for (int c = 0; c < 3; ++c) R[0][c] = -(R[1][c] + R[2][c] + R[3][c]);
+#pragma unroll
   for (int i = 0; i < 10; ++i) {
-    T d0, d1, d2;
-    natural_grad<T>(i, L, d0, d1, d2);
-    const T fx = quad_sum(Tg[0] * d0 + Tg[1] * d1 + Tg[2] * d2);
-    const T fy = quad_sum(Tg[3] * d0 + Tg[4] * d1 + Tg[5] * d2);
-    const T fz = quad_sum(Tg[6] * d0 + Tg[7] * d1 + Tg[8] * d2);
+    T fi[3];
+    if (i < 4) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fi[c] = R[i][c] * cc[i];
+    } else {
+      const int p = kEdgeP[i - 4], q = kEdgeQ[i - 4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fi[c] = R[p][c] * c4[q] + R[q][c] * c4[p];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fi[c] = quad_sum(fi[c]);
     if ((i & 3) == g) {
       // layout [local node][element][3]
       T* o = fe + ((size_t)i * M + e) * 3;
-      o[0] = fx;
-      o[1] = fy;
-      o[2] = fz;
+      o[0] = fi[0];
+      o[1] = fi[1];
+      o[2] = fi[2];
     }
   }
 }
@@ -556,8 +595,12 @@ __global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_kernel(
   if (guard && flags[0]) return;
   __shared__ double sv[kGatherWaves][kChunkInc][3];
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-  const int c = blockIdx.x * kGatherWaves + w;
-  const bool live = c < n_chunks;
+  // XCD-contiguous chunk ranges: neighbouring chunks read the same element
+  // contribution lines, which then meet in one L2
+  const int nb = (n_chunks + kGatherWaves - 1) / kGatherWaves;
+  const int lb = slice_of_block(nb, 1);
+  const int c = lb * kGatherWaves + w;
+  const bool live = lb < nb && c < n_chunks;
   // chunk c: nodes [a.x, b.x), incidences [a.y, b.y)
   const int2 a = live ? chunk[c] : make_int2(0, 0);
   const int2 b = live ? chunk[c + 1] : make_int2(0, 0);
@@ -647,8 +690,12 @@ __global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_op_kernel(
   if (skip && *skip) return;
   __shared__ T sv[kGatherWaves][kChunkInc][3];
   const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-  const int c = blockIdx.x * kGatherWaves + w;
-  const bool live = c < n_chunks;
+  // XCD-contiguous chunk ranges: neighbouring chunks read the same element
+  // contribution lines, which then meet in one L2
+  const int nb = (n_chunks + kGatherWaves - 1) / kGatherWaves;
+  const int lb = slice_of_block(nb, 1);
+  const int c = lb * kGatherWaves + w;
+  const bool live = lb < nb && c < n_chunks;
   const int2 a = live ? chunk[c] : make_int2(0, 0);
   const int2 b = live ? chunk[c + 1] : make_int2(0, 0);
   const int k0 = a.y, k1 = b.y;
@@ -793,8 +840,26 @@ int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s) {
   WN_HIP(hipMalloc(&op.qd, sizeof(double) * 40 * (size_t)op.M));
   WN_HIP(hipMalloc(&op.fe, sizeof(double) * 30 * (size_t)op.M));
   WN_HIP(hipMalloc(&op.mask, sizeof(double) * n));
-  hipLaunchKernelGGL(c3d10_qdata_kernel, dim3(ceil_div(op.M, 128)), dim3(128), 0, s, op.M, op.conn, nodes, op.qd, bad);
+  int* curved = nullptr;
+  WN_HIP(hipMalloc(&curved, sizeof(int)));
+  WN_HIP(hipMemsetAsync(curved, 0, sizeof(int), s));
+  hipLaunchKernelGGL(c3d10_qdata_kernel, dim3(ceil_div(op.M, 128)), dim3(128), 0, s, op.M, op.conn, nodes, op.qd, bad,
+                     curved);
   hipLaunchKernelGGL(dof_scale_kernel<double>, dim3(vec_grid(n)), dim3(256), 0, s, n, nullptr, op.fix, op.mask);
+  int h_curved = 1;
+  const hipError_t ce = hipMemcpyAsync(&h_curved, curved, sizeof(int), hipMemcpyDeviceToHost, s);
+  const hipError_t se = ce == hipSuccess ? hipStreamSynchronize(s) : ce;
+  (void)hipFree(curved);
+  WN_HIP(se);
+  if (!h_curved) {  // affine mesh: keep one set of quadrature data per element
+    double* qa = nullptr;
+    WN_HIP(hipMalloc(&qa, sizeof(double) * 10 * (size_t)op.M));
+    hipLaunchKernelGGL(qdata_affine_kernel, dim3(vec_grid(10LL * op.M)), dim3(256), 0, s, 10LL * op.M, op.qd, qa);
+    WN_HIP(hipStreamSynchronize(s));
+    (void)hipFree(op.qd);
+    op.qd = qa;
+    op.affine = 1;
+  }
   return static_cast<int>(hipGetLastError());
 }
 
@@ -802,19 +867,26 @@ void mf_release(MfOp& op) {
   for (void* p : {(void*)op.qd, (void*)op.fe, (void*)op.mask, (void*)op.chunk})
     if (p) (void)hipFree(p);
   op.qd = op.fe = op.mask = nullptr;
+  op.affine = 0;
   op.chunk = nullptr;
   op.n_chunks = 0;
 }
 
-int mf_grid(const MfOp& op) { return std::max(1, ceil_div(op.n_chunks, kGatherWaves)); }
+// gather grid: chunk blocks padded to a multiple of the 8 XCDs
+int mf_grid(const MfOp& op) { return 8 * ceil_div(std::max(1, ceil_div(op.n_chunks, kGatherWaves)), 8); }
+static int elem_grid(int M) { return 8 * ceil_div(ceil_div(4L * M, 256), 8); }
 
 int mf_apply(const MfOp& op, const double* u, const double* f, double* y, double* partials, int mode,
              const int* flags, int guard, hipStream_t s) {
   if (!op.chunk) return kNotReady;
   if (op.M > 0) {
     if (!op.qd || !op.fe || !op.mask) return kNotReady;
-    hipLaunchKernelGGL(c3d10_pa_elem_kernel<double>, dim3(ceil_div(4L * op.M, 256)), dim3(256), 0, s, op.M, op.conn,
-                       op.qd, op.E, op.nu, op.mask, u, op.fe, flags, guard);
+    if (op.affine)
+      hipLaunchKernelGGL((c3d10_pa_elem_kernel<double, true>), dim3(elem_grid(op.M)), dim3(256), 0, s, op.M,
+                         op.conn, op.qd, op.E, op.nu, op.mask, u, op.fe, flags, guard);
+    else
+      hipLaunchKernelGGL((c3d10_pa_elem_kernel<double, false>), dim3(elem_grid(op.M)), dim3(256), 0, s, op.M,
+                         op.conn, op.qd, op.E, op.nu, op.mask, u, op.fe, flags, guard);
   }
   hipLaunchKernelGGL(mf_gather_kernel, dim3(mf_grid(op)), dim3(64 * kGatherWaves), 0, s, op.n_chunks, op.M, op.chunk,
                      op.n2e_ptr, op.n2e, op.fe, op.fix, u, f, y, partials, mode, flags, guard);
@@ -864,13 +936,13 @@ int mf_smooth_attach(MfSmooth& m, const MfOp* op, const double* scale, int want3
   if (want32) {
     WN_HIP(hipMalloc(&m.s32, sizeof(float) * nv));
     WN_HIP(hipMalloc(&m.s32e, sizeof(float) * nv));
-    WN_HIP(hipMalloc(&m.qd32, sizeof(float) * 40 * std::max<size_t>((size_t)op->M, 1)));
+    const long long nq = (op->affine ? 10LL : 40LL) * op->M;
+    WN_HIP(hipMalloc(&m.qd32, sizeof(float) * std::max<long long>(nq, 1)));
     WN_HIP(hipMalloc(&m.fe32, sizeof(float) * 30 * std::max<size_t>((size_t)op->M, 1)));
     hipLaunchKernelGGL(round_to_float_kernel, dim3(vec_grid(n)), dim3(256), 0, s, n, scale, m.s32);
     hipLaunchKernelGGL(dof_scale_kernel<float>, dim3(vec_grid(n)), dim3(256), 0, s, n, scale, op->fix, m.s32e);
     if (op->M > 0)
-      hipLaunchKernelGGL(round_to_float_kernel, dim3(vec_grid(40LL * op->M)), dim3(256), 0, s, 40LL * op->M, op->qd,
-                         m.qd32);
+      hipLaunchKernelGGL(round_to_float_kernel, dim3(vec_grid(nq)), dim3(256), 0, s, nq, op->qd, m.qd32);
   }
   WN_HIP(hipGetLastError());
   m.op = op;
@@ -900,9 +972,12 @@ static int mf_smooth_run(const MfSmooth& m, const T* x, const Op& o, const int* 
   if (!mf_smooth_ready<T>(m)) return kNotReady;
   const MfOp& op = *m.op;
   const T* sc = mf_scale<T>(m);
-  if (op.M > 0)
-    hipLaunchKernelGGL(c3d10_pa_elem_kernel<T>, dim3(ceil_div(4L * op.M, 256)), dim3(256), 0, s, op.M, op.conn,
-                       mf_qd<T>(m), op.E, op.nu, mf_scale_in<T>(m), x, mf_fe<T>(m), skip, skip ? 1 : 0);
+  if (op.M > 0 && op.affine)
+    hipLaunchKernelGGL((c3d10_pa_elem_kernel<T, true>), dim3(elem_grid(op.M)), dim3(256), 0, s, op.M,
+                       op.conn, mf_qd<T>(m), op.E, op.nu, mf_scale_in<T>(m), x, mf_fe<T>(m), skip, skip ? 1 : 0);
+  else if (op.M > 0)
+    hipLaunchKernelGGL((c3d10_pa_elem_kernel<T, false>), dim3(elem_grid(op.M)), dim3(256), 0, s, op.M,
+                       op.conn, mf_qd<T>(m), op.E, op.nu, mf_scale_in<T>(m), x, mf_fe<T>(m), skip, skip ? 1 : 0);
   hipLaunchKernelGGL((mf_gather_op_kernel<T, Op>), dim3(mf_grid(op)), dim3(64 * kGatherWaves), 0, s, op.n_chunks,
                      op.M, op.chunk, op.n2e_ptr, op.n2e, mf_fe<T>(m), op.fix, sc, x, o, skip);
   return static_cast<int>(hipGetLastError());

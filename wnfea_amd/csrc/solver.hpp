@@ -144,7 +144,7 @@ struct Solver {
   // Replayable chunk of PCG iterations (HIP graph) and the key of everything
   // its kernel arguments were taken from: a solve with the same key replays
   // it, any other solve captures a new one.
-  int use_graphs = 1;
+  int use_graphs = 0;
   // multi-right-hand-side workspace (multi.hip); freed through multi_free
   void* multi = nullptr;
   void (*multi_free)(void*) = nullptr;

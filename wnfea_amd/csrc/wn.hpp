@@ -88,7 +88,11 @@ struct MfOp {
   const unsigned char* fix = nullptr;  // per node, bit c: DOF c fixed (may be null)
   const int* n2e_ptr = nullptr;
   const int* n2e = nullptr;
-  double* qd = nullptr;    // [10][M][4] inverse Jacobian entries and det/24 per Gauss point
+  // inverse Jacobian entries and det/24: [10][M][4] per Gauss point, or
+  // [10][M] once per element when every element is affine (straight edges,
+  // midside nodes at the edge midpoints: J is then the same at every point)
+  double* qd = nullptr;
+  int affine = 0;
   double* fe = nullptr;    // [10][M][3] element contributions per local node
   double* mask = nullptr;  // [3 n_nodes] 0 for a fixed DOF, else 1
   int2* chunk = nullptr;  // [n_chunks+1] {first node, first incidence} of each gather chunk
@@ -105,7 +109,7 @@ struct MfSmooth {
   double* s64e = nullptr;       // s with fixed DOFs zeroed (the element kernel's input scaling)
   float* s32 = nullptr;
   float* s32e = nullptr;
-  float* qd32 = nullptr;  // [10][M][4]
+  float* qd32 = nullptr;  // rounded copy of the operator's qd (same layout)
   float* fe32 = nullptr;  // [10][M][3]
 };
 int mf_smooth_attach(MfSmooth& m, const MfOp* op, const double* scale, int want32, hipStream_t s);

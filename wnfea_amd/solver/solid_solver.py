@@ -291,14 +291,15 @@ class _Solver:
             cls._h = L.wnfea_solver_create()
             if not cls._h:
                 raise _lib.HipError("wnfea_solver_create failed")
-            _lib.check(L.wnfea_solver_set_graphs(cls._h, int(os.environ.get("WNFEA_GRAPHS", "1") != "0")),
+            _lib.check(L.wnfea_solver_set_graphs(cls._h, int(os.environ.get("WNFEA_GRAPHS", "0") != "0")),
                        "wnfea_solver_set_graphs")
         return cls._h
 
 
 def set_graphs(enable: bool) -> None:
-    """Run full chunks of PCG iterations as a captured HIP graph (default) or
-    launch every kernel from the host."""
+    """Run full chunks of PCG iterations as a captured HIP graph, or launch
+    every kernel from the host (default: on the I-beam solve host launches
+    are about 3 % faster than graph replay, the queue never runs dry)."""
     L = _lib.load()
     _lib.check(L.wnfea_solver_set_graphs(_Solver.get(), int(bool(enable))), "wnfea_solver_set_graphs")
 
@@ -388,6 +389,15 @@ class MatrixFreeOperator:
             self.close()
             raise ValueError(f"Non-positive Jacobian determinant in C3D10 element {int(bad.item()) - 1}")
         self.n = 3 * m["n_nodes"]
+
+    @property
+    def affine(self) -> bool:
+        """True when every element is affine and one inverse Jacobian per
+        element is stored (else one per Gauss point)."""
+        r = self.lib.wnfea_mf_affine(self.handle)
+        if r < 0:
+            _lib.check(r, "wnfea_mf_affine")
+        return r == 1
 
     def attach_smoother(self, amg):
         """Make ``amg``'s finest-level smoothing and defect products use this
