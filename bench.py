@@ -210,23 +210,29 @@ def run_batch(args, log) -> int:
     t_batch = (time.perf_counter() - t0) / args.steps
     log(f"[bench] batched solve {t_batch * 1e3:.1f} ms, {iters} iterations, max per-mesh rel {per.max():.2e}")
     bs.close()
-    # the same meshes one by one (each its own system and hierarchy); solve time only
+    # the same meshes one by one (each its own system, hierarchy and matrix-free
+    # operator, as in the batch); solve time only
+    from wnfea_amd.solver.solid_solver import MatrixFreeOperator
     systems = []
     for m in models:
         mr, _, _ = apply_rcm_to_model(m)
         s_ = assemble_c3d10_bsr(mr)
-        systems.append((s_, build_amg(s_, storage=("fp32",))))
-    for s_, a_ in systems:
-        solve_mixed(s_, a_, tol=1e-8)
+        a_ = build_amg(s_, storage=("fp32",))
+        op_ = MatrixFreeOperator(s_)
+        op_.attach_smoother(a_)
+        systems.append((s_, a_, op_))
+    for s_, a_, op_ in systems:
+        solve_mixed(s_, a_, tol=1e-8, operator=op_)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     seq_iters = []
-    for s_, a_ in systems:
-        seq_iters.append(solve_mixed(s_, a_, tol=1e-8)[1])
+    for s_, a_, op_ in systems:
+        seq_iters.append(solve_mixed(s_, a_, tol=1e-8, operator=op_)[1])
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
-    for _, a_ in systems:
+    for _, a_, op_ in systems:
         a_.close()
+        op_.close()
     log(f"[bench] one-by-one solves {t_seq * 1e3:.1f} ms (iterations {min(seq_iters)}..{max(seq_iters)})")
     print(json.dumps({
         "metric": "s to rel-res 1e-8 per mesh, 64 uneven C3D10 boxes in one system",

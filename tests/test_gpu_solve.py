@@ -201,7 +201,8 @@ def test_gmsh_box_solves_like_generated_box(tmp_path):
     np.testing.assert_allclose(u_back, ug.reshape(-1, 3)[order], rtol=0, atol=1e-8 * np.abs(ug).max())
 
 
-def test_many_mesh_batch_matches_oracle():
+@pytest.mark.parametrize("operator", ["matrix_free", "assembled"])
+def test_many_mesh_batch_matches_oracle(operator):
     """Six uneven boxes solved as one block-diagonal system: every mesh meets
     1e-8 on its own FP64 residual and matches its own exact solution."""
     from scipy.sparse.linalg import spsolve
@@ -209,7 +210,7 @@ def test_many_mesh_batch_matches_oracle():
     from wnfea_amd.solver.batch import BatchedSolids, generate_box_batch
     batch = generate_box_batch(6, seed=1, dof_range=(2000, 12000))
     models = [b[0] for b in batch]
-    bs = BatchedSolids(models)
+    bs = BatchedSolids(models, operator=operator)
     try:
         u, iters, per, _ = bs.solve(tol=1e-8)
         assert np.all(per <= 1e-8)

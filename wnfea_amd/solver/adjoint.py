@@ -17,7 +17,10 @@ sensitivities; this follows its element and solver conventions.
 
 ``C3D10Adjoint`` keeps one assembled system and one SA-AMG hierarchy; new
 moduli are assembled in place into the same pattern, and the hierarchy is
-rebuilt only when the moduli moved by more than ``rebuild_rel``.
+rebuilt only when the moduli moved by more than ``rebuild_rel``.  With
+``operator="matrix_free"`` (default) the solves run CG on the matrix-free
+operator and the V-cycle's finest smoothing steps on it, as the fastest
+forward solve does.
 ``compliance(E)`` and ``tip_displacement(E)`` are torch.autograd functions
 of the modulus tensor.
 """
@@ -62,7 +65,7 @@ class C3D10Adjoint:
     """Forward solves and adjoint gradients of one C3D10 model on the device."""
 
     def __init__(self, model: FEAModel, tol: float = 1e-10, amg_precision: str = "fp32", rebuild_rel: float = 0.05,
-                 amg_kw: dict | None = None):
+                 amg_kw: dict | None = None, operator: str = "matrix_free"):
         import torch
         from .solid_solver import assemble_c3d10_bsr
         self.model = model
@@ -76,11 +79,24 @@ class C3D10Adjoint:
         self.E_now = self.system.mesh["E"].clone()
         self.solves = 0
         self._torch = torch
+        if operator not in ("matrix_free", "assembled"):
+            raise ValueError(f"unknown operator {operator!r}")
+        self.op = None
+        if operator == "matrix_free":
+            from .solid_solver import MatrixFreeOperator
+            self.op = MatrixFreeOperator(self.system)  # reads the moduli in place
 
-    def close(self):
+    def _drop_amg(self):
         if self.amg is not None:
             self.amg.close()
             self.amg = None
+
+    def close(self):
+        """Free the hierarchy and the matrix-free operator."""
+        self._drop_amg()
+        if self.op is not None:
+            self.op.close()
+            self.op = None
 
     # ------------------------------------------------------------ forward
     def set_moduli(self, E):
@@ -89,14 +105,18 @@ class C3D10Adjoint:
         from .solid_solver import build_amg, reassemble_c3d10_values
         torch = self._torch
         E = torch.as_tensor(E, dtype=torch.float64, device=self.system.K.device).reshape(-1).contiguous()
-        if not torch.equal(E, self.E_now):
+        changed = not torch.equal(E, self.E_now)
+        if changed:
             reassemble_c3d10_values(self.system, E=E)
             self.E_now = E.clone()
         if self.amg is None or float(((E - self.E_amg).abs() / self.E_amg).max()) > self.rebuild_rel:
-            self.close()
+            self._drop_amg()
             storage = {"fp64": ("fp64",), "fp32": ("fp32",)}.get(self.amg_precision, ("fp32", "fp16"))
             self.amg = build_amg(self.system, storage=storage, **self.amg_kw)
             self.E_amg = E.clone()
+            changed = True
+        if changed and self.op is not None:
+            self.op.attach_smoother(self.amg)  # the scaling moved with E
 
     def solve(self, b=None):
         """u = K^-1 b (b = the load vector by default), FP64-verified to tol."""
@@ -105,13 +125,13 @@ class C3D10Adjoint:
         sysm = self.system
         if b is None:
             u, _, _, rel, _ = solve_mixed(sysm, self.amg, tol=self.tol, amg_precision=self.amg_precision,
-                                          max_restarts=6)
+                                          max_restarts=6, operator=self.op)
         else:
             saved = sysm.F
             try:
                 sysm.F = torch.as_tensor(b, dtype=torch.float64, device=sysm.K.device).contiguous()
                 u, _, _, rel, _ = solve_mixed(sysm, self.amg, tol=self.tol, amg_precision=self.amg_precision,
-                                              max_restarts=6)
+                                              max_restarts=6, operator=self.op)
             finally:
                 sysm.F = saved
         self.solves += 1

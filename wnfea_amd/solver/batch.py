@@ -81,9 +81,10 @@ def merge_models(models: list[FEAModel]) -> tuple[FEAModel, np.ndarray]:
 class BatchedSolids:
     """Assemble, precondition and solve a list of C3D10 models as one system."""
 
-    def __init__(self, models: list[FEAModel], amg_precision: str = "fp32", rcm: bool = True, amg_kw=None):
+    def __init__(self, models: list[FEAModel], amg_precision: str = "fp32", rcm: bool = True, amg_kw=None,
+                 operator: str = "matrix_free"):
         from ..mesh.rcm import apply_rcm_to_model
-        from .solid_solver import assemble_c3d10_bsr, build_amg
+        from .solid_solver import MatrixFreeOperator, assemble_c3d10_bsr, build_amg
         t0 = time.perf_counter()
         merged, offs = merge_models(models)
         self.n2o = None
@@ -98,6 +99,13 @@ class BatchedSolids:
         storage = ("fp64",) if amg_precision == "fp64" else ("fp32",)
         self.amg = build_amg(self.system, storage=storage, **dict(amg_kw or {}))
         self.amg_precision = amg_precision
+        # CG and the finest smoothing steps on the matrix-free operator (default)
+        self.op = None
+        if operator == "matrix_free":
+            self.op = MatrixFreeOperator(self.system)
+            self.op.attach_smoother(self.amg)
+        elif operator != "assembled":
+            raise ValueError(f"unknown operator {operator!r}")
         self.timings = {"assembly_sec": t1 - t0, "amg_setup_sec": time.perf_counter() - t1}
         import torch
         dev = self.system.K.device
@@ -109,6 +117,9 @@ class BatchedSolids:
 
     def close(self):
         self.amg.close()
+        if self.op is not None:
+            self.op.close()
+            self.op = None
 
     def per_mesh_residuals(self, u):
         """||f_k - K_k u_k|| / ||f_k|| for every mesh (device FP64 residual)."""
@@ -128,7 +139,7 @@ class BatchedSolids:
         fall = float(torch.linalg.norm(self.system.F))
         tol_global = tol * fmin / fall if fall > 0 else tol
         u, iters, runs, rel, _ = solve_mixed(self.system, self.amg, tol=tol_global,
-                                             amg_precision=self.amg_precision, max_restarts=4)
+                                             amg_precision=self.amg_precision, max_restarts=4, operator=self.op)
         per = self.per_mesh_residuals(u)
         torch.cuda.synchronize()
         return u, iters, per.cpu().numpy(), time.perf_counter() - t0

@@ -204,10 +204,11 @@ def reassemble_c3d10_values(system: SolidSystem, E=None, nu=None) -> SolidSystem
     M = mesh["n_elements"]
     for key, val in (("E", E), ("nu", nu)):
         if val is not None:
-            t = torch.as_tensor(val, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+            t = torch.as_tensor(val, dtype=torch.float64, device=dev).reshape(-1)
             if t.numel() != M:
                 raise ValueError(f"{key} must have one value per element ({M})")
-            mesh[key] = t
+            # in place: matrix-free operators of this system read the same buffer
+            mesh[key].copy_(t)
     stream = _lib.stream_of()
     Ke = torch.empty((M, 30, 30), dtype=torch.float64, device=dev)
     bad = torch.zeros(1, dtype=torch.int32, device=dev)
@@ -384,6 +385,8 @@ class MatrixFreeOperator:
                                             _lib.dptr(m["n2e_ptr"]), _lib.dptr(m["n2e"]), _lib.dptr(bad),
                                             _lib.ctypes.byref(h), _lib.stream_of()), "wnfea_mf_create")
         self.handle = h.value
+        # the library borrows these device buffers: keep them alive with the handle
+        self._borrowed = tuple(m[k] for k in ("conn", "nodes", "E", "nu", "n2e_ptr", "n2e"))
         self._attached = weakref.WeakSet()  # hierarchies smoothing on this operator
         if int(bad.item()):
             self.close()
