@@ -606,7 +606,11 @@ def build_amg(system: SolidSystem, storage: tuple = ("fp64", "fp32", "fp16"), au
     lists the value precisions the V-cycle may run with.  Unless a launch
     shape is given (waves/unroll/xcd_remap) or ``autotune`` is off, the SpMV
     launch shapes of K's FP64 and FP32 values are autotuned (cached per
-    matrix shape) and used by K and by the cycle alike."""
+    matrix shape) and used by K and by the cycle alike.  The strength
+    threshold (``theta`` on the finest level, times ``theta_decay`` per level)
+    is mesh-dependent: 0.08 on every level suits the cantilever boxes, the
+    1.05M-DOF I-beam converges in 44 instead of 50 iterations with 0.1 and
+    decay 0.4."""
     from .amg_preconditioner import SmoothedAggregationAMG
     tunes = None
     if autotune and not any(k in kw for k in ("waves", "unroll", "xcd_remap")):
