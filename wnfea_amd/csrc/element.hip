@@ -437,6 +437,25 @@ __device__ __forceinline__ double quad_perm(double v) {
   const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), Ctrl, 0xF, 0xF, false);
   return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
+// v of lane o (a compile-time constant after unrolling) of the quad
+template <int O>
+__device__ __forceinline__ float quad_bcast_c(float v) {
+  return quad_perm<O | (O << 2) | (O << 4) | (O << 6)>(v);
+}
+template <int O>
+__device__ __forceinline__ double quad_bcast_c(double v) {
+  return quad_perm<O | (O << 2) | (O << 4) | (O << 6)>(v);
+}
+template <typename T>
+__device__ __forceinline__ T quad_bcast(T v, int o) {
+  switch (o) {
+    case 0: return quad_bcast_c<0>(v);
+    case 1: return quad_bcast_c<1>(v);
+    case 2: return quad_bcast_c<2>(v);
+    default: return quad_bcast_c<3>(v);
+  }
+}
+
 // Sum over the 4 lanes of a quad; every lane gets the same bits,
 // (v0 + v1) + (v2 + v3) up to commutation.  Plain adds (no contraction with
 // the product that made v), so each step is one v_add_f32_dpp in FP32.
@@ -500,16 +519,27 @@ __global__ __launch_bounds__(256) void c3d10_pa_elem_kernel(int M, const int* __
 #pragma unroll
   for (int k = 0; k < 9; ++k) Ji[k] = Affine ? qd[(size_t)k * M + e] : qd[((size_t)k * M + e) * 4 + g];
   const T wd = Affine ? qd[(size_t)9 * M + e] : qd[((size_t)9 * M + e) * 4 + g];
-  int nn[10];
+  // lane g loads the scaled values of nodes g, g + 4, g + 8 only; the quad
+  // shares them by DPP broadcast (a third of the memory instructions)
+  T own[3][3];
 #pragma unroll
-  for (int i = 0; i < 10; ++i) nn[i] = conn[(size_t)e * 10 + i];
+  for (int j = 0; j < 3; ++j) {
+    const int i = g + 4 * j;
+    if (i < 10) {
+      const int n3 = 3 * conn[(size_t)e * 10 + i];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) own[j][c] = U[n3 + c] * sc[n3 + c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) own[j][c] = T(0);
+    }
+  }
   T Q[4][3];
 #pragma unroll
   for (int i = 0; i < 10; ++i) {
-    const int n3 = 3 * nn[i];
     T u[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) u[c] = U[n3 + c] * sc[n3 + c];
+    for (int c = 0; c < 3; ++c) u[c] = quad_bcast(own[i >> 2][c], i & 3);
     if (i < 4) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) Q[i][c] = cc[i] * u[c];  // corners come first: initialise
