@@ -474,3 +474,28 @@ def test_matrix_free_operator_curved_and_affine_elements():
         assert _rel(z32, z_ref) <= 1e-4, curved
         op.close()
         amg.close()
+
+
+def test_graph_replay_with_matrix_free_operators_after_reattach(c0_system):
+    """Replayed iteration graphs that contain the matrix-free kernels are
+    keyed on every buffer those kernels take: re-attaching the smoother (new
+    scaling and quadrature copies) captures a new graph, and every result is
+    bitwise the host-launch result."""
+    import torch
+    from wnfea_amd.solver.solid_solver import MatrixFreeOperator, build_amg, set_graphs, solve_mixed
+    amg = build_amg(c0_system, storage=("fp64", "fp32"))
+    op = MatrixFreeOperator(c0_system)
+    try:
+        op.attach_smoother(amg)
+        set_graphs(False)
+        x0, it0, _, _, _ = solve_mixed(c0_system, amg, tol=1e-8, operator=op, check_every=4)
+        set_graphs(True)
+        x1, it1, _, _, _ = solve_mixed(c0_system, amg, tol=1e-8, operator=op, check_every=4)
+        op.attach_smoother(amg)  # fresh copies
+        x2, it2, _, _, _ = solve_mixed(c0_system, amg, tol=1e-8, operator=op, check_every=4)
+        assert it0 == it1 == it2
+        assert torch.equal(x0, x1) and torch.equal(x0, x2)
+    finally:
+        set_graphs(False)
+        op.close()
+        amg.close()

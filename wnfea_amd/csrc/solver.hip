@@ -347,8 +347,8 @@ static void key_amg(std::vector<uint64_t>& k, const Amg* g) {
   for (int v : {g->store, g->n_coarse, (int)g->L.size()}) key_push(k, v);
   for (const void* p : {(const void*)g->inv, (const void*)g->inv32, (const void*)g->tiles64, (const void*)g->tiles32,
                         (const void*)g->tij, (const void*)g->tpart, (const void*)g->mfs.op,
-                        (const void*)g->mfs.s64, (const void*)g->mfs.s32, (const void*)g->mfs.qd32,
-                        (const void*)g->mfs.fe32})
+                        (const void*)g->mfs.s64, (const void*)g->mfs.s64e, (const void*)g->mfs.s32,
+                        (const void*)g->mfs.s32e, (const void*)g->mfs.qd32, (const void*)g->mfs.fe32})
     key_push(k, p);
   key_tune(k, g->tune);
   for (const auto& lp : g->L) {
@@ -480,9 +480,12 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
       key_push(key, p);
     key_push(key, n);
     if (mf) {
-      for (const void* p : {(const void*)mf->qd, (const void*)mf->fe, (const void*)mf->conn, (const void*)mf->fix,
-                            (const void*)mf->E, (const void*)mf->nu, (const void*)mf->n2e})
+      // every pointer and size the matrix-free kernels take
+      for (const void* p : {(const void*)mf->qd, (const void*)mf->fe, (const void*)mf->mask, (const void*)mf->conn,
+                            (const void*)mf->fix, (const void*)mf->E, (const void*)mf->nu, (const void*)mf->n2e,
+                            (const void*)mf->n2e_ptr, (const void*)mf->chunk})
         key_push(key, p);
+      for (int v : {mf->M, mf->n_nodes, mf->n_chunks, mf->affine}) key_push(key, v);
     } else {
       key_matrix(key, A);
     }
