@@ -347,9 +347,10 @@ def tune_spmv(K, shape: str | None, log) -> dict:
 
 
 def apply_tune(amg, tunes: dict) -> None:
-    """The cycle's FP64 shape (the constructor set the FP32 one for both)."""
-    t = tunes["fp64"]
-    amg.set_tune("fp64", t["waves"], t["unroll"], t["xcd_remap"])
+    """The cycle's launch shapes from the fine matrix's (FP32/FP16 and FP64 values)."""
+    for p in ("fp32", "fp64"):
+        t = tunes[p]
+        amg.set_tune(p, t["waves"], t["unroll"], t["xcd_remap"])
 
 
 def run_multirhs(args, log) -> int:
@@ -593,8 +594,17 @@ def main() -> int:
     main_op = mf_op if args.operator == "matrix_free" else None
     alt_op = None if args.operator == "matrix_free" else mf_op
     alt_name = "assembled" if args.operator == "matrix_free" else "matrix_free"
+    cycle_tune = None
     if args.smoother_operator == "matrix_free":
         mf_op.attach_smoother(amg)
+        # the cycle's products are now the coarse-level and transfer SpMVs:
+        # their own launch shapes (timed on whole cycles)
+        from wnfea_amd.solver.autotune import autotune_cycle
+        cycle_tune = {}
+        for p in ("fp32", "fp64"):
+            c = autotune_cycle(amg, p)
+            cycle_tune[p] = {k: c[k] for k in ("waves", "unroll", "xcd_remap", "seconds")}
+        log(f"[bench] cycle launch shapes {cycle_tune}")
 
     def mixed_solve():
         return solve_mixed(system, amg, tol=1e-8, amg_precision=args.amg_precision, operator=main_op)
@@ -655,6 +665,8 @@ def main() -> int:
     # the same two solves with the other operator in the finest-level smoother
     alt_sm_name = "assembled" if args.smoother_operator == "matrix_free" else "matrix_free"
     amg.attach_matrix_free(mf_op if alt_sm_name == "matrix_free" else None)
+    if alt_sm_name == "assembled":
+        apply_tune(amg, tunes)  # the fine matrix's shapes for a cycle that uses it
     alt_sm = {"operator": alt_sm_name}
     for name, fn in (("mixed", mixed_solve), ("fp64", fp64_solve)):
         fn()
@@ -671,6 +683,9 @@ def main() -> int:
             f"rel {relm:.2e}")
     alt_sm["speedup_mixed_vs_fp64"] = round(alt_sm["fp64"]["seconds"] / alt_sm["mixed"]["seconds"], 3)
     amg.attach_matrix_free(mf_op if args.smoother_operator == "matrix_free" else None)
+    if cycle_tune:
+        for p_, c in cycle_tune.items():
+            amg.set_tune(p_, c["waves"], c["unroll"], c["xcd_remap"])
     # the same two solves with the other launch mode (graph replay or host launches)
     set_graphs(not args.graphs)
     eager = {}
@@ -769,6 +784,7 @@ def main() -> int:
                      "kernel": f"sell_spmv_kernel<3,3,float,float,{tune['waves']},{tune['unroll']},OpAxpby> "
                                f"(FP32 3x3 BSR SpMV, {info['lanes']} lanes per block row)"},
         "spmv_tune": tunes,
+        "cycle_tune": cycle_tune,
         "matrix_free_operator": mf_info,
         "setup": {"mesh_sec": round(t_mesh, 2), "rcm_sec": round(t_rcm, 2), "assembly_sec": round(t_asm, 2),
                   "amg_setup_sec": round(t_amg, 2), "amg": st},

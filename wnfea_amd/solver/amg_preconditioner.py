@@ -326,6 +326,12 @@ class SmoothedAggregationAMG:
         self._mf, self._mf_scale = op, scale
         if op is not None:
             op._attached.add(self)
+            if getattr(self, "autotuned", False):
+                # the cycle's products changed: retune its launch shapes
+                from .autotune import autotune_cycle
+                for prec in ("fp32", "fp64"):
+                    if prec in self.storage:
+                        autotune_cycle(self, prec)
 
     @property
     def matrix_free_smoothing(self) -> bool:

@@ -94,3 +94,51 @@ def autotune_spmv(K, precision: str = "fp32", reps: int = 20, use_cache: bool = 
         cache[key] = choice
         _store_cache(cache)
     return dict(choice, cached=False, table=table)
+
+
+def _cycle_key(amg, precision: str) -> str:
+    import torch
+    name = torch.cuda.get_device_name(amg.fine.device)
+    shape = ",".join(f"{lv.A.shape[0]}/{lv.A.nnz}" for lv in amg.levels)
+    mf = "mf" if getattr(amg, "matrix_free_smoothing", False) else "asm"
+    return f"cycle-rev{_KERNEL_REV}|{name}|{shape}|{mf}|{precision}"
+
+
+def autotune_cycle(amg, precision: str = "fp32", reps: int = 10, use_cache: bool = True) -> dict:
+    """Pick and apply the SpMV launch shape of the V-cycle for one value
+    precision by timing whole cycles (HIP events on the launch stream): with
+    the finest level matrix-free the cycle's products are the coarse-level
+    and transfer SpMVs, whose best shape differs from the fine matrix's
+    (I-beam FP32: 4 wavefronts per slice, 1 block in flight, 3 % faster per
+    solve than the fine matrix's 2 x 2).  Cached per hierarchy shape."""
+    import torch
+    key = _cycle_key(amg, precision)
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        c = cache[key]
+        amg.set_tune(precision, c["waves"], c["unroll"], c["xcd_remap"])
+        return dict(c, cached=True)
+    dt = torch.float32 if precision != "fp64" else torch.float64
+    n = amg.levels[0].A.shape[0]
+    r = torch.randn(n, device=amg.fine.device).to(dt)
+    z = torch.empty_like(r)
+    table = []
+    for waves, unroll in itertools.product((1, 2, 4), (1, 2)):
+        amg.set_tune(precision, waves, unroll, True)
+        for _ in range(2):
+            amg.vcycle(r, z)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(reps):
+            amg.vcycle(r, z)
+        b.record()
+        torch.cuda.synchronize()
+        table.append({"waves": waves, "unroll": unroll, "xcd_remap": True, "seconds": a.elapsed_time(b) / 1e3 / reps})
+    best = min(table, key=lambda t: t["seconds"])
+    amg.set_tune(precision, best["waves"], best["unroll"], True)
+    choice = {k: best[k] for k in ("waves", "unroll", "xcd_remap", "seconds")}
+    if use_cache:
+        cache[key] = choice
+        _store_cache(cache)
+    return dict(choice, cached=False, table=table)

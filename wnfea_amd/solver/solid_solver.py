@@ -635,6 +635,7 @@ def build_amg(system: SolidSystem, storage: tuple = ("fp64", "fp32", "fp16"), au
     if tunes:
         t = tunes["fp64"]
         amg.set_tune("fp64", t["waves"], t["unroll"], t["xcd_remap"])
+    amg.autotuned = tunes is not None  # attach_matrix_free then retunes the cycle
     return amg
 
 
