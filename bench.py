@@ -526,8 +526,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how the "
-              "solve would shard); rerun with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: the solve path has no multi-GPU implementation (it would shard "
+              "by contiguous RCM row ranges with one halo exchange per SpMV); rerun with --gpus 1", file=sys.stderr)
         return 2
     import torch
     from wnfea_amd.mesh.rcm import apply_rcm_to_model
