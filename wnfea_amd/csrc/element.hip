@@ -872,7 +872,10 @@ int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s) {
   WN_HIP(hipMalloc(&op.mask, sizeof(double) * n));
   int* curved = nullptr;
   WN_HIP(hipMalloc(&curved, sizeof(int)));
-  WN_HIP(hipMemsetAsync(curved, 0, sizeof(int), s));
+  if (const hipError_t me = hipMemsetAsync(curved, 0, sizeof(int), s); me != hipSuccess) {
+    (void)hipFree(curved);
+    return static_cast<int>(me);
+  }
   hipLaunchKernelGGL(c3d10_qdata_kernel, dim3(ceil_div(op.M, 128)), dim3(128), 0, s, op.M, op.conn, nodes, op.qd, bad,
                      curved);
   hipLaunchKernelGGL(dof_scale_kernel<double>, dim3(vec_grid(n)), dim3(256), 0, s, n, nullptr, op.fix, op.mask);
