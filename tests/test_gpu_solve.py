@@ -499,3 +499,32 @@ def test_graph_replay_with_matrix_free_operators_after_reattach(c0_system):
         set_graphs(False)
         op.close()
         amg.close()
+
+
+def test_cycle_launch_shape_autotune(c0_system, tmp_path, monkeypatch):
+    """The V-cycle's launch shapes are timed on whole cycles once the finest
+    level is matrix-free: the pick is one of the candidates, it is cached, and
+    every shape gives the same cycle up to FP32 rounding (the shapes only
+    change the order of each row's sum)."""
+    import torch
+    from wnfea_amd.solver.autotune import autotune_cycle
+    from wnfea_amd.solver.solid_solver import MatrixFreeOperator, build_amg
+    monkeypatch.setenv("WNFEA_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    amg = build_amg(c0_system, storage=("fp64", "fp32"), autotune=False, waves=1, unroll=1, xcd_remap=True)
+    op = MatrixFreeOperator(c0_system)
+    try:
+        op.attach_smoother(amg)  # built without autotuning: the shapes stay as given
+        r = torch.as_tensor(np.random.default_rng(21).standard_normal(c0_system.n_dofs), device="cuda").float()
+        z_ref = amg.vcycle(r).double().cpu().numpy()
+        c = autotune_cycle(amg, "fp32", reps=3)
+        assert not c["cached"] and (c["waves"], c["unroll"]) in {(w, u) for w in (1, 2, 4) for u in (1, 2)}
+        assert len(c["table"]) == 6 and all(t["seconds"] > 0 for t in c["table"])
+        assert autotune_cycle(amg, "fp32")["cached"]
+        for w in (1, 2, 4):
+            for u in (1, 2):
+                amg.set_tune("fp32", w, u, True)
+                z = amg.vcycle(r).double().cpu().numpy()
+                assert _rel(z, z_ref) <= 1e-5, (w, u)
+    finally:
+        op.close()
+        amg.close()
