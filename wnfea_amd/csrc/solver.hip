@@ -96,6 +96,24 @@ __global__ __launch_bounds__(256) void sym_tile_pack_kernel(int n, int nt, const
   }
 }
 
+// v from another lane of its 16-lane DPP row (both 32-bit halves)
+template <int Ctrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(u & 0xffffffffu), Ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), Ctrl, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+// Sum over the 16 lanes of a DPP row; every lane of the row gets the same
+// bits (each step adds two equal-bit partners up to commutation).
+__device__ __forceinline__ double dpp_row16_sum(double v) {
+  v += dpp_f64<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x141>(v);  // row_half_mirror: lane i <- 7 - i within 8
+  v += dpp_f64<0x140>(v);  // row_mirror: lane i <- 15 - i within 16
+  return v;
+}
+
 // Thread t of the block holds V = 16 B / sizeof(T) consecutive columns
 // col0 = (t % CPR) * V of NI rows t / CPR + RPI * m: every load instruction
 // of the block covers 4 KiB of the tile contiguously.
@@ -139,11 +157,13 @@ __global__ __launch_bounds__(256) void sym_tile_gemv_kernel(int n, const T* __re
     }
     rowp[m] = s;
   }
-  // row partials: sum over the CPR lanes of a row (all in one wavefront)
+  // row partials: sum over the CPR lanes of a row, 16 of them in one DPP row
+  // (quad swaps, then the half-row and row mirrors), plus one shuffle when a
+  // row spans 32 lanes
 #pragma unroll
   for (int m = 0; m < NI; ++m) {
-#pragma unroll
-    for (int off = CPR / 2; off > 0; off >>= 1) rowp[m] += __shfl_xor(rowp[m], off, 64);
+    rowp[m] = dpp_row16_sum(rowp[m]);
+    if (CPR == 32) rowp[m] += __shfl_xor(rowp[m], 16, 64);
   }
   double* prow = part + (2 * k) * kTile;
   if (t % CPR == 0) {
@@ -151,21 +171,18 @@ __global__ __launch_bounds__(256) void sym_tile_gemv_kernel(int n, const T* __re
     for (int m = 0; m < NI; ++m) prow[r0 + RPI * m] = rowp[m];
   }
   if (ij.x == ij.y) return;  // the diagonal tile has no transposed product
-  // column partials: lanes with the same column inside a wavefront, then the
-  // four wavefronts in order through LDS
+  // column partials: every lane's V column sums meet in LDS, one thread per
+  // column adds the block's RPI row groups in order
+  __shared__ double cs[256 / (kTile / 4)][kTile];
 #pragma unroll
-  for (int v = 0; v < V; ++v) {
-#pragma unroll
-    for (int off = CPR; off < 64; off <<= 1) colp[v] += __shfl_xor(colp[v], off, 64);
-  }
-  __shared__ double cs[4][kTile];
-  const int w = t / 64, lane = t % 64;
-  if (lane < CPR) {
-#pragma unroll
-    for (int v = 0; v < V; ++v) cs[w][c0 + v] = colp[v];
-  }
+  for (int v = 0; v < V; ++v) cs[r0][c0 + v] = colp[v];
   __syncthreads();
-  if (t < kTile) part[(2 * k + 1) * kTile + t] = ((cs[0][t] + cs[1][t]) + cs[2][t]) + cs[3][t];
+  if (t < kTile) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPI; ++q) a += cs[q][t];
+    part[(2 * k + 1) * kTile + t] = a;
+  }
 }
 
 // y_I = sum over m of (m < I ? column partial of tile (m, I) : row partial of
