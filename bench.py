@@ -439,7 +439,12 @@ def run_beams(args, log) -> int:
     bs = BeamSystem(model)
     U0 = torch.zeros(6 * bs.n_nodes, dtype=torch.float64, device="cuda")
     bs.assemble_tangent(U0)
-    bs.build_amg()
+    amg_kw = {}
+    if args.amg_theta is not None:
+        amg_kw["theta"] = args.amg_theta
+    if args.amg_theta_decay is not None:
+        amg_kw["theta_decay"] = args.amg_theta_decay
+    bs.build_amg(**amg_kw)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     log(f"[bench] lattice {meta['n_nodes']} nodes {meta['n_beams']} beams; setup {t_setup:.1f}s "
@@ -477,7 +482,8 @@ def run_beams(args, log) -> int:
         "ms_per_step": round(t_newton * 1e3, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 residual/Newton, f64 CG / f32 AMG", "data": "synthetic (seeded loads)",
         "config": {"workload": f"corot_beam_lattice_{args.cells}^3", "n_dofs": meta["n_dofs"], "n_beams": M,
-                   "load_steps": args.load_steps, "seed": 4, "parallelism": "single-gpu"},
+                   "load_steps": args.load_steps, "seed": 4, "amg_theta": amg_kw.get("theta", 0.02),
+                   "amg_theta_decay": amg_kw.get("theta_decay", 1.0), "parallelism": "single-gpu"},
         "newton_iters_per_step": st["newton_iters"], "pcg_iters": st["pcg_iters"], "tangents": st["tangents"],
         "final_residual": st["final_residual"], "setup_sec": round(t_setup, 2),
         "force_kernel": {"seconds": t_f, "elements_per_s": M / t_f, "gbs": bytes_f / t_f / 1e9,
@@ -510,7 +516,7 @@ def main() -> int:
     ap.add_argument("--operator", choices=["matrix_free", "assembled"], default="matrix_free",
                     help="CG's operator K: matrix-free (quadrature data) or the assembled FP64 BSR matrix")
     ap.add_argument("--amg-theta", type=float, default=None,
-                    help="SA strength threshold on the finest level (I-beam default 0.1, box 0.08)")
+                    help="SA strength threshold on the finest level (I-beam default 0.1, box 0.08, beam lattice 0.02)")
     ap.add_argument("--amg-theta-decay", type=float, default=None,
                     help="factor on the threshold per coarser level (I-beam default 0.4, box 1.0)")
     ap.add_argument("--smoother-operator", choices=["matrix_free", "assembled"], default="matrix_free",
