@@ -448,7 +448,8 @@ def run_beams(args, log) -> int:
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     log(f"[bench] lattice {meta['n_nodes']} nodes {meta['n_beams']} beams; setup {t_setup:.1f}s "
-        f"(AMG levels {[l['n'] for l in bs.amg.stats()['levels']]})")
+        f"(AMG levels {[l['n'] for l in bs.amg.stats()['levels']]}, cycle launch shape "
+        f"{ {k: v for k, v in (getattr(bs, 'cycle_tune', None) or {}).items() if k != 'table'} })")
     U, st = bs.solve_newton(n_load_steps=args.load_steps)
     t_newton = st["solve_sec"]
     log(f"[bench] newton {t_newton:.2f}s its/step {st['newton_iters']} pcg {st['pcg_iters']} "
@@ -486,6 +487,7 @@ def run_beams(args, log) -> int:
                    "amg_theta_decay": amg_kw.get("theta_decay", 1.0), "parallelism": "single-gpu"},
         "newton_iters_per_step": st["newton_iters"], "pcg_iters": st["pcg_iters"], "tangents": st["tangents"],
         "final_residual": st["final_residual"], "setup_sec": round(t_setup, 2),
+        "cycle_tune": {k: v for k, v in (getattr(bs, "cycle_tune", None) or {}).items() if k != "table"},
         "force_kernel": {"seconds": t_f, "elements_per_s": M / t_f, "gbs": bytes_f / t_f / 1e9,
                          "bytes_per_call": bytes_f, "rel_diff_vs_oracle": err},
         "cpu_baseline": {"value": round(t_cpu, 4), "unit": "s per internal-force evaluation", "cores": 1,

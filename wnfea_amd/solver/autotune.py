@@ -41,9 +41,31 @@ def _store_cache(cache: dict) -> None:
         pass  # a read-only home only costs a re-tune next time
 
 
-# bumped whenever the SpMV kernels change, so that cached choices of an older
-# build are not reused
-_KERNEL_REV = 2
+# bumped whenever the SpMV kernels or the timing method change, so that cached
+# choices of an older build are not reused
+_KERNEL_REV = 3
+
+# GPU cycles of the spin kernel that holds the stream while the timed launches
+# are queued (about 1 ms at the shader clock)
+_HOLD_CYCLES = 2_000_000
+
+
+def _timed(fn, reps: int) -> float:
+    """Average device seconds per call of ``fn`` (which launches on torch's
+    current stream).  A spin kernel holds the stream while the host queues
+    the ``reps`` calls, so that short kernels are timed back to back on the
+    device instead of at the rate the host can launch them (a 13 us SpMV is
+    otherwise as fast as a Python-level launch and the shapes time alike)."""
+    import torch
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    torch.cuda._sleep(_HOLD_CYCLES)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / 1e3 / reps
 
 
 def _key(K, precision: str) -> str:
@@ -63,14 +85,7 @@ def time_config(K, precision: str, waves: int, unroll: int, xcd: bool, reps: int
     y = torch.empty(K.shape[0], dtype=dt, device=K.device)
     for _ in range(2):
         K.spmv(x, y, precision=precision)
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    a.record()
-    for _ in range(reps):
-        K.spmv(x, y, precision=precision)
-    b.record()
-    torch.cuda.synchronize()
-    return a.elapsed_time(b) / 1e3 / reps
+    return _timed(lambda: K.spmv(x, y, precision=precision), reps)
 
 
 def autotune_spmv(K, precision: str = "fp32", reps: int = 20, use_cache: bool = True) -> dict:
@@ -127,14 +142,8 @@ def autotune_cycle(amg, precision: str = "fp32", reps: int = 10, use_cache: bool
         amg.set_tune(precision, waves, unroll, True)
         for _ in range(2):
             amg.vcycle(r, z)
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        a.record()
-        for _ in range(reps):
-            amg.vcycle(r, z)
-        b.record()
-        torch.cuda.synchronize()
-        table.append({"waves": waves, "unroll": unroll, "xcd_remap": True, "seconds": a.elapsed_time(b) / 1e3 / reps})
+        table.append({"waves": waves, "unroll": unroll, "xcd_remap": True,
+                      "seconds": _timed(lambda: amg.vcycle(r, z), reps)})
     best = min(table, key=lambda t: t["seconds"])
     amg.set_tune(precision, best["waves"], best["unroll"], True)
     choice = {k: best[k] for k in ("waves", "unroll", "xcd_remap", "seconds")}

@@ -92,6 +92,7 @@ class BeamSystem:
                                   fixed_dofs=model.fixed_dof_list(6), diag=diag, scale=scale, n_nodes=n,
                                   coords=nodes, mesh={}, dpn=6)
         self.amg = None
+        self.cycle_tune = None
         _sync()
         self.timings = {"setup_sec": time.perf_counter() - t0}
 
@@ -160,11 +161,15 @@ class BeamSystem:
         """FP32 SA-AMG of the current tangent.  The strength threshold
         defaults to 0.02: the 6-DOF lattice tangents converge fastest with
         weak filtering (40^3 lattice, 20 load steps: 3 120 CG iterations at
-        theta 0.01-0.03, 3 460 at 0.08, 8 152 at 0.15)."""
+        theta 0.01-0.03, 3 460 at 0.08, 8 152 at 0.15).  Unless a launch shape
+        is given, the cycle's SpMV launch shape is then tuned on whole cycles."""
+        from .autotune import autotune_cycle
         from .solid_solver import build_amg
         self.close()
         kw.setdefault("theta", 0.02)
         self.amg = build_amg(self.system, storage=("fp32",), **kw)
+        if self.amg.autotuned:
+            self.cycle_tune = autotune_cycle(self.amg, "fp32")
         return self.amg
 
     # ------------------------------------------------------------ Newton
