@@ -58,12 +58,18 @@ SellMatrix& SellMatrix::equilibrated() {
 // as many as keep about six blocks per lane (more row splitting shortens the
 // dependent load chain of each lane: the fine C3D10 operator with ~25 blocks
 // per row runs 15-20 % faster with 4 lanes per row than with 1), but no more
-// than an eighth of the average row length for the first rule.
-static int auto_lanes(int n_rows, long long nnzb) {
+// than an eighth of the average row length for the first rule.  A 6x6 block
+// is four 3x3 blocks' worth of loads, so those rows split further, to about
+// two blocks per lane, up to 8 lanes per row (the 6-DOF beam lattice's
+// V-cycle: 177 instead of 189 us; beyond 8 lanes the 8-column SpMM of the
+// multi-RHS cycle loses more to its wider shuffle reduction than it gains).
+static int auto_lanes(int n_rows, long long nnzb, int bb) {
   const double avg = n_rows > 0 ? double(nnzb) / n_rows : 0.0;
   int L = 1;
   while (L < 64 && (long long)n_rows * L < 64LL * 2048 && L * 8 < avg) L *= 2;
   while (L < 64 && 2 * L * 6 <= avg) L *= 2;
+  if (bb >= 36)
+    while (L < 8 && 2 * L * 2 <= avg) L *= 2;
   return L;
 }
 
@@ -82,7 +88,7 @@ int SellMatrix::build(int br_, int bc_, int n_rows_, int n_cols_, const int* row
   sigma = sigma_ <= 0 ? 4096 : sigma_;
   sigma = ((sigma + 63) / 64) * 64;
   nnzb = rowptr[n_rows];
-  lanes = lanes_ > 0 ? lanes_ : auto_lanes(n_rows, nnzb);
+  lanes = lanes_ > 0 ? lanes_ : auto_lanes(n_rows, nnzb, br_ * bc_);
   const int rows_per_slice = 64 / lanes;
   h_rowptr.assign(rowptr, rowptr + n_rows + 1);
   h_colidx.assign(colidx, colidx + nnzb);

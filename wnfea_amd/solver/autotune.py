@@ -43,7 +43,7 @@ def _store_cache(cache: dict) -> None:
 
 # bumped whenever the SpMV kernels or the timing method change, so that cached
 # choices of an older build are not reused
-_KERNEL_REV = 3
+_KERNEL_REV = 4
 
 # GPU cycles of the spin kernel that holds the stream while the timed launches
 # are queued (about 1 ms at the shader clock)
