@@ -484,7 +484,8 @@ def run_beams(args, log) -> int:
         "dtype": "f64 residual/Newton, f64 CG / f32 AMG", "data": "synthetic (seeded loads)",
         "config": {"workload": f"corot_beam_lattice_{args.cells}^3", "n_dofs": meta["n_dofs"], "n_beams": M,
                    "load_steps": args.load_steps, "seed": 4, "amg_theta": amg_kw.get("theta", 0.02),
-                   "amg_theta_decay": amg_kw.get("theta_decay", 1.0), "parallelism": "single-gpu"},
+                   "amg_theta_decay": amg_kw.get("theta_decay", 1.0), "chebyshev_ratio": 20.0,
+                   "parallelism": "single-gpu"},
         "newton_iters_per_step": st["newton_iters"], "pcg_iters": st["pcg_iters"], "tangents": st["tangents"],
         "final_residual": st["final_residual"], "setup_sec": round(t_setup, 2),
         "cycle_tune": {k: v for k, v in (getattr(bs, "cycle_tune", None) or {}).items() if k != "table"},

@@ -161,12 +161,16 @@ class BeamSystem:
         """FP32 SA-AMG of the current tangent.  The strength threshold
         defaults to 0.02: the 6-DOF lattice tangents converge fastest with
         weak filtering (40^3 lattice, 20 load steps: 3 120 CG iterations at
-        theta 0.01-0.03, 3 460 at 0.08, 8 152 at 0.15).  Unless a launch shape
-        is given, the cycle's SpMV launch shape is then tuned on whole cycles."""
+        theta 0.01-0.03, 3 460 at 0.08, 8 152 at 0.15), and the Chebyshev
+        smoother's interval [lmax/ratio, lmax] defaults to ratio 20 (2 931
+        iterations, 5 % faster than the solids' 30; 10-20 are alike, 60 takes
+        3 652).  Unless a launch shape is given, the cycle's SpMV launch shape
+        is then tuned on whole cycles."""
         from .autotune import autotune_cycle
         from .solid_solver import build_amg
         self.close()
         kw.setdefault("theta", 0.02)
+        kw.setdefault("ratio", 20.0)
         self.amg = build_amg(self.system, storage=("fp32",), **kw)
         if self.amg.autotuned:
             self.cycle_tune = autotune_cycle(self.amg, "fp32")
