@@ -520,6 +520,8 @@ def main() -> int:
                     help="CG's operator K: matrix-free (quadrature data) or the assembled FP64 BSR matrix")
     ap.add_argument("--amg-theta", type=float, default=None,
                     help="SA strength threshold on the finest level (I-beam default 0.1, box 0.08, beam lattice 0.02)")
+    ap.add_argument("--amg-ratio", type=float, default=30.0,
+                    help="Chebyshev smoother interval [lmax/ratio, lmax] of the I-beam and box hierarchies")
     ap.add_argument("--amg-theta-decay", type=float, default=None,
                     help="factor on the threshold per coarser level (I-beam default 0.4, box 1.0)")
     ap.add_argument("--smoother-operator", choices=["matrix_free", "assembled"], default="matrix_free",
@@ -589,7 +591,7 @@ def main() -> int:
     th = args.amg_theta if args.amg_theta is not None else (0.1 if args.config == "ibeam" else 0.08)
     thd = args.amg_theta_decay if args.amg_theta_decay is not None else (0.4 if args.config == "ibeam" else 1.0)
     amg = build_amg(system, waves=tune["waves"], unroll=tune["unroll"], xcd_remap=tune["xcd_remap"],
-                    theta=th, theta_decay=thd)
+                    theta=th, theta_decay=thd, ratio=args.amg_ratio)
     apply_tune(amg, tunes)
     t_amg = time.perf_counter() - t0
     st = amg.stats()
