@@ -198,7 +198,7 @@ def run_batch(args, log) -> int:
     models = [b[0] for b in batch]
     n_dofs = sum(b[1]["n_dofs"] for b in batch)
     log(f"[bench] batch: 64 boxes, {n_dofs} DOF total ({time.perf_counter() - t0:.1f}s)")
-    bs = BatchedSolids(models)
+    bs = BatchedSolids(models, amg_kw={"ratio": args.amg_ratio})
     log(f"[bench] batched setup {bs.timings}")
     for _ in range(args.warmup):
         bs.solve(1e-8)
@@ -217,7 +217,7 @@ def run_batch(args, log) -> int:
     for m in models:
         mr, _, _ = apply_rcm_to_model(m)
         s_ = assemble_c3d10_bsr(mr)
-        a_ = build_amg(s_, storage=("fp32",))
+        a_ = build_amg(s_, storage=("fp32",), ratio=args.amg_ratio)
         op_ = MatrixFreeOperator(s_)
         op_.attach_smoother(a_)
         systems.append((s_, a_, op_))
@@ -521,7 +521,7 @@ def main() -> int:
     ap.add_argument("--amg-theta", type=float, default=None,
                     help="SA strength threshold on the finest level (I-beam default 0.1, box 0.08, beam lattice 0.02)")
     ap.add_argument("--amg-ratio", type=float, default=30.0,
-                    help="Chebyshev smoother interval [lmax/ratio, lmax] of the I-beam and box hierarchies")
+                    help="Chebyshev smoother interval [lmax/ratio, lmax] of the I-beam, box and batch hierarchies")
     ap.add_argument("--amg-theta-decay", type=float, default=None,
                     help="factor on the threshold per coarser level (I-beam default 0.4, box 1.0)")
     ap.add_argument("--smoother-operator", choices=["matrix_free", "assembled"], default="matrix_free",
