@@ -124,3 +124,99 @@ def test_oracle_newton_small_load_is_linear():
     Ulin = spla.spsolve(K0.tocsc(), F)
     assert np.linalg.norm(U - Ulin) <= 2e-3 * np.linalg.norm(Ulin)
     assert max(its) <= 5
+
+
+def _rigid_modes(X1, X2):
+    """12 x 6: translations and infinitesimal rotations u = w x X, theta = w."""
+    Rm = np.zeros((12, 6))
+    for n, X in enumerate((X1, X2)):
+        Rm[6 * n:6 * n + 3, :3] = np.eye(3)
+        for k in range(3):
+            w = np.eye(3)[k]
+            Rm[6 * n:6 * n + 3, 3 + k] = np.cross(w, X)
+            Rm[6 * n + 3:6 * n + 6, 3 + k] = w
+    return Rm
+
+
+# assembler.py:106-121: the axis-aligned members' local frames (rows lx, ly, lz)
+_AXIS_FRAMES = {
+    "+x": ([1, 0, 0], [[1, 0, 0], [0, 1, 0], [0, 0, 1]]),
+    "-x": ([-1, 0, 0], [[-1, 0, 0], [0, 1, 0], [0, 0, 1]]),
+    "+y": ([0, 1, 0], [[0, 1, 0], [1, 0, 0], [0, 0, 1]]),
+    "-y": ([0, -1, 0], [[0, -1, 0], [1, 0, 0], [0, 0, 1]]),
+    "+z": ([0, 0, 1], [[0, 0, 1], [1, 0, 0], [0, 1, 0]]),
+    "-z": ([0, 0, -1], [[0, 0, -1], [1, 0, 0], [0, 1, 0]]),
+}
+
+
+@pytest.mark.parametrize("key", sorted(_AXIS_FRAMES))
+def test_axis_aligned_beam_stiffness_pins(key):
+    """The axis-aligned branch of the linear beam stiffness
+    (assembler.py:13-148) pinned by what the mechanics fixes, not by a
+    restatement of the frame code:
+
+    * Euler-Bernoulli end stiffnesses: with node 1 held, a unit displacement
+      of node 2 along the member gives the axial force EA/L, along the frame's
+      ly the shear 12 E Iz / L^3 (and the end moment 6 E Iz / L^2 about lz),
+      along lz the shear 12 E Iy / L^3, and a unit twist the torque GJ/L
+      (Iy != Iz, so the assignment of the two bending stiffnesses to the two
+      transverse directions is checked);
+    * right-handed frames (+x, -y, +z): K annihilates the six rigid motions and,
+      for a circular section, equals the stiffness of the same member in any
+      other right-handed frame with the member axis (roll invariance), built
+      here from Gram-Schmidt;
+    * left-handed frames (-x, +y, -z; the reference rule makes them improper):
+      K equals the stiffness of the opposite-pointing member conjugated by the
+      reflection of that axis applied to translations and rotations alike,
+      and it does not annihilate rigid rotations (the reference's quirk)."""
+    d, F = _AXIS_FRAMES[key]
+    d, F = np.array(d, float), np.array(F, float)
+    L = 1.7
+    X1 = np.array([0.3, -0.2, 0.5])
+    X2 = X1 + L * d
+    nodes = np.vstack([X1, X2])
+    E, G, A, Iy, Iz, J = 70e9, 26e9, 1.2e-3, 3.1e-7, 5.3e-7, 2.2e-7
+    K = B.ke_linear(nodes, np.array([[0, 1]]), np.array([[E, G, A, Iy, Iz, J]]))[0]
+    lx, ly, lz = F
+    assert np.allclose(lx, d)
+
+    def response(u2, th2=np.zeros(3)):
+        u = np.zeros(12)
+        u[6:9], u[9:12] = u2, th2
+        return K @ u
+
+    f = response(lx)
+    assert np.allclose(f[6:9], E * A / L * lx, rtol=1e-12, atol=1e-6)
+    f = response(ly)
+    assert np.allclose(f[6:9], 12 * E * Iz / L ** 3 * ly, rtol=1e-12, atol=1e-6)
+    assert np.allclose(np.abs(f[9:12]), np.abs(6 * E * Iz / L ** 2 * lz), rtol=1e-12, atol=1e-6)
+    f = response(lz)
+    assert np.allclose(f[6:9], 12 * E * Iy / L ** 3 * lz, rtol=1e-12, atol=1e-6)
+    assert np.allclose(np.abs(f[9:12]), np.abs(6 * E * Iy / L ** 2 * ly), rtol=1e-12, atol=1e-6)
+    f = response(np.zeros(3), lx)
+    assert np.allclose(f[9:12], G * J / L * lx, rtol=1e-12, atol=1e-6)
+    assert np.allclose(K, K.T, rtol=0, atol=1e-9 * np.abs(K).max())
+    Rm = _rigid_modes(X1, X2)
+    right_handed = np.linalg.det(F) > 0
+    assert right_handed == (key in ("+x", "-y", "+z"))
+    if right_handed:
+        assert np.abs(K @ Rm).max() <= 1e-9 * np.abs(K).max()
+        # roll invariance with a circular section
+        Ic = 4.0e-7
+        Kc = B.ke_linear(nodes, np.array([[0, 1]]), np.array([[E, G, A, Ic, Ic, J]]))[0]
+        h = np.array([0.3, 0.5, 0.8])
+        e2 = h - (h @ d) * d
+        e2 /= np.linalg.norm(e2)
+        Rg = np.array([d, e2, np.cross(d, e2)])
+        Tg = np.kron(np.eye(4), Rg)
+        Kg = Tg.T @ B.ke_local(L, E, G, A, Ic, Ic, J) @ Tg
+        assert np.abs(Kc - Kg).max() <= 1e-9 * np.abs(Kg).max()
+    else:
+        assert np.abs(K @ Rm[:, 3:]).max() > 1e-3 * np.abs(K).max()
+        X2o = X1 - L * d  # the opposite-pointing member, a right-handed frame
+        Ko = B.ke_linear(np.vstack([X1, X2o]), np.array([[0, 1]]), np.array([[E, G, A, Iy, Iz, J]]))[0]
+        axis = int(np.argmax(np.abs(d)))
+        Q = np.eye(3)
+        Q[axis, axis] = -1.0
+        Qb = np.kron(np.eye(4), Q)
+        assert np.abs(K - Qb @ Ko @ Qb).max() <= 1e-9 * np.abs(K).max()

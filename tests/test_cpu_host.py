@@ -270,3 +270,136 @@ def test_precision_config_presets():
     assert mp.TRI_PRECISION.use_tri_precision and mp.TRI_PRECISION.switch_tol == 1e-2
     assert "Tri-precision" in mp.TRI_PRECISION.summary() and "Uniform" in mp.FULL_FP64.summary()
     assert abs(mp.MIXED_FP32.inner_eps_scale - np.sqrt(np.finfo(np.float32).eps)) == 0.0
+
+
+@pytest.mark.parametrize("tets_per_hex", [5, 6])
+def test_sa_setup_rank_deficient_aggregates(tets_per_hex):
+    """A high strength threshold leaves aggregates of one or two nodes that
+    cannot carry all six rigid-body modes: P gets zero columns and the
+    Galerkin product zero rows.  Setup must still give a nonsingular coarse
+    level (unit diagonal on the vanished rows, as amg_preconditioner.py:288-293
+    of the reference) whose stored inverse is its inverse, and the cycle must
+    still precondition CG."""
+    from oracle import oracle as O
+    from vcycle_mirror import pcg, vcycle
+    from wnfea_amd.solver.amg_preconditioner import sa_setup
+    model, _ = G.generate_c3d10_structured_block(1.0, 0.1, 0.1, 40, 4, 4, tip_load_total=-1000.0,
+                                                 tets_per_hex=tets_per_hex, load_direction="z")
+    E, nu = model.element_moduli()
+    fixed = model.fixed_dof_mask(3)
+    K, F = O.assemble_csr(model.mesh_nodes, model.solid_elements, E, nu, fixed, model.load_vector(3))
+    d = np.abs(K.diagonal())
+    s = 1.0 / np.sqrt(d)
+    A = (sp.diags(s) @ K @ sp.diags(s)).tocsr()
+    levels, inv = sa_setup(A.tobsr(blocksize=(3, 3)), model.mesh_nodes, fixed, theta=0.25, coarse_size=1000,
+                           dof_scale=np.sqrt(d))
+    assert len(levels) >= 2
+    P = levels[0].P.tocsc()
+    colnorm = np.sqrt(np.asarray(P.multiply(P).sum(axis=0))).ravel()
+    vanished = np.flatnonzero(colnorm == 0.0)
+    assert len(vanished) > 0  # the case this test is about
+    A1 = levels[1].A.tocsr()
+    assert np.all(A1.diagonal()[vanished] == 1.0)
+    Acd = levels[-1].A.toarray()
+    assert np.linalg.eigvalsh(Acd).min() > 0
+    assert np.abs(inv @ Acd - np.eye(len(Acd))).max() < 1e-8
+    u_hat, it = pcg(A, s * F, lambda r: vcycle(levels, inv, r), 1e-8)
+    assert it < 400  # a poor hierarchy (threshold 0.25), but a convergent one
+    assert np.linalg.norm(F - K @ (s * u_hat)) / np.linalg.norm(F) < 1e-7
+
+
+def test_reference_import_path():
+    """Reference callers' imports (tests/test_solid_bench.py:12-13,
+    tests/test_bsr_rcm.py:24-35, test_c3d10.py:11-19, wnfea/__init__.py:7-31)
+    resolve unchanged to this implementation's modules, one module object per
+    name; modules off the solve path stay absent."""
+    import importlib
+    import wnfea
+    from wnfea.solver.solid_solver import (assemble_c3d10_sparse_3dof, compute_c3d10_stress_field,  # noqa: F401
+                                           solve_c3d10_pcg, solve_c3d10_system)
+    from wnfea.mesh.rcm import (apply_rcm_to_model, compute_matrix_bandwidth,  # noqa: F401
+                                compute_node_adjacency_matrix, compute_rcm_order, permute_solution_to_original)
+    from wnfea.solver.bsr_matrix import bsr6x6_to_csr, csr_to_bsr6x6, hip_spmv_bsr6x6  # noqa: F401
+    from wnfea.elements.c3d10 import (b_matrix_c3d10, element_internal_forces_c3d10,  # noqa: F401
+                                      element_stiffness_c3d10, element_stresses_c3d10, jacobian_c3d10,
+                                      shape_functions_c3d10)
+    from wnfea.mesh.solid_generator import generate_c3d10_structured_block  # noqa: F401
+    from wnfea.solver.fast_kernels import compute_internal_forces_fast  # noqa: F401
+    from wnfea.solver.assembler import assemble_global_system_sparse  # noqa: F401
+    from wnfea.solver.residual import build_external_force_vector, compute_equilibrium_residual  # noqa: F401
+    from wnfea.solver.nonlinear_solver import pcg_solve, solve_nonlinear_jfnk  # noqa: F401
+    from wnfea.solver.dof_manager import DOFManager  # noqa: F401
+    from wnfea.boundary.conditions import LoadDef, create_fixed_support  # noqa: F401
+    from wnfea.model import FEAModel, PropertyAssignment  # noqa: F401
+    from wnfea import BlockBeamAMGPreconditioner, FEAModel as F2, MIXED_FP32  # noqa: F401
+    for name in ("solver.solid_solver", "mesh.rcm", "elements.c3d10", "model", "solver.amg_preconditioner"):
+        assert importlib.import_module("wnfea." + name) is importlib.import_module("wnfea_amd." + name)
+    assert wnfea.solver.solid_solver.solve_c3d10_system is solve_c3d10_system and F2 is FEAModel
+    with pytest.raises(ImportError):
+        importlib.import_module("wnfea.geometry.step_parser")
+
+
+@pytest.mark.parametrize("tets_per_hex", [5, 6])
+def test_structured_block_conformity(tets_per_hex):
+    """The reference-default 5-tet split (solid_generator.py:66-163) and the
+    6-tet split: node count of the half lattice actually used (5 921 for the
+    5-tet 40x4x4 box: the 5-tet split uses only some face diagonals), every
+    interior triangle shared by exactly two tets with opposite orientation,
+    every boundary triangle on the box surface, positive volumes, total
+    volume L*W*H, midside nodes at edge midpoints and shared by every tet on
+    the edge."""
+    from wnfea_amd.mesh.solid_generator import generate_c3d10_structured_block
+    L, W, H = 1.0, 0.1, 0.1
+    model, meta = generate_c3d10_structured_block(L, W, H, 40, 4, 4, tip_load_total=-1000.0,
+                                                  tets_per_hex=tets_per_hex, load_direction="z")
+    X, C = model.mesh_nodes, model.solid_elements
+    assert len(C) == tets_per_hex * 40 * 4 * 4
+    assert len(X) == (5921 if tets_per_hex == 5 else 6561)
+    assert len(np.unique(C)) == len(X)  # every node is used
+    P = X[C]
+    vol = np.einsum("mi,mi->m", np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), P[:, 3] - P[:, 0]) / 6.0
+    assert vol.min() > 0.0
+    assert abs(vol.sum() - L * W * H) < 1e-12
+    for m, (a, b) in enumerate(G.EDGES):
+        assert np.abs(P[:, 4 + m] - 0.5 * (P[:, a] + P[:, b])).max() < 1e-15
+    # edge -> midside node is a function of the edge (conforming quadratic mesh)
+    ends = np.sort(np.stack([C[:, [a, b]] for a, b in G.EDGES], axis=1).reshape(-1, 2), axis=1)
+    mids = C[:, 4:10].reshape(-1)
+    key = ends[:, 0].astype(np.int64) * len(X) + ends[:, 1]
+    order = np.argsort(key, kind="stable")
+    k, md = key[order], mids[order]
+    same = k[1:] == k[:-1]
+    assert np.all(md[1:][same] == md[:-1][same])
+    # faces: oriented outward from each tet (vertex order 0-1-2-3 has positive volume)
+    faces = np.concatenate([C[:, [0, 2, 1]], C[:, [0, 1, 3]], C[:, [1, 2, 3]], C[:, [0, 3, 2]]])
+    srt = np.sort(faces, axis=1)
+    _, inv, cnt = np.unique(srt, axis=0, return_inverse=True, return_counts=True)
+    inv = inv.ravel()
+    assert cnt.max() == 2
+    # interior faces appear twice with opposite orientation (an even and an odd
+    # permutation of the sorted triple)
+    def parity(f):
+        s = np.argsort(f, axis=1)
+        inv_count = (s[:, 0] > s[:, 1]).astype(int) + (s[:, 0] > s[:, 2]) + (s[:, 1] > s[:, 2])
+        return inv_count % 2
+    par = parity(faces)
+    twice = np.flatnonzero(cnt == 2)
+    first = np.full(len(cnt), -1)
+    second = np.full(len(cnt), -1)
+    for i, f in enumerate(inv):
+        if first[f] < 0:
+            first[f] = i
+        else:
+            second[f] = i
+    assert np.all(par[first[twice]] != par[second[twice]])
+    # boundary faces lie on the box surface
+    once = first[cnt == 1]
+    c = X[faces[once]]  # (nb, 3, 3)
+    on = np.zeros(len(once), dtype=bool)
+    for ax, ext in enumerate((L, W, H)):
+        for v in (0.0, ext):
+            on |= np.all(np.abs(c[:, :, ax] - v) < 1e-12, axis=1)
+    assert on.all()
+    boundary_area = 2 * (L * W + L * H + W * H)
+    area = 0.5 * np.linalg.norm(np.cross(c[:, 1] - c[:, 0], c[:, 2] - c[:, 0]), axis=1).sum()
+    assert abs(area - boundary_area) < 1e-12

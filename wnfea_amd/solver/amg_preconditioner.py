@@ -178,6 +178,16 @@ def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None
         R.sort_indices()
         Ac = (R @ (A @ P).tobsr(blocksize=(b, k))).tobsr(blocksize=(k, k))
         Ac = (0.5 * (Ac + Ac.T)).tobsr(blocksize=(k, k))
+        # An aggregate that cannot carry all k near-null-space modes (one or
+        # two solid nodes, collinear nodes) leaves zero columns in P, hence
+        # zero rows and columns in Ac.  Give them a unit diagonal (the
+        # reference does the same, amg_preconditioner.py:288-293): they
+        # receive no restricted residual and prolong nothing, and the
+        # coarser levels and the dense coarse solve stay nonsingular.
+        dAc = Ac.diagonal()
+        vanished = dAc == 0.0
+        if vanished.any():
+            Ac = (Ac + sp.diags(vanished.astype(np.float64))).tobsr(blocksize=(k, k))
         Ac.sum_duplicates()
         Ac.sort_indices()
         levels.append(AMGLevelData(A=A, dinv=dinv, lmax=lmax, P=P, R=R, n_aggregates=na))
@@ -188,14 +198,29 @@ def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None
     # on the I-beam an exact solve on ~11 k rows instead of ~170 rows takes CG
     # from 90 to 50 iterations.
     Acd = levels[-1].A.toarray()
-    try:
-        import scipy.linalg as sla
-        c = sla.cho_factor(Acd, lower=True, check_finite=False)
-        inv = sla.cho_solve(c, np.eye(len(Acd)), check_finite=False)
-    except np.linalg.LinAlgError:
-        inv = np.linalg.inv(Acd)
-    inv = 0.5 * (inv + inv.T)
+    inv = _coarse_inverse(Acd)
     return levels, inv
+
+
+def _coarse_inverse(Acd: np.ndarray) -> np.ndarray:
+    """Symmetric inverse of the coarsest operator by Cholesky.  A matrix that
+    is only semidefinite in floating point gets the reference's relative
+    1e-12 diagonal shift (amg_preconditioner.py:314) before a second
+    factorisation; an indefinite one falls back to LU."""
+    import scipy.linalg as sla
+    n = len(Acd)
+    if n == 0:
+        return np.zeros((0, 0))
+    for shift in (0.0, 1e-12 * max(float(np.abs(np.diag(Acd)).max()), 1e-300)):
+        try:
+            c = sla.cho_factor(Acd + shift * np.eye(n) if shift else Acd, lower=True, check_finite=False)
+            inv = sla.cho_solve(c, np.eye(n), check_finite=False)
+            break
+        except np.linalg.LinAlgError:
+            continue
+    else:
+        inv = np.linalg.inv(Acd)
+    return 0.5 * (inv + inv.T)
 
 
 class SmoothedAggregationAMG:

@@ -157,8 +157,10 @@ class BeamSystem:
                    "wnfea_beam_ke_linear")
         self._assemble(Ke, 0)
 
-    def build_amg(self, **kw):
-        """FP32 SA-AMG of the current tangent.  The strength threshold
+    def build_amg(self, storage: tuple = ("fp32",), **kw):
+        """SA-AMG of the current tangent, values kept in ``storage`` (FP32 by
+        default; FP64 for an all-FP64 inner solve, FP32 + FP16 for the
+        tri-precision switch).  The strength threshold
         defaults to 0.02: the 6-DOF lattice tangents converge fastest with
         weak filtering (40^3 lattice, 20 load steps: 3 120 CG iterations at
         theta 0.01-0.03, 3 460 at 0.08, 8 152 at 0.15), and the Chebyshev
@@ -171,24 +173,39 @@ class BeamSystem:
         self.close()
         kw.setdefault("theta", 0.02)
         kw.setdefault("ratio", 20.0)
-        self.amg = build_amg(self.system, storage=("fp32",), **kw)
+        self.amg = build_amg(self.system, storage=tuple(storage), **kw)
         if self.amg.autotuned:
-            self.cycle_tune = autotune_cycle(self.amg, "fp32")
+            self.cycle_tune = autotune_cycle(self.amg, "fp32" if "fp32" in storage else "fp64")
         return self.amg
 
     # ------------------------------------------------------------ Newton
     def solve_newton(self, n_load_steps: int = 20, max_newton_iter: int = 25, tol_rel: float = 1e-5,
-                     tol_abs: float = 1e-5, inner_max_iter: int = 500, verbose: bool = False, amg_kw=None):
-        """Load-stepped Newton (see module doc).  Returns (U (device, 6 per
+                     tol_abs: float = 1e-5, inner_max_iter: int = 500, verbose: bool = False, amg_kw=None,
+                     preconditioner: str = "amg", amg_precision: str = "fp32", tri_switch: float | None = None):
+        """Load-stepped Newton (see module doc).  The inner solve is FP64 CG
+        on the tangent, preconditioned by point Jacobi (``preconditioner=
+        "jacobi"``) or by the SA-AMG V-cycle in ``amg_precision`` ("fp32" or
+        "fp64"); with ``tri_switch`` the cycle runs with FP16-stored values
+        until the inner relative residual is below tri_switch and with FP32
+        values after that (the reference's tri-precision preconditioner,
+        ``amg_preconditioner.py:538-584``).  Returns (U (device, 6 per
         node), stats)."""
         import torch
         from .solid_solver import pcg_device
+        if preconditioner not in ("amg", "jacobi"):
+            raise ValueError("preconditioner must be 'amg' or 'jacobi'")
+        if amg_precision not in ("fp32", "fp64"):
+            raise ValueError("amg_precision must be 'fp32' or 'fp64'")
+        use_amg = preconditioner == "amg"
+        storage = (amg_precision,) if tri_switch is None else ("fp32", "fp16")
+        if tri_switch is not None:
+            amg_precision = "fp32"
         t0 = time.perf_counter()
         dev = self.system.K.device
         U = torch.zeros(6 * self.n_nodes, dtype=torch.float64, device=dev)
-        if self.amg is None:
+        if use_amg and (self.amg is None or not set(storage) <= set(self.amg.storage)):
             self.assemble_tangent(U)
-            self.build_amg(**dict(amg_kw or {}))
+            self.build_amg(storage=storage, **dict(amg_kw or {}))
         stats = {"newton_iters": [], "pcg_iters": 0, "residual_evals": 0, "tangents": 0}
         for step, lam in enumerate(np.linspace(1.0 / n_load_steps, 1.0, n_load_steps), 1):
             R = self.residual(U, lam)
@@ -207,9 +224,28 @@ class BeamSystem:
                 inner_rel = max(eta * rn, 1e-10) / (rn + 1e-20)
                 self.assemble_tangent(U)
                 stats["tangents"] += 1
-                dU, pits, _, _ = pcg_device(self.system, b=-R, rtol=inner_rel, max_iter=inner_max_iter,
-                                            preconditioner="amg", precision="fp64", amg=self.amg,
-                                            amg_precision="fp32")
+                if not use_amg:
+                    dU, pits, _, _ = pcg_device(self.system, b=-R, rtol=inner_rel, max_iter=inner_max_iter,
+                                                preconditioner="jacobi", precision="fp64")
+                elif tri_switch is not None:
+                    # FP16-stored cycles down to the switch, then FP32 cycles
+                    # on the remaining residual
+                    d16, p16, _, _ = pcg_device(self.system, b=-R, rtol=max(tri_switch, inner_rel),
+                                                max_iter=inner_max_iter, preconditioner="amg", precision="fp64",
+                                                amg=self.amg, amg_precision="fp16")
+                    R16 = -R - self.system.K.spmv(d16)
+                    rel16 = float(torch.linalg.norm(R16)) / (rn + 1e-300)
+                    dU, pits = d16, p16
+                    if rel16 > inner_rel:
+                        d32, p32, _, _ = pcg_device(self.system, b=R16, rtol=inner_rel / rel16,
+                                                    max_iter=inner_max_iter, preconditioner="amg",
+                                                    precision="fp64", amg=self.amg, amg_precision="fp32")
+                        dU, pits = d16 + d32, p16 + p32
+                    stats["fp16_pcg_iters"] = stats.get("fp16_pcg_iters", 0) + p16
+                else:
+                    dU, pits, _, _ = pcg_device(self.system, b=-R, rtol=inner_rel, max_iter=inner_max_iter,
+                                                preconditioner="amg", precision="fp64", amg=self.amg,
+                                                amg_precision=amg_precision)
                 stats["pcg_iters"] += pits
                 prev = rn
                 alpha = 1.0
@@ -260,16 +296,33 @@ def solve_nonlinear_jfnk(model: FEAModel, n_load_steps: int = 5, max_newton_iter
                          tol_abs: float = 1e-5, use_amg: bool = True, use_mixed_precision: bool = False,
                          use_tri_precision: bool = False, switch_tol: float = 1e-2, verbose: bool = True):
     """Reference signature (``nonlinear_solver.py:159-170``); returns U_full
-    (N*6, numpy) and stores it in ``model.displacements``.  The inner solve
-    is always FP64 CG with the FP32 AMG V-cycle (the precision flags are
-    accepted for compatibility)."""
+    (N*6, numpy) and stores it in ``model.displacements``.
+
+    The flags select the inner solve as the reference's do
+    (``nonlinear_solver.py:203-255``): ``use_amg`` the SA-AMG V-cycle (else
+    point Jacobi, the reference's diagonal fallback); the cycle runs in FP64
+    by default (the reference's FULL_FP64 preset), in FP32 with
+    ``use_mixed_precision`` (MIXED_FP32), and with FP16-stored values above
+    ``switch_tol`` and FP32 below it with ``use_tri_precision``
+    (TRI_PRECISION).  The Newton residual, the Krylov vectors and the
+    update stay FP64 in every mode."""
     errors = model.validate_for_solving()
     if errors:
         raise ValueError("Cannot solve model:\n  " + "\n  ".join(errors))
+    if not (model.mesh_elements is not None and len(model.mesh_elements)) or \
+            (getattr(model, "solid_elements", None) is not None and len(model.solid_elements)):
+        raise ValueError("solve_nonlinear_jfnk runs the device Newton solver on beam-only models; mixed "
+                         "beam-solid models go through compute_equilibrium_residual, MatrixFreeJFNKOperator "
+                         "and pcg_solve")
+    if use_tri_precision:
+        amg_precision, tri = "fp32", float(switch_tol)
+    else:
+        amg_precision, tri = ("fp32" if use_mixed_precision else "fp64"), None
     bs = BeamSystem(model)
     try:
         U, _ = bs.solve_newton(n_load_steps=n_load_steps, max_newton_iter=max_newton_iter, tol_rel=tol_rel,
-                               tol_abs=tol_abs, verbose=verbose)
+                               tol_abs=tol_abs, verbose=verbose, preconditioner="amg" if use_amg else "jacobi",
+                               amg_precision=amg_precision, tri_switch=tri)
         out = U.cpu().numpy()
     finally:
         bs.close()
