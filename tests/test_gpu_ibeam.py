@@ -1,9 +1,18 @@
-"""The 1.05M-DOF C3D10 I-beam (3 m, 300 layers) at full size: properties
-that do not need an oracle solve of this size.  The matrix-free operator
-applies the assembled FP64 K and is symmetric; the mixed solve (FP64 CG on
-the matrix-free K, FP32 V-cycle with the matrix-free finest smoother)
-reaches 1e-8 on the assembled FP64 residual, agrees with the all-FP64 solve
-to 1e-6 and is bitwise reproducible."""
+"""The 1.05M-DOF C3D10 I-beam (3 m, 300 layers) at full size.
+
+Against the CPU FP64 oracle on the same mesh: the RCM permutation equals
+scipy's reverse_cuthill_mckee on this machine (rcm.py:85), the device 3x3 BSR
+pattern is bit-exact with the oracle's scalar CSR (solid_solver.py:26-129)
+converted by the block rule of bsr_matrix.py:49-122 and its values agree to
+FP64 round-off, the mixed solution meets 1e-8 on the oracle's own K and CPU
+SpMV, and the device cell / nodal von Mises fields equal the oracle's
+(solid_solver.py:311-365) on the same displacements.
+
+Self-consistency at size: the matrix-free operator applies the assembled
+FP64 K and is symmetric; the mixed solve (FP64 CG on the matrix-free K, FP32
+V-cycle with the matrix-free finest smoother) reaches 1e-8 on the assembled
+FP64 residual, agrees with the all-FP64 solve to 1e-6 and is bitwise
+reproducible."""
 import numpy as np
 import pytest
 
@@ -20,21 +29,98 @@ def ibeam():
     from wnfea_amd.mesh.rcm import apply_rcm_to_model
     from wnfea_amd.mesh.solid_generator import generate_c3d10_ibeam
     from wnfea_amd.solver.solid_solver import MatrixFreeOperator, assemble_c3d10_bsr, build_amg
-    model, meta = generate_c3d10_ibeam(n_layers=300, length=3.0)
+    model0, meta = generate_c3d10_ibeam(n_layers=300, length=3.0)
     assert meta["n_dofs"] == 1054755
-    model, _, _ = apply_rcm_to_model(model)
+    model, n2o, _ = apply_rcm_to_model(model0)
     system = assemble_c3d10_bsr(model, want_fp64_equilibrated=True)
     amg = build_amg(system, storage=("fp64", "fp32"), theta=0.1, theta_decay=0.4)
     op = MatrixFreeOperator(system)
     op.attach_smoother(amg)
-    yield system, amg, op
+    yield {"model0": model0, "model": model, "n2o": n2o, "system": system, "amg": amg, "op": op}
     op.close()
     amg.close()
 
 
+@pytest.fixture(scope="module")
+def oracle_c1(ibeam):
+    """The oracle's FP64 CSR K and F of the RCM-ordered I-beam."""
+    from oracle import oracle as O
+    m = ibeam["model"]
+    E, nu = m.element_moduli()
+    K, F = O.assemble_csr(m.mesh_nodes, m.solid_elements, E, nu, m.fixed_dof_mask(3), m.load_vector(3))
+    return K, F
+
+
+@pytest.fixture(scope="module")
+def mixed_solution(ibeam):
+    from wnfea_amd.solver.solid_solver import solve_mixed
+    u, it, runs, rel, _ = solve_mixed(ibeam["system"], ibeam["amg"], tol=1e-8, amg_precision="fp32",
+                                      operator=ibeam["op"])
+    return u, it, rel
+
+
+def test_rcm_permutation_is_scipy_on_this_machine(ibeam):
+    """The library's RCM (host C++, seeded by numpy's argsort of the degrees)
+    gives scipy's permutation on the machine the solve runs on: the argsort
+    tie order is platform-dependent, so it is checked where it is used."""
+    import scipy.sparse.csgraph as csg
+    from wnfea_amd.mesh.rcm import compute_node_adjacency_matrix, compute_rcm_order
+    m0 = ibeam["model0"]
+    A = compute_node_adjacency_matrix(len(m0.mesh_nodes), None, m0.solid_elements)
+    ref = csg.reverse_cuthill_mckee(A, symmetric_mode=True)
+    n2o, o2n = compute_rcm_order(A)
+    assert np.array_equal(n2o, ref)
+    assert np.array_equal(ibeam["n2o"], ref)
+    assert np.array_equal(o2n[n2o], np.arange(len(n2o)))
+
+
+@pytest.mark.timeout(300)
+def test_assembly_pattern_bit_exact_and_values_vs_oracle(ibeam, oracle_c1):
+    from oracle import oracle as O
+    system = ibeam["system"]
+    K, F = oracle_c1
+    rp, ci, vals = O.csr_to_bsr(K, 3)
+    assert len(ci) == 8823109
+    assert np.array_equal(system.K.rowptr, rp) and np.array_equal(system.K.colidx, ci)
+    v = system.K.get_values().cpu().numpy()
+    assert v.shape == vals.shape
+    assert np.abs(v - vals).max() <= 1e-12 * np.abs(vals).max()
+    assert np.array_equal(system.F.cpu().numpy(), F)
+    assert np.array_equal(system.fixed_mask, ibeam["model"].fixed_dof_mask(3))
+
+
+def test_mixed_solution_residual_on_oracle_k(ibeam, oracle_c1, mixed_solution):
+    """||F - K u|| / ||F|| with the oracle's K and its CPU SpMV, u the mixed
+    (FP64 CG / FP32 V-cycle) device solution."""
+    from oracle import oracle as O
+    K, F = oracle_c1
+    u, it, rel = mixed_solution
+    uh = u.cpu().numpy()
+    r = F - O.csr_spmv(K, uh)
+    assert np.linalg.norm(r) / np.linalg.norm(F) <= 1e-8
+    assert it <= 60
+
+
+def test_von_mises_fields_vs_oracle(ibeam, mixed_solution):
+    """Cell stresses, cell and nodal von Mises from the device kernel equal the
+    oracle's stress field on the same displacements."""
+    from oracle import oracle as O
+    from wnfea_amd.solver.solid_solver import compute_c3d10_stress_field
+    m = ibeam["model"]
+    u, _, _ = mixed_solution
+    uh = u.cpu().numpy()
+    cs, cv, nv = compute_c3d10_stress_field(m, u, ibeam["system"].mesh)
+    E, nu = m.element_moduli()
+    rcs, rcv, rnv = O.stress_field(m.mesh_nodes, m.solid_elements, E, nu, uh)
+    assert np.abs(cs - rcs).max() <= 1e-10 * np.abs(rcs).max()
+    assert np.abs(cv - rcv).max() <= 1e-10 * np.abs(rcv).max()
+    assert np.abs(nv - rnv).max() <= 1e-10 * np.abs(rnv).max()
+    assert np.linalg.norm(nv - rnv) <= 1e-12 * np.linalg.norm(rnv)
+
+
 def test_matrix_free_operator_is_the_assembled_operator(ibeam):
     import torch
-    system, _, op = ibeam
+    system, op = ibeam["system"], ibeam["op"]
     assert op.affine
     g = torch.Generator(device="cuda").manual_seed(5)
     u = torch.randn(system.n_dofs, dtype=torch.float64, device="cuda", generator=g)
@@ -44,10 +130,10 @@ def test_matrix_free_operator_is_the_assembled_operator(ibeam):
     assert abs(float(v @ Ku) - float(u @ Kv)) <= 1e-12 * float(torch.linalg.norm(u) * torch.linalg.norm(Kv))
 
 
-def test_mixed_solve_meets_tolerance_and_matches_fp64(ibeam):
+def test_mixed_solve_meets_tolerance_and_matches_fp64(ibeam, mixed_solution):
     from wnfea_amd.solver.solid_solver import solve_mixed, true_residual
-    system, amg, op = ibeam
-    u, it, runs, rel, _ = solve_mixed(system, amg, tol=1e-8, amg_precision="fp32", operator=op)
+    system, amg, op = ibeam["system"], ibeam["amg"], ibeam["op"]
+    u, it, rel = mixed_solution
     assert true_residual(system, u) <= 1e-8 and it <= 60
     u64, it64, _, _, _ = solve_mixed(system, amg, tol=1e-8, amg_precision="fp64", operator=op)
     assert true_residual(system, u64) <= 1e-8

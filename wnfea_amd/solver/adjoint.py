@@ -22,7 +22,9 @@ rebuilt only when the moduli moved by more than ``rebuild_rel``.  With
 operator and the V-cycle's finest smoothing steps on it, as the fastest
 forward solve does.
 ``compliance(E)`` and ``tip_displacement(E)`` are torch.autograd functions
-of the modulus tensor.
+of the modulus tensor: their forward runs the forward solve only; the
+adjoint solve (tip displacement) and the element sensitivities run in
+backward, so a caller under ``torch.no_grad()`` pays for neither.
 """
 from __future__ import annotations
 
@@ -139,32 +141,90 @@ class C3D10Adjoint:
         return u
 
     # ------------------------------------------------------------ responses
-    def compliance_and_grad(self, E):
-        """C = f^T u and dC/dE.  C is evaluated as the adjoint-corrected
+    def compliance_value(self, E):
+        """(C, u): C = f^T u(E), evaluated as the adjoint-corrected
         2 f^T u - u^T K u, whose error is quadratic in the solve error."""
         self.set_moduli(E)
         u = self.solve()
         F = self.system.F
-        C = float(2.0 * (F @ u) - u @ self.system.K.spmv(u))
+        return float(2.0 * (F @ u) - u @ self.system.K.spmv(u)), u
+
+    def compliance_and_grad(self, E):
+        """C = f^T u and dC/dE (compliance is self-adjoint: lam = u)."""
+        C, u = self.compliance_value(E)
         return C, element_sensitivities(self.system, u, u), u
 
-    def response_and_grad(self, E, c):
-        """J = c^T u and dJ/dE (one forward and one adjoint solve); J carries
-        the adjoint correction lam^T (f - K u)."""
+    def response_value(self, E, c):
+        """(J, u): J = c^T u(E) for a load-independent output vector c."""
         torch = self._torch
         self.set_moduli(E)
         c = torch.as_tensor(c, dtype=torch.float64, device=self.system.K.device).contiguous()
         u = self.solve()
+        return float(c @ u), u
+
+    def response_grad(self, E, c, u):
+        """(dJ/dE, lam) for J = c^T u: one adjoint solve K lam = c at the
+        moduli E the forward solution u belongs to."""
+        torch = self._torch
+        self.set_moduli(E)
+        c = torch.as_tensor(c, dtype=torch.float64, device=self.system.K.device).contiguous()
         lam = self.solve(c)
+        return element_sensitivities(self.system, u, lam), lam
+
+    def response_and_grad(self, E, c):
+        """J = c^T u and dJ/dE (one forward and one adjoint solve); J carries
+        the adjoint correction lam^T (f - K u)."""
+        _, u = self.response_value(E, c)
+        g, lam = self.response_grad(E, c, u)
+        c = self._torch.as_tensor(c, dtype=self._torch.float64, device=self.system.K.device)
         J = float(c @ u + lam @ (self.system.F - self.system.K.spmv(u)))
-        return J, element_sensitivities(self.system, u, lam), u, lam
+        return J, g, u, lam
+
+    def compliance_central_difference(self, E, k: int, h: float, tol: float = 1e-12):
+        """(C(E + h e_k) - C(E - h e_k)) / (2 h) from forward solves of the two
+        perturbed systems K(E +- h e_k) u+- = f.
+
+        Each perturbed solution is written u+- = u0 + d+- around the solution
+        u0 of K(E), so that d+- solves K(E +- h e_k) d = r0 - (K(E +- h e_k) -
+        K(E)) u0 with r0 = f - K(E) u0, and C+ - C- = f^T (d+ - d-).  Only the
+        rows of element k differ between K(E +- h e_k) and K(E), so the
+        right-hand sides are formed without the cancellation of f - K u over
+        the whole mesh, and the difference of the two compliances is not the
+        small difference of two large numbers.  The perturbed systems are
+        assembled on the device (the same reassembly as set_moduli)."""
+        torch = self._torch
+        E = np.asarray(E.detach().cpu() if isinstance(E, torch.Tensor) else E, dtype=np.float64).copy()
+        saved_tol = self.tol
+        try:
+            self.tol = tol
+            self.set_moduli(E)
+            u0 = self.solve()
+            F = self.system.F
+            K0u = self.system.K.spmv(u0)
+            r0 = F - K0u
+            vals = []
+            for sgn in (1.0, -1.0):
+                Ep = E.copy()
+                Ep[k] += sgn * h
+                self.set_moduli(Ep)
+                b = r0 - (self.system.K.spmv(u0) - K0u)
+                d = self.solve(b)
+                vals.append(float(F @ d))
+        finally:
+            self.tol = saved_tol
+            self.set_moduli(E)
+        return (vals[0] - vals[1]) / (2.0 * h)
 
     def compliance(self, E):
-        """Compliance f^T u(E) as a differentiable function of the modulus tensor."""
+        """Compliance f^T u(E) as a differentiable function of the modulus
+        tensor: the forward solve runs here, the element sensitivities only
+        when the graph is differentiated."""
         return _Compliance.apply(E, self)
 
     def tip_displacement(self, E, tip_nodes, component: int = 2):
-        """Mean tip displacement u(E) as a differentiable function of the moduli."""
+        """Mean tip displacement u(E) as a differentiable function of the
+        moduli: the forward solve runs here, the adjoint solve and the
+        element sensitivities only when the graph is differentiated."""
         c = tip_vector(self.model, tip_nodes, component)
         return _Response.apply(E, self, c)
 
@@ -175,26 +235,32 @@ def _autograd():
     class Compliance(torch.autograd.Function):
         @staticmethod
         def forward(ctx, E, problem):
-            C, g, _ = problem.compliance_and_grad(E.detach())
-            ctx.save_for_backward(g.to(E.dtype))
+            C, u = problem.compliance_value(E.detach())
+            ctx.problem, ctx.u = problem, u
+            ctx.save_for_backward(E)
             return torch.as_tensor(C, dtype=E.dtype, device=E.device)
 
         @staticmethod
         def backward(ctx, go):
-            (g,) = ctx.saved_tensors
-            return go * g.to(go.device), None
+            (E,) = ctx.saved_tensors
+            problem = ctx.problem
+            problem.set_moduli(E.detach())
+            g = element_sensitivities(problem.system, ctx.u, ctx.u)
+            return go * g.to(device=go.device, dtype=go.dtype), None
 
     class Response(torch.autograd.Function):
         @staticmethod
         def forward(ctx, E, problem, c):
-            J, g, _, _ = problem.response_and_grad(E.detach(), c)
-            ctx.save_for_backward(g.to(E.dtype))
+            J, u = problem.response_value(E.detach(), c)
+            ctx.problem, ctx.u, ctx.c = problem, u, c
+            ctx.save_for_backward(E)
             return torch.as_tensor(J, dtype=E.dtype, device=E.device)
 
         @staticmethod
         def backward(ctx, go):
-            (g,) = ctx.saved_tensors
-            return go * g.to(go.device), None, None
+            (E,) = ctx.saved_tensors
+            g, _ = ctx.problem.response_grad(E.detach(), ctx.c, ctx.u)
+            return go * g.to(device=go.device, dtype=go.dtype), None, None
 
     return Compliance, Response
 
