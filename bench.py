@@ -5,15 +5,19 @@ Workload (default): the C3D10 I-beam, 3 m long, 0.2 x 0.02 flanges, 0.012 web,
 DOF, 208 800 elements), root clamped, -50 kN shear + 5 kN m torsion at the
 tip.  One step = one solve of the resident system from u = 0 to
 ||f - K u|| / ||f|| <= 1e-8 (FP64-verified) with the mixed-precision solver:
-FP64 conjugate gradients on K preconditioned by a smoothed-aggregation AMG
-V-cycle that runs in FP32 on the equilibrated operator S K S (chunks of
-iterations replayed as HIP graphs).
+FP64 conjugate gradients on the matrix-free K preconditioned by a
+smoothed-aggregation AMG V-cycle that runs in FP32 on the equilibrated
+operator S K S (the finest level's smoothing steps matrix-free).
 
 The printed JSON line carries value = seconds per solve (lower is better),
-the same solve with the all-FP64 AMG-PCG and the speedup, the FP32 / FP64 BSR
-SpMV bandwidth, a roofline object for the FP32 3x3 BSR SpMV kernel (the
-dominant kernel of the solve) and a CPU baseline (the FP64 oracle on the host
-cores, bounded sample).
+the same solve with the all-FP64 AMG-PCG and the speedup with a per-part
+breakdown, the FP32 / FP64 3x3 BSR SpMV bandwidth (the metric's SpMV
+figure), a roofline object for the dominant kernel of the timed solve (the
+FP32 6x6 Chebyshev smoothing step of AMG level 1, three launches per
+iteration), the displacement and von Mises errors against the CPU FP64
+oracle, and a CPU baseline (the FP64 oracle on the host cores: a bounded
+sample on the I-beam, extrapolated, and a measured solve to tolerance of the
+19 683-DOF box).
 """
 from __future__ import annotations
 
@@ -38,16 +42,67 @@ def spmv_bytes(nnzb: int, nb: int, b: int, sv: int, sx: int) -> int:
     return nnzb * (b * b * sv + 4) + 4 * (nb + 1) + 2 * b * nb * sx
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sell_spmv_ibeam_1.05M_hbm_traffic.md")
+def cheb_bytes(nnzb: int, n: int, b: int, s: int) -> int:
+    """Algorithmic bytes of one Chebyshev smoothing step on a BSR(b) level of
+    n scalar rows with s-byte values and vectors: the matrix as in
+    spmv_bytes, and per scalar row x, b, D^-1 and d read, d and x' written."""
+    return nnzb * (b * b * s + 4) + 4 * (n // b + 1) + 6 * n * s
 
 
-def measured_traffic(precision: str):
-    """HBM bytes per launch of the I-beam SpMV from the committed rocprofv3
-    --pmc FETCH_SIZE / WRITE_SIZE summary (counters scaled by a 256 MiB copy
-    kernel), or None when that summary is absent."""
+def time_level_cheb(amg, level: int, precision: str, reps: int) -> float:
+    """Average device time (s) of one Chebyshev smoothing-step launch of an
+    AMG level, HIP events on the stream the library launches on."""
+    import torch
+    from wnfea_amd import _lib
+    L = _lib.load()
+    vt = {"fp64": 0, "fp32": 1}[precision]
+    stream = torch.cuda.current_stream()
+    _lib.check(L.wnfea_amg_level_cheb(amg.handle, level, vt, 4, _lib.stream_of()), "wnfea_amg_level_cheb")
+    st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    st.record(stream)
+    _lib.check(L.wnfea_amg_level_cheb(amg.handle, level, vt, reps, _lib.stream_of()), "wnfea_amg_level_cheb")
+    en.record(stream)
+    torch.cuda.synchronize()
+    return st.elapsed_time(en) / 1e3 / reps
+
+
+def jacobi_to_tol(system, tol: float, max_total: int) -> dict:
+    """FP64 Jacobi-PCG on the device (the reference's algorithm), restarted on
+    the true FP64 residual until ||f - K u|| <= tol ||f|| or max_total
+    iterations; reports whether it converged."""
+    import torch
+    from wnfea_amd.solver.solid_solver import pcg_device, true_residual
+    fn = float(system.F.norm())
+    b, u, total, rel, runs = system.F, None, 0, float("inf"), 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while total < max_total and runs < 8:
+        runs += 1
+        rt = 0.5 * tol * fn / float(b.norm())
+        x, it, _, _ = pcg_device(system, b=b, rtol=rt, max_iter=max_total - total, preconditioner="jacobi",
+                                 precision="fp64", check_every=64)
+        total += it
+        u = x if u is None else u + x
+        prev, rel = rel, true_residual(system, u)
+        if rel <= tol or rel > 0.5 * prev:
+            break
+        b = system.F - system.K.spmv(u)
+    torch.cuda.synchronize()
+    return {"seconds": round(time.perf_counter() - t0, 4), "iterations": total, "cg_runs": runs, "rel_res": rel,
+            "converged": rel <= tol}
+
+
+CHEB_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "amg_level1_cheb_ibeam_1.05M_hbm_traffic.md")
+
+
+def measured_traffic(path: str, precision: str):
+    """HBM bytes per launch from a committed rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE summary (counters scaled by a 256 MiB copy kernel), or None
+    when that summary is absent."""
     import re
     try:
-        txt = open(TRAFFIC_FILE).read()
+        txt = open(path).read()
     except OSError:
         return None
     m = re.search(r"^traffic_bytes_per_launch\s+fp32=(\d+)\s+fp64=(\d+)", txt, re.M)
@@ -75,12 +130,38 @@ def time_spmv(K, precision: str, reps: int) -> float:
     return st.elapsed_time(en) / 1e3 / reps
 
 
-def cpu_baseline(model, gpu_jacobi_iters: int | None, sample_iters: int = 60) -> dict:
-    """FP64 oracle (OpenMP C) on the host: assembly + a bounded number of
-    Jacobi-PCG iterations of the reference algorithm on the same system; the
-    per-iteration time is scaled by the iteration count the same algorithm
-    needs to reach 1e-8 (measured on the GPU)."""
+def _oracle_pcg_to_tol(O, K, F, tol: float, max_total: int):
+    """Oracle FP64 Jacobi-PCG restarted on its true residual (CPU SpMV) until
+    ||F - K x|| <= tol ||F||; (x, iterations, true relative residual)."""
+    fn = np.linalg.norm(F)
+    b, x, total, rel = F, np.zeros_like(F), 0, float("inf")
+    for _ in range(8):
+        d, it, _ = O.pcg(K, b, rtol=0.5 * tol * fn / np.linalg.norm(b), max_iter=max_total - total)
+        total += it
+        x = x + d
+        b = F - O.csr_spmv(K, x)
+        prev, rel = rel, np.linalg.norm(b) / fn
+        if rel <= tol or total >= max_total or rel > 0.5 * prev:
+            break
+    return x, total, rel
+
+
+def cpu_baseline(model, u_mixed, u_ref, vm_device, gpu_jacobi_iters: int | None, sample_iters: int = 60):
+    """The CPU leg: the FP64 oracle (OpenMP C restatement of the reference's
+    path) on the host cores.
+
+    Baseline: assembly and a bounded sample of `sample_iters` Jacobi-PCG
+    iterations on the same I-beam system, extrapolated to the iterations the
+    same algorithm takes to 1e-8 (the device Jacobi-PCG, restarted on its
+    true residual); and a measured oracle solve to 1e-8 of the 19 683-DOF
+    box.  Check: the oracle's K gives the residual of the mixed solution and
+    of the FP64 reference solution; the displacement error of the mixed
+    solution against the oracle-verified FP64 solution; the device von Mises
+    fields against the oracle's on the FP64 solution (error) and on the same
+    mixed displacements (kernel parity)."""
     from oracle import oracle as O
+    from wnfea_amd.mesh.rcm import apply_rcm_to_model
+    from wnfea_amd.mesh.solid_generator import generate_c3d10_structured_block
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     E, nu = model.element_moduli()
     t0 = time.perf_counter()
@@ -92,13 +173,44 @@ def cpu_baseline(model, gpu_jacobi_iters: int | None, sample_iters: int = 60) ->
     t_it = (time.perf_counter() - t0) / max(it, 1)
     need = gpu_jacobi_iters
     out = {"unit": "s", "cores": threads, "kind": "port",
-           "sample": (f"oracle FP64 Jacobi-PCG (reference algorithm, solid_solver.py:197-308), {it} iterations "
-                      f"on the same I-beam system ({t_it * 1e3:.1f} ms/iteration), assembly {t_asm:.1f} s; "
-                      f"scaled to the {need} iterations the same algorithm takes to 1e-8")}
+           "sample": (f"oracle FP64 Jacobi-PCG (the reference's algorithm, solid_solver.py:197-308) on the same "
+                      f"I-beam system: {it} iterations sampled ({t_it * 1e3:.1f} ms/iteration), assembly "
+                      f"{t_asm:.1f} s; value EXTRAPOLATED from those {it} iterations to the {need} iterations the "
+                      f"same algorithm takes to a true residual of 1e-8 (device Jacobi-PCG restarted on its FP64 "
+                      f"residual)"),
+           "extrapolated": True}
     out["value"] = round(t_it * need, 3) if need else None
     out["seconds_per_iteration"] = t_it
+    out["sampled_iterations"] = it
     out["assembly_sec"] = t_asm
-    return out
+    # a measured CPU solve to tolerance: the 19 683-DOF cantilever box
+    box, _ = generate_c3d10_structured_block(1.0, 0.1, 0.1, 40, 4, 4, tip_load_total=-1000.0, tets_per_hex=6,
+                                             load_direction="z")
+    box, _, _ = apply_rcm_to_model(box)
+    Eb, nub = box.element_moduli()
+    t0 = time.perf_counter()
+    Kb, Fb = O.assemble_csr(box.mesh_nodes, box.solid_elements, Eb, nub, box.fixed_dof_mask(3), box.load_vector(3))
+    t_asm_b = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, itb, relb = _oracle_pcg_to_tol(O, Kb, Fb, 1e-8, 200000)
+    out["box_19683_dof_measured"] = {"assembly_sec": round(t_asm_b, 3), "solve_sec": round(time.perf_counter() - t0, 3),
+                                     "iterations": itb, "rel_res": relb, "solver": "oracle FP64 Jacobi-PCG, restarted "
+                                     "on the true residual"}
+    # the check against the oracle
+    uh, urh = u_mixed.cpu().numpy(), u_ref.cpu().numpy()
+    fn = np.linalg.norm(F)
+    _, cv_ref, nv_ref = O.stress_field(model.mesh_nodes, model.solid_elements, E, nu, urh)
+    _, cv_same, nv_same = O.stress_field(model.mesh_nodes, model.solid_elements, E, nu, uh)
+    cv_d, nv_d = vm_device
+    err = {"reference_solution": "device FP64 AMG-PCG to the FP64 residual floor, verified on the oracle's K",
+           "rel_res_mixed_on_oracle_K": float(np.linalg.norm(F - O.csr_spmv(K, uh)) / fn),
+           "rel_res_reference_on_oracle_K": float(np.linalg.norm(F - O.csr_spmv(K, urh)) / fn),
+           "displacement_rel_err": float(np.linalg.norm(uh - urh) / np.linalg.norm(urh)),
+           "von_mises_nodal_rel_err": float(np.linalg.norm(nv_d - nv_ref) / np.linalg.norm(nv_ref)),
+           "von_mises_cell_rel_err": float(np.linalg.norm(cv_d - cv_ref) / np.linalg.norm(cv_ref)),
+           "von_mises_kernel_vs_oracle_same_u_max_rel": float(max(np.abs(nv_d - nv_same).max() / np.abs(nv_same).max(),
+                                                               np.abs(cv_d - cv_same).max() / np.abs(cv_same).max()))}
+    return out, err
 
 
 def spmv_only(system, info, reps: int, log) -> int:
@@ -533,7 +645,10 @@ def main() -> int:
                     help="time the matrix-free operator and V-cycle kernels alone (for counter runs)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV launches (and a 256 MiB copy for counter calibration), for rocprofv3 --pmc")
-    ap.add_argument("--ir", action="store_true", help="also time the restart-IR (FP32 Krylov) scheme")
+    ap.add_argument("--cheb-only", action="store_true",
+                    help="only the level-1 Chebyshev smoothing steps (FP32 then FP64) and a 256 MiB copy, "
+                         "for rocprofv3 --pmc")
+    ap.add_argument("--no-ir", action="store_true", help="skip the restart-IR (FP32 Krylov) scheme")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -616,6 +731,26 @@ def main() -> int:
             c = autotune_cycle(amg, p)
             cycle_tune[p] = {k: c[k] for k in ("waves", "unroll", "xcd_remap", "seconds")}
         log(f"[bench] cycle launch shapes {cycle_tune}")
+
+    if args.cheb_only:
+        solve_mixed(system, amg, tol=1e-8, amg_precision="fp32", operator=main_op)  # fills the level vectors
+        solve_mixed(system, amg, tol=1e-8, amg_precision="fp64", operator=main_op)
+        lv1 = amg.stats()["levels"][1]
+        nnzb1 = lv1["nnz"] // (lv1["block"] ** 2)
+        t32 = time_level_cheb(amg, 1, "fp32", args.spmv_reps)
+        t64 = time_level_cheb(amg, 1, "fp64", args.spmv_reps)
+        n = 64 * 2 ** 20
+        a = torch.ones(n, dtype=torch.float32, device=system.K.device)
+        c = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"cheb_fp32_us": t32 * 1e6, "cheb_fp64_us": t64 * 1e6, "n": lv1["n"], "nnzb": nnzb1,
+                          "bytes_fp32": cheb_bytes(nnzb1, lv1["n"], lv1["block"], 4),
+                          "bytes_fp64": cheb_bytes(nnzb1, lv1["n"], lv1["block"], 8),
+                          "copy_bytes_each_way": 4 * n, "cycle_tune": cycle_tune}), flush=True)
+        return 0
 
     def mixed_solve():
         return solve_mixed(system, amg, tol=1e-8, amg_precision=args.amg_precision, operator=main_op)
@@ -710,19 +845,63 @@ def main() -> int:
     set_graphs(args.graphs)
     log(f"[bench] {'host launches' if args.graphs else 'graphs'}: {eager}")
     ir = None
-    if args.ir:
+    if not args.no_ir:
+        # the reference's mixed scheme (solid_solver.py:271-300): FP32 Krylov on S K S inside FP64 refinement
         _, ir_outer, ir_inner, ir_rel, ir_t = ir_device(system, tol=1e-8, amg=amg, max_outer=30)
-        ir = {"seconds": round(ir_t, 4), "outer": ir_outer, "inner": ir_inner, "rel_res": ir_rel}
+        ir = {"seconds": round(ir_t, 4), "outer": ir_outer, "inner": ir_inner, "rel_res": ir_rel,
+              "converged": ir_rel <= 1e-8}
         log(f"[bench] restart IR (FP32 Krylov): {ir}")
-    # the reference's algorithm on the GPU: FP64 Jacobi-PCG
+    # the reference's algorithm on the GPU: FP64 Jacobi-PCG, restarted on its true residual
     jac = None
     if not args.no_jacobi:
-        xj, itj, _, tj = pcg_device(system, rtol=1e-8, max_iter=60000, preconditioner="jacobi", precision="fp64",
-                                    check_every=64)
-        relj = true_residual(system, xj)
-        jac = {"seconds": round(tj, 4), "iterations": itj, "rel_res": relj}
-        log(f"[bench] fp64 Jacobi-PCG: {tj:.2f}s iters {itj} rel {relj:.2e}")
-    # SpMV bandwidth and roofline of the FP32 3x3 kernel
+        jac = jacobi_to_tol(system, 1e-8, 200000)
+        log(f"[bench] fp64 Jacobi-PCG: {jac}")
+    # per-iteration breakdown of the mixed and the all-FP64 solve: CG's product,
+    # one V-cycle in each precision (device time), the rest (vector kernels and
+    # reductions) from the measured solve times
+    r64 = torch.randn(system.n_dofs, dtype=torch.float64, device=system.K.device)
+    r32 = r64.float()
+
+    def ev_time(fn, reps):
+        for _ in range(3):
+            fn()
+        st_, en_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        st_.record()
+        for _ in range(reps):
+            fn()
+        en_.record()
+        torch.cuda.synchronize()
+        return st_.elapsed_time(en_) / 1e3 / reps
+    z32, z64, y64 = torch.empty_like(r32), torch.empty_like(r64), torch.empty_like(r64)
+    t_vc32 = ev_time(lambda: amg.vcycle(r32, z32, storage=args.amg_precision if args.amg_precision == "fp32"
+                                        else "fp32"), 20)
+    t_vc64 = ev_time(lambda: amg.vcycle(r64, z64, storage="fp64"), 20)
+    t_kp = ev_time(lambda: mf_op.apply(r64, y64), 20)
+    rest_mixed = t_mixed / iters - t_kp - t_vc32
+    rest_fp64 = t_fp64 / it64 - t_kp - t_vc64
+    breakdown = {"cg_product_fp64_us": round(t_kp * 1e6, 1), "vcycle_fp32_us": round(t_vc32 * 1e6, 1),
+                 "vcycle_fp64_us": round(t_vc64 * 1e6, 1),
+                 "vector_kernels_and_host_mixed_us": round(rest_mixed * 1e6, 1),
+                 "vector_kernels_and_host_fp64_us": round(rest_fp64 * 1e6, 1),
+                 "iteration_mixed_us": round(t_mixed / iters * 1e6, 1),
+                 "iteration_fp64_us": round(t_fp64 / it64 * 1e6, 1),
+                 "vcycle_speedup_fp32_vs_fp64": round(t_vc64 / t_vc32, 3),
+                 "speedup_bound_if_vcycle_exactly_2x": round((t_kp + rest_fp64 + t_vc64) /
+                                                             (t_kp + rest_fp64 + 0.5 * t_vc64), 3),
+                 "note": "FP64 CG keeps its product and vector work in FP64 in both solvers; only the V-cycle "
+                         "changes precision"}
+    log(f"[bench] breakdown {breakdown}")
+    # the dominant kernel of the timed solve: the FP32 6x6 Chebyshev step of AMG level 1
+    lv1 = st["levels"][1]
+    b1 = lv1["block"]
+    nnzb1 = lv1["nnz"] // (b1 * b1)
+    t_ch32 = time_level_cheb(amg, 1, "fp32", args.spmv_reps)
+    t_ch64 = time_level_cheb(amg, 1, "fp64", args.spmv_reps)
+    ch32, ch64 = cheb_bytes(nnzb1, lv1["n"], b1, 4), cheb_bytes(nnzb1, lv1["n"], b1, 8)
+    log(f"[bench] level-1 Chebyshev step fp32 {t_ch32 * 1e6:.1f} us {ch32 / t_ch32 / 1e9:.0f} GB/s | "
+        f"fp64 {t_ch64 * 1e6:.1f} us {ch64 / t_ch64 / 1e9:.0f} GB/s")
+    # SpMV bandwidth of the FP32 and FP64 3x3 kernel (the metric's SpMV figure)
     nnzb, nb = info["nnzb"], system.n_nodes
     t32 = time_spmv(system.K, "fp32", args.spmv_reps)
     t64 = time_spmv(system.K, "fp64", args.spmv_reps)
@@ -747,11 +926,18 @@ def main() -> int:
     mf_info = {"apply_fp64_us": round(t_mf * 1e6, 2), "algorithmic_bytes": b_mf,
                "gbs": round(b_mf / t_mf / 1e9, 1), "affine": mf_op.affine}
     log(f"[bench] matrix-free FP64 apply {t_mf * 1e6:.1f} us {b_mf / t_mf / 1e9:.0f} GB/s algorithmic")
-    cpu = None
+    # FP64 reference solution (device FP64 AMG-PCG to the FP64 residual floor)
+    # and the device von Mises fields of the mixed solution, for the oracle check
+    from wnfea_amd.solver.solid_solver import compute_c3d10_stress_field
+    u_ref, it_ref, _, rel_ref, _ = solve_mixed(system, amg, tol=1e-12, amg_precision="fp64", operator=main_op,
+                                               max_restarts=6)
+    _, cv_d, nv_d = compute_c3d10_stress_field(model, u, system.mesh)
+    cpu, err_oracle = None, None
     if not args.no_cpu:
-        log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(model, jac["iterations"] if jac else None)
-        log(f"[bench] cpu baseline {cpu['value']} s ({cpu['cores']} threads)")
+        log("[bench] cpu baseline and oracle check ...")
+        cpu, err_oracle = cpu_baseline(model, u, u_ref, (cv_d, nv_d),
+                                       jac["iterations"] if jac and jac["converged"] else None)
+        log(f"[bench] cpu baseline {cpu['value']} s ({cpu['cores']} threads); oracle check {err_oracle}")
     out = {
         "metric": METRIC,
         "value": round(t_mixed, 5),
@@ -780,20 +966,29 @@ def main() -> int:
         "smoother_operator": args.smoother_operator,
         "other_smoother_operator": alt_sm,
         "speedup_mixed_vs_fp64_amg_pcg": round(t_fp64 / t_mixed, 3),
+        "speedup_breakdown": breakdown,
+        "err_vs_oracle": err_oracle,
         "fp64_jacobi_pcg": jac,
         "restart_ir_fp32_krylov": ir,
         **others,
         ("host_launch_seconds" if args.graphs else "graph_replay_seconds"): eager,
         "graphs": args.graphs,
-        "speedup_mixed_vs_fp64_jacobi_pcg": round(jac["seconds"] / t_mixed, 2) if jac else None,
+        "speedup_mixed_vs_fp64_jacobi_pcg": round(jac["seconds"] / t_mixed, 2) if jac and jac["converged"] else None,
         "displacement_rel_err_mixed_vs_fp64": err_u,
         "spmv_gbs": {"fp32": round(gbs32, 1), "fp64": round(gbs64, 1), "fp32_us": round(t32 * 1e6, 2),
                      "fp64_us": round(t64 * 1e6, 2), "bytes_fp32": b32, "bytes_fp64": b64},
-        "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": measured_traffic("fp32"),
-                     "algorithmic_bytes": b32,
-                     "kernel": f"sell_spmv_kernel<3,3,float,float,{tune['waves']},{tune['unroll']},OpAxpby> "
-                               f"(FP32 3x3 BSR SpMV, {info['lanes']} lanes per block row)"},
+        "roofline": {"bound": "hbm", "achieved": round(ch32 / t_ch32 / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ch32 / t_ch32 / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(CHEB_TRAFFIC_FILE, "fp32"),
+                     "traffic_source": "committed rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE summary "
+                                       "(profiles/amg_level1_cheb_ibeam_1.05M_hbm_traffic.md), not measured in this run",
+                     "algorithmic_bytes": ch32, "avg_launch_us": round(t_ch32 * 1e6, 2),
+                     "launches_per_iteration": 3,
+                     "kernel": f"sell_spmv_kernel<6,6,float,float,{cycle_tune['fp32']['waves'] if cycle_tune else tune['waves']},"
+                               f"{cycle_tune['fp32']['unroll'] if cycle_tune else tune['unroll']},OpCheb<6,float>> "
+                               f"(FP32 Chebyshev smoothing step of AMG level 1: {lv1['n']} rows, {nnzb1} 6x6 blocks)",
+                     "fp64_twin": {"avg_launch_us": round(t_ch64 * 1e6, 2), "algorithmic_bytes": ch64,
+                                   "frac": round(ch64 / t_ch64 / 1e9 / HBM_PEAK_GBS, 4)}},
         "spmv_tune": tunes,
         "cycle_tune": cycle_tune,
         "matrix_free_operator": mf_info,

@@ -303,6 +303,14 @@ int wnfea_amg_tune_precision(void *h, int prec, int waves, int xcd_remap, int un
  * and arithmetic). */
 int wnfea_amg_vcycle(void *h, int vt, const void *r, void *z, void *stream);
 
+/* reps Chebyshev smoothing steps of level lvl (0 <= lvl < last) on the
+ * level's own cycle vectors, with its assembled values in FP64 (vt 0) or
+ * FP32 (vt 1): the smoother kernel of the V-cycle alone, so that its time per
+ * launch and its HBM rate can be measured.  No reference counterpart (the
+ * reference times only its SpMV, hip_spmv_bsr6x6_benchmark,
+ * wnfea/solver/native/hip_kernels.cpp:3089-3164). */
+int wnfea_amg_level_cheb(void *h, int lvl, int vt, int reps, void *stream);
+
 /* ---------------------------------------------------------------- solvers */
 
 /* Solver workspace (Krylov vectors, reduction buffers, the replayable

@@ -28,6 +28,7 @@ struct Amg;
 struct AmgLevel;
 struct Solver;
 int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s);
+int amg_level_cheb(Amg& g, int l, int vt, int reps, hipStream_t s);
 template <typename T>
 int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, int amg_vt, const double* scale,
             const T* b, T* x, double rtol, int max_iter, int check_every, const SpmvTune& tune0, hipStream_t s,
@@ -373,6 +374,13 @@ int wnfea_amg_tune_precision(void* h, int prec, int waves, int xcd_remap, int un
 int wnfea_amg_vcycle(void* h, int vt, const void* r, void* z, void* stream) {
   if (!h || !r || !z) return kBadArg;
   return amg_apply(*static_cast<Amg*>(h), vt, r, z, nullptr, S(stream));
+}
+
+// reps Chebyshev smoothing steps of one level on its own vectors (vt 0 FP64,
+// 1 FP32 values): the cycle's smoother kernel alone, for timing
+int wnfea_amg_level_cheb(void* h, int lvl, int vt, int reps, void* stream) {
+  if (!h) return kBadArg;
+  return amg_level_cheb(*static_cast<Amg*>(h), lvl, vt, reps, S(stream));
 }
 
 // ------------------------------------------------------------ solvers

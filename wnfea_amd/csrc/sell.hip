@@ -372,7 +372,7 @@ template <int BR, int BC, typename VT, typename XT, int W, class Op>
 static void launch_w(const SellView<VT>& v, const XT* x, const Op& op, int grid, int unroll, int xcd, const int* flags,
                      int guard, hipStream_t s) {
   // keep the unrolled value registers within budget: U*BB values of VT
-  constexpr int kMaxU = (BR * BC * (int)sizeof(VT) <= 48) ? 4 : ((BR * BC * (int)sizeof(VT) <= 96) ? 2 : 1);
+  constexpr int kMaxU = (BR * BC * (int)sizeof(VT) <= 48) ? 4 : ((BR * BC * (int)sizeof(VT) <= 160) ? 2 : 1);
   const int u = std::min(unroll, kMaxU);
   if (u >= 4 && kMaxU >= 4)
     hipLaunchKernelGGL((sell_spmv_kernel<BR, BC, VT, XT, W, (kMaxU >= 4 ? 4 : 1), Op>), dim3(grid), dim3(64 * W), 0,

@@ -140,6 +140,15 @@ __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const
   if (guard_skip(flags, guard)) return;
   using AT = typename AccT<VT>::type;
   constexpr int BB = BR * BC;
+  // Loads per block and lane: the column index, the value chunks and the x
+  // gathers (at most one per entry).  For blocks of more than 96 bytes the
+  // loop bodies below ask the scheduler to issue all of a trip's loads
+  // before its FMAs, so that they are in flight together: left to itself it
+  // interleaves them with the FMAs to save registers and waits on each one
+  // in turn (six or more serial waits per FP32 6x6 block).  Smaller blocks
+  // are scheduled well without the hint.
+  constexpr int kLoadsPerBlock = 1 + (BB * (int)sizeof(VT) + 15) / 16 + BC;
+  constexpr bool kWideBlock = BB * (int)sizeof(VT) > 96;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int s = slice_of_block(A.n_slices, xcd_remap);  // uniform over the block
@@ -182,6 +191,10 @@ __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const
           for (int r = 0; r < BR; ++r)
 #pragma unroll
             for (int q = 0; q < BC; ++q) acc[r] = fma(AT(v[u][r * BC + q]), AT(xv[u][q]), acc[r]);
+        if constexpr (kWideBlock) {
+          __builtin_amdgcn_sched_group_barrier(0x020, kLoadsPerBlock * U, 0);
+          __builtin_amdgcn_sched_group_barrier(0x002, 4 * BB * U, 0);
+        }
       }
     }
     for (; j < len; j += W) {
@@ -195,6 +208,10 @@ __global__ __launch_bounds__(64 * W) void sell_spmv_kernel(SellView<VT> A, const
       for (int r = 0; r < BR; ++r)
 #pragma unroll
         for (int q = 0; q < BC; ++q) acc[r] = fma(AT(v[r * BC + q]), AT(xv[q]), acc[r]);
+      if constexpr (kWideBlock) {
+        __builtin_amdgcn_sched_group_barrier(0x020, kLoadsPerBlock, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 4 * BB, 0);
+      }
     }
     if constexpr (W > 1) {
       __shared__ AT red[W - 1][BR][64];

@@ -326,6 +326,38 @@ int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream
   }
 }
 
+// `reps` Chebyshev steps (the k = 1 coefficients) of level l < last on the
+// level's own vectors (b, x <-> tmp, d), with the level's assembled values in
+// the precision of vt (0 FP64, 1 FP32): the smoother kernel of a V-cycle on
+// its own, so that its time per launch can be measured.  The vectors hold
+// whatever the last cycle left in them.
+int amg_level_cheb(Amg& g, int l, int vt, int reps, hipStream_t s) {
+  if (l < 0 || l + 1 >= (int)g.L.size() || reps < 0) return kBadArg;
+  AmgLevel& L = *g.L[l];
+  const int deg = std::max(2, std::min(L.degree, 16));
+  const ChebCoef c = cheb_coefs(L.lmax, L.ratio, deg);
+  SpmvTune t = g.tune;
+  t.flags = nullptr;
+  t.guard = 0;
+  auto run = [&](auto* tag) -> int {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    T** v = level_vecs<T>(L);
+    const T* dinv = level_dinv<T>(L);
+    if (!v[0] || !v[1] || !v[2] || !v[3] || !dinv) return kNotReady;
+    T* cur = v[1];
+    T* oth = v[2];
+    for (int i = 0; i < reps; ++i) {
+      const int r = spmv_cheb<T, T>(*L.A, cur, oth, v[0], dinv, v[3], c.c1[1], c.c2[1], t, s);
+      if (r) return r;
+      std::swap(cur, oth);
+    }
+    return kOk;
+  };
+  if (vt == 0) return (g.store & 1) ? run((double*)nullptr) : kNotReady;
+  if (vt == 1) return (g.store & 2) ? run((float*)nullptr) : kNotReady;
+  return kBadMode;
+}
+
 // ------------------------------------------------------------------ PCG
 __global__ void pcg_start_kernel(double* sc, int* flags, double rtol2) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {

@@ -43,11 +43,13 @@ def _store_cache(cache: dict) -> None:
 
 # bumped whenever the SpMV kernels or the timing method change, so that cached
 # choices of an older build are not reused
-_KERNEL_REV = 4
+_KERNEL_REV = 5
 
 # GPU cycles of the spin kernel that holds the stream while the timed launches
-# are queued (about 1 ms at the shader clock)
+# are queued (about 1 ms at the shader clock), and the fastest shader clock of
+# the device (Hz): a hold of H cycles lasts at least H / _CLOCK_HZ seconds
 _HOLD_CYCLES = 2_000_000
+_CLOCK_HZ = 2.4e9
 
 
 def _timed(fn, reps: int) -> float:
@@ -55,16 +57,27 @@ def _timed(fn, reps: int) -> float:
     current stream).  A spin kernel holds the stream while the host queues
     the ``reps`` calls, so that short kernels are timed back to back on the
     device instead of at the rate the host can launch them (a 13 us SpMV is
-    otherwise as fast as a Python-level launch and the shapes time alike)."""
+    otherwise as fast as a Python-level launch and the shapes time alike).
+    The host's queueing time is measured: when it was not well inside the
+    hold (the device may have drained the queue and waited for the host),
+    the measurement is repeated with a hold sized from it."""
+    import time
     import torch
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    torch.cuda._sleep(_HOLD_CYCLES)
-    a.record()
-    for _ in range(reps):
-        fn()
-    b.record()
-    torch.cuda.synchronize()
+    hold = _HOLD_CYCLES
+    for _ in range(4):
+        torch.cuda.synchronize()
+        torch.cuda._sleep(hold)
+        a.record()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        t_queue = time.perf_counter() - t0
+        b.record()
+        torch.cuda.synchronize()
+        if t_queue * _CLOCK_HZ <= 0.5 * hold:
+            break
+        hold = int(4 * t_queue * _CLOCK_HZ)
     return a.elapsed_time(b) / 1e3 / reps
 
 
