@@ -363,6 +363,17 @@ int wnfea_bsr_spmm(void *h, int prec, int nrhs, const void *X, void *Y, void *st
 /* r = f - K u (FP64; r may be NULL) and out[0] = ||r||, out[1] = ||f||. */
 int wnfea_residual(void *solver, void *bsr, const double *u, const double *f, double *r, double *out, void *stream);
 
+/* wnfea_residual without any allocation, host copy or synchronization:
+ * r = f - K u (FP64; r may be NULL) and out_dev[0] = ||r||, out_dev[1] = ||f||
+ * written to DEVICE memory.  The reduction partials live in a static
+ * per-GPU scratch array of the library (2 MiB, held by each GPU it runs on
+ * until the process ends), so a matrix of more than 65 528 sliced-ELL slices
+ * (about 1M block rows at 16 rows per slice) is rejected with -2 (bad size).
+ * Calls that overlap on one GPU share that scratch and race.  No counterpart
+ * in the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops). */
+int wnfea_residual_noalloc(void *bsr, const double *u, const double *f, double *r, double *out_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

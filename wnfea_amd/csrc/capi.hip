@@ -9,6 +9,7 @@
 // replaces are in include/wnfea_hip.h.
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 #include "wn.hpp"
 
@@ -566,6 +567,75 @@ int wnfea_residual(void* solver, void* bsr, const double* u, const double* f, do
   out[0] = std::sqrt(h[kScTrue]);
   out[1] = std::sqrt(h[kScF2]);
   return kOk;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ no allocation
+// Scratch of wnfea_residual_noalloc: the reduction partials (up to
+// 4 * kNoAllocBlocks values, as a Workspace holds them) followed by the
+// device scalar slots.  Each GPU the library runs on holds its own copy of
+// this array until the process ends.  Host code never passes the symbol
+// itself to a kernel: it takes the array's device address on each device
+// once (hipGetSymbolAddress) and uses that address.
+constexpr int kNoAllocBlocks = 65536;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[4 * kNoAllocBlocks + kScCount];
+static double* g_scratch_addr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device (looked up once per device).
+static int scratch_address(double** out) {
+  int dev = -1;
+  WN_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return kBadArg;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    WN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return kNotReady;
+    g_scratch_addr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_addr[dev];
+  return kOk;
+}
+
+__global__ void norms_from_squares_kernel(const double* sc, double* out) {
+  if (threadIdx.x == 0) {
+    out[0] = sqrt(sc[kScTrue]);
+    out[1] = sqrt(sc[kScF2]);
+  }
+}
+
+extern "C" {
+
+// r = f - K u (when r is not null) and out_dev[0] = ||f - K u||,
+// out_dev[1] = ||f|| on the device: wnfea_residual's kernels with their
+// reduction partials in the library's static per-GPU scratch instead of a
+// solver workspace.  Allocates nothing, copies nothing to the host and does
+// not synchronize.  Calls that overlap on one GPU share the scratch and race.
+int wnfea_residual_noalloc(void* bsr, const double* u, const double* f, double* r, double* out_dev, void* stream) {
+  if (!bsr || !u || !f || !out_dev) return kBadArg;
+  auto* B = static_cast<WnBsr*>(bsr);
+  const SellMatrix& A = B->m;
+  if (A.br != A.bc) return kBadArg;
+  // the SpMV's partial blocks (n_slices rounded up to the 8 XCDs) and the dot's
+  // (at most 2048) must fit the scratch
+  if ((long long)A.n_slices + 8 > kNoAllocBlocks) return kBadSize;
+  double* base = nullptr;
+  int e = scratch_address(&base);
+  if (e) return e;
+  Workspace w;  // a view of the scratch: Workspace::alloc sees enough room and allocates nothing
+  w.partials = base;
+  w.sc = base + 4LL * kNoAllocBlocks;
+  w.max_blocks = kNoAllocBlocks;
+  const long long n = (long long)A.n_rows * A.br;
+  hipStream_t s = S(stream);
+  if ((e = launch_fill<double>(kScCount, w.sc, 0.0, s))) return e;  // a kernel, not a memset: capturable
+  if ((e = spmv_residual64(A, u, f, r, nullptr, nullptr, w, w.sc + kScTrue, B->tune, s))) return e;
+  if ((e = launch_dot<double>(n, f, f, w, w.sc + kScF2, s))) return e;
+  hipLaunchKernelGGL(norms_from_squares_kernel, dim3(1), dim3(64), 0, s, w.sc, out_dev);
+  return static_cast<int>(hipGetLastError());
 }
 
 }  // extern "C"
