@@ -67,6 +67,59 @@ def time_level_cheb(amg, level: int, precision: str, reps: int) -> float:
     return st.elapsed_time(en) / 1e3 / reps
 
 
+# Element stiffness kernel cost per C3D10 element, the dense convention of a
+# B^T D B contraction: per Gauss point D B (6 x 6 x 30 multiply-adds) and
+# B^T (D B) (30 x 30 x 6), 2 flop each, 4 points; bytes: 40 B connectivity,
+# 10 nodes x 24 B coordinates, E and nu (16 B), and the 900-value FP64 Ke.
+KE_FLOP_PER_ELEMENT = 4 * 2 * (6 * 6 * 30 + 30 * 30 * 6)
+KE_BYTES_PER_ELEMENT = 40 + 240 + 16 + 900 * 8
+
+
+def time_ke(system, reps: int) -> dict:
+    """Device time of the element stiffness kernel on every element of the
+    system (both kernel variants: vector ALU and matrix cores), HIP events on
+    the launch stream; the process default (vector ALU) is restored."""
+    import torch
+    from wnfea_amd import _lib
+    L = _lib.load()
+    mesh = system.mesh
+    M = mesh["n_elements"]
+    Ke = torch.empty((M, 30, 30), dtype=torch.float64, device=system.K.device)
+    bad = torch.zeros(1, dtype=torch.int32, device=system.K.device)
+    stream = torch.cuda.current_stream()
+    out = {}
+    try:
+        for variant, name in ((0, "valu"), (1, "mfma")):
+            _lib.check(L.wnfea_c3d10_ke_variant(variant), "wnfea_c3d10_ke_variant")
+
+            def run():
+                _lib.check(L.wnfea_c3d10_ke(M, _lib.dptr(mesh["conn"]), _lib.dptr(mesh["nodes"]),
+                                            _lib.dptr(mesh["E"]), _lib.dptr(mesh["nu"]), _lib.dptr(Ke),
+                                            _lib.dptr(bad), _lib.stream_of()), "wnfea_c3d10_ke")
+            run()
+            st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            st.record(stream)
+            for _ in range(reps):
+                run()
+            en.record(stream)
+            torch.cuda.synchronize()
+            t = st.elapsed_time(en) / 1e3 / reps
+            out[name] = {"us": round(t * 1e6, 1),
+                         "tflops_fp64": round(M * KE_FLOP_PER_ELEMENT / t / 1e12, 2),
+                         "frac_of_78.6_tflops": round(M * KE_FLOP_PER_ELEMENT / t / 78.6e12, 3),
+                         "gbs": round(M * KE_BYTES_PER_ELEMENT / t / 1e9, 1),
+                         "frac_of_hbm": round(M * KE_BYTES_PER_ELEMENT / t / 1e9 / HBM_PEAK_GBS, 3)}
+    finally:
+        L.wnfea_c3d10_ke_variant(0)
+    out["elements"] = M
+    out["flop_per_element"] = KE_FLOP_PER_ELEMENT
+    out["bytes_per_element"] = KE_BYTES_PER_ELEMENT
+    del Ke
+    torch.cuda.empty_cache()
+    return out
+
+
 def jacobi_to_tol(system, tol: float, max_total: int) -> dict:
     """FP64 Jacobi-PCG on the device (the reference's algorithm), restarted on
     the true FP64 residual until ||f - K u|| <= tol ||f|| or max_total
@@ -144,6 +197,37 @@ def _oracle_pcg_to_tol(O, K, F, tol: float, max_total: int):
         if rel <= tol or total >= max_total or rel > 0.5 * prev:
             break
     return x, total, rel
+
+
+def oracle_check(model, U, mesh=None, owner=None, n_groups: int = 0) -> dict:
+    """The CPU FP64 oracle's check of device solutions of one C3D10 model:
+    U holds one displacement vector (3 DOF per node, the model's node order)
+    or one per column; the relative residual of each on the oracle's own K and
+    CPU SpMV (with ``owner``, the node -> mesh map of a merged model, per
+    mesh), and the device cell / nodal von Mises fields of the first column
+    against the oracle's stress field of the same displacements."""
+    import torch
+    from oracle import oracle as O
+    from wnfea_amd.solver.solid_solver import compute_c3d10_stress_field
+    E, nu = model.element_moduli()
+    K, F = O.assemble_csr(model.mesh_nodes, model.solid_elements, E, nu, model.fixed_dof_mask(3),
+                          model.load_vector(3))
+    Uh = U.detach().cpu().numpy() if isinstance(U, torch.Tensor) else np.asarray(U)
+    cols = Uh.reshape(len(F), -1)
+    out = {}
+    if owner is not None:
+        r = F - O.csr_spmv(K, cols[:, 0])
+        own3 = np.repeat(np.asarray(owner), 3)
+        r2 = np.bincount(own3, weights=r * r, minlength=n_groups)
+        f2 = np.bincount(own3, weights=F * F, minlength=n_groups)
+        rel = np.sqrt(r2) / np.where(f2 > 0, np.sqrt(f2), 1.0)
+        out["rel_res_on_oracle_K_max_over_meshes"] = float(rel.max())
+    out["cols"] = cols.shape[1]
+    _, cv, nv = compute_c3d10_stress_field(model, torch.as_tensor(cols[:, 0].copy()), mesh)
+    _, rcv, rnv = O.stress_field(model.mesh_nodes, model.solid_elements, E, nu, cols[:, 0])
+    out["von_mises_kernel_vs_oracle_same_u_max_rel"] = float(max(np.abs(nv - rnv).max() / np.abs(rnv).max(),
+                                                                  np.abs(cv - rcv).max() / np.abs(rcv).max()))
+    return out, K, F
 
 
 def cpu_baseline(model, u_mixed, u_ref, vm_device, gpu_jacobi_iters: int | None, sample_iters: int = 60):
@@ -321,6 +405,9 @@ def run_batch(args, log) -> int:
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / args.steps
     log(f"[bench] batched solve {t_batch * 1e3:.1f} ms, {iters} iterations, max per-mesh rel {per.max():.2e}")
+    chk, _, _ = oracle_check(bs.merged, u, bs.system.mesh, owner=bs.owner_host, n_groups=bs.n_meshes)
+    chk.pop("cols")
+    log(f"[bench] oracle check {chk}")
     bs.close()
     # the same meshes one by one (each its own system, hierarchy and matrix-free
     # operator, as in the batch); solve time only
@@ -355,7 +442,7 @@ def run_batch(args, log) -> int:
                    "parallelism": "single-gpu"},
         "dof_per_s": round(n_dofs / t_batch, 1), "iterations": iters, "max_per_mesh_rel_res": float(per.max()),
         "one_by_one_seconds": round(t_seq, 5), "speedup_batched_vs_one_by_one": round(t_seq / t_batch, 2),
-        "setup": bs.timings}), flush=True)
+        "err_vs_oracle": chk, "setup": bs.timings}), flush=True)
     return 0
 
 
@@ -394,6 +481,12 @@ def run_adjoint(args, log) -> int:
     t_t = time.perf_counter() - t0
     log(f"[bench] compliance {C:.6e} + gradient {t_c * 1e3:.1f} ms; tip {T:.6e} + gradient {t_t * 1e3:.1f} ms")
     uh, lh = u.cpu().numpy(), lam.cpu().numpy()
+    chk, Ko, Fo = oracle_check(model, u, prob.system.mesh)
+    chk.pop("cols")
+    chk["rel_res_forward_on_oracle_K"] = float(np.linalg.norm(Fo - O.csr_spmv(Ko, uh)) / np.linalg.norm(Fo))
+    chk["rel_res_adjoint_on_oracle_K"] = float(np.linalg.norm(c - O.csr_spmv(Ko, lh)) / np.linalg.norm(c))
+    del Ko
+    log(f"[bench] oracle check {chk}")
     t0 = time.perf_counter()
     refC = O.adjoint_grad(model.mesh_nodes, model.solid_elements, E, model.solid_nu, uh, uh)
     refT = O.adjoint_grad(model.mesh_nodes, model.solid_elements, E, model.solid_nu, uh, lh)
@@ -403,15 +496,12 @@ def run_adjoint(args, log) -> int:
     errT = float(np.linalg.norm(gTh - refT) / np.linalg.norm(refT))
     rng = np.random.default_rng(3)
     fd_err = []
+    t0 = time.perf_counter()
     for e in rng.choice(M, 20, replace=False):
-        h = 1e-4 * E[e]
-        vals = []
-        for sgn in (1.0, -1.0):
-            Ep = E.copy()
-            Ep[e] += sgn * h
-            vals.append(prob.compliance_and_grad(Ep)[0])
-        fd = (vals[0] - vals[1]) / (2 * h)
+        # central difference of C from forward solves of K(E +- h e_e) (FP64 floor)
+        fd = prob.compliance_central_difference(E, int(e), 1e-4 * E[e])
         fd_err.append((abs(fd - gCh[e]) / abs(gCh[e]), abs(fd - gCh[e]) / np.abs(gCh).max(), abs(gCh[e])))
+    t_fd = time.perf_counter() - t0
     prob.close()
     fd_rel = [a for a, _, _ in fd_err]
     fd_scaled = [b for _, b, _ in fd_err]
@@ -429,9 +519,10 @@ def run_adjoint(args, log) -> int:
         "grad_rel_err_vs_cpu_fp64": {"compliance": errC, "tip": errT},
         "fd_20_elements": {"max_err_over_max_abs_grad": float(max(fd_scaled)),
                            "median_rel_err": float(np.median(fd_rel)), "max_rel_err": float(max(fd_rel)),
-                           "note": "per-element relative error grows where |g_e| is small: the FD difference "
-                                   "of C is then near FP64 round-off of C"},
-        "cpu_reference_gradient_seconds": round(t_cpu, 3),
+                           "h": "1e-4 E_e", "seconds": round(t_fd, 2),
+                           "method": "C(E+h)-C(E-h) = f.(d+ - d-) with u+- = u0 + d+-, each d solved on the "
+                                     "device-assembled perturbed system to the FP64 floor"},
+        "cpu_reference_gradient_seconds": round(t_cpu, 3), "err_vs_oracle": chk,
         "solves": prob.solves}), flush=True)
     return 0
 
@@ -520,6 +611,20 @@ def run_multirhs(args, log) -> int:
     amg.close()
     op.close()
     log(f"[bench] 32 single-column solves (matrix-free operators): {t_single * 1e3:.1f} ms; max rel diff {err:.1e}")
+    chk, Ko, _ = oracle_check(model, U[:, :1], system.mesh)
+    from oracle import oracle as O
+    Uh = U.cpu().numpy()
+    rel_cols = [float(np.linalg.norm(B[:, k] - O.csr_spmv(Ko, Uh[:, k])) / np.linalg.norm(B[:, k])) for k in range(32)]
+    chk.pop("cols")
+    chk["rel_res_on_oracle_K_max_over_cases"] = max(rel_cols)
+    kmax = int(np.argmax(rel_cols))
+    # the FP64 rounding floor of each case's residual, eps * || |K| |u_k| || / ||b_k||
+    absK = abs(Ko)
+    floor = [float(2.2e-16 * np.linalg.norm(absK @ np.abs(Uh[:, k])) / np.linalg.norm(B[:, k])) for k in range(32)]
+    chk["worst_case"] = {"case": kmax, "rel_res_device": float(rels[kmax]), "rel_res_oracle": rel_cols[kmax],
+                         "fp64_floor_estimate": floor[kmax]}
+    chk["fp64_floor_estimate_max_over_cases"] = max(floor)
+    log(f"[bench] oracle check {chk}")
     print(json.dumps({
         "metric": "s to rel-res 1e-8 for 32 load cases on the C3D10 I-beam (multi-RHS)",
         "value": round(t_multi, 5), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -530,7 +635,7 @@ def run_multirhs(args, log) -> int:
         "seconds_per_load_case": round(t_multi / 32, 6), "single_column_seconds_total": round(t_single, 5),
         "speedup_vs_single_column": round(t_single / t_multi, 2), "iterations_min_max": [int(iters.min()),
                                                                                         int(iters.max())],
-        "max_rel_res": float(rels.max()), "max_rel_diff_vs_single": err}), flush=True)
+        "max_rel_res": float(rels.max()), "max_rel_diff_vs_single": err, "err_vs_oracle": chk}), flush=True)
     return 0
 
 
@@ -586,6 +691,11 @@ def run_beams(args, log) -> int:
     t_cpu = time.perf_counter() - t0
     Fg = bs.internal_forces(U).cpu().numpy()
     err = float(np.linalg.norm(Fg - Fo) / np.linalg.norm(Fo))
+    # equilibrium of the final state with the oracle's forces (full load)
+    Fext = model.load_vector(6)
+    free = ~model.fixed_dof_mask(6)
+    eq_res = float(np.linalg.norm((Fext - Fo)[free]) / np.linalg.norm(Fext[free]))
+    log(f"[bench] final state: ||F_ext - f_int_oracle(U)|| / ||F_ext|| on free DOFs {eq_res:.2e}")
     bs.close()
     log(f"[bench] force kernel {t_f * 1e6:.1f} us ({M / t_f / 1e6:.0f} M elem/s), numpy host {t_cpu * 1e3:.1f} ms, "
         f"rel diff {err:.1e}")
@@ -600,6 +710,8 @@ def run_beams(args, log) -> int:
                    "parallelism": "single-gpu"},
         "newton_iters_per_step": st["newton_iters"], "pcg_iters": st["pcg_iters"], "tangents": st["tangents"],
         "final_residual": st["final_residual"], "setup_sec": round(t_setup, 2),
+        "err_vs_oracle": {"equilibrium_rel_res_with_oracle_forces": eq_res,
+                          "force_kernel_rel_diff_vs_oracle": err},
         "cycle_tune": {k: v for k, v in (getattr(bs, "cycle_tune", None) or {}).items() if k != "table"},
         "force_kernel": {"seconds": t_f, "elements_per_s": M / t_f, "gbs": bytes_f / t_f / 1e9,
                          "bytes_per_call": bytes_f, "rel_diff_vs_oracle": err},
@@ -645,6 +757,8 @@ def main() -> int:
                     help="time the matrix-free operator and V-cycle kernels alone (for counter runs)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only the SpMV launches (and a 256 MiB copy for counter calibration), for rocprofv3 --pmc")
+    ap.add_argument("--ke-only", action="store_true",
+                    help="only the element stiffness kernels (both variants) on the I-beam, for rocprofv3")
     ap.add_argument("--cheb-only", action="store_true",
                     help="only the level-1 Chebyshev smoothing steps (FP32 then FP64) and a 256 MiB copy, "
                          "for rocprofv3 --pmc")
@@ -692,6 +806,13 @@ def main() -> int:
     t_asm = time.perf_counter() - t0
     info = system.K.info()
     log(f"[bench] rcm {t_rcm:.1f}s assembly {t_asm:.1f}s nnzb {info['nnzb']} slots {info['slots']}")
+    ke = None
+    if args.config == "ibeam" and not (args.spmv_only or args.mf_only or args.cheb_only):
+        ke = time_ke(system, 5)
+        log(f"[bench] element stiffness kernel {ke}")
+    if args.ke_only:
+        print(json.dumps(time_ke(system, 10)), flush=True)
+        return 0
     if args.no_autotune and not args.spmv_shape:
         args.spmv_shape = "1,1,1"
     tunes = tune_spmv(system.K, args.spmv_shape, log)
@@ -992,6 +1113,7 @@ def main() -> int:
         "spmv_tune": tunes,
         "cycle_tune": cycle_tune,
         "matrix_free_operator": mf_info,
+        "element_stiffness_kernel": ke,
         "setup": {"mesh_sec": round(t_mesh, 2), "rcm_sec": round(t_rcm, 2), "assembly_sec": round(t_asm, 2),
                   "amg_setup_sec": round(t_amg, 2), "amg": st},
         "cpu_baseline": cpu,

@@ -44,6 +44,14 @@ int wnfea_device_info(long long *out);
 int wnfea_c3d10_ke(int M, const int *conn, const double *nodes, const double *E, const double *nu, double *Ke,
                    int *bad, void *stream);
 
+/* Kernel behind wnfea_c3d10_ke for this process: 0 (default) the vector-ALU
+ * contraction (one lane per upper 3x3 block pair), 1 the matrix-core
+ * contraction Ke = B^T (w D B) with v_mfma_f64_16x16x4f64.  Both give Ke to
+ * FP64 rounding and store it as contiguous 16-byte chunks.  No reference
+ * counterpart (a kernel choice, as the reference's launch-configuration
+ * helper get_optimal_launch_config, wnfea/solver/fast_kernels.py:243-260). */
+int wnfea_c3d10_ke_variant(int variant);
+
 /* Matrix-free internal forces F = sum_e K_e u_e, FP64, deterministic.
  * U and F have `stride` (3 or 6) doubles per node (device); fe_work[M][30]
  * is device scratch; n2e_ptr/n2e is the node->(element*10+local) incidence
@@ -131,9 +139,11 @@ int wnfea_adjoint_grad(int M, const int *conn, const double *nodes, const double
 int wnfea_beam_forces(int M, int n_nodes, const int *conn, const double *nodes, const double *props, const double *U,
                       const int *n2e_ptr, const int *n2e, double *fe_work, double *F, int accumulate, void *stream);
 
-/* Element tangents Kt[M][12][12] = d f_e / d u_e by central differences of
- * the co-rotational forces (h = h_rel * L0 for translations, h_rel for
- * rotations).  Replaces the finite-difference Jacobian action of
+/* Element tangents Kt[M][12][12] = d f_e / d u_e of the co-rotational
+ * forces: h_rel <= 0 gives the exact (consistent) derivative, by forward-mode
+ * differentiation of the force routine in dual numbers; h_rel > 0 central
+ * differences (h = h_rel * L0 for translations, h_rel for rotations).
+ * Replaces the finite-difference Jacobian action of
  * MatrixFreeJFNKOperator._matvec (wnfea/solver/jfnk_operator.py:98-134) with
  * an assembled tangent. */
 int wnfea_beam_tangent(int M, const int *conn, const double *nodes, const double *props, const double *U,
@@ -188,6 +198,36 @@ long long wnfea_sa_aggregate(int n, const int *ptr, const int *ind, int *agg);
  * (wnfea/solver/amg_preconditioner.py:241-281). */
 int wnfea_sa_tentative(int n_nodes, int b, int k, const int *agg, int n_agg, const double *B, int *p_rowptr,
                        int *p_colidx, double *p_vals, double *Bc);
+
+/* Host block-sparse kernels of the smoothed-aggregation set-up (host
+ * pointers, host threads; `threads` 0 = one per hardware thread).  They
+ * replace the scipy sparse products, transposes and matrix-vector products
+ * of the reference's setup (wnfea/solver/amg_preconditioner.py:241-293,
+ * its Galerkin product R A P at :283-293).
+ *
+ * C = A B (block CSR, row-major blocks ra x ca and ca x cb, nc block
+ * columns of B).  With c_idx == NULL: structure pass, c_ptr[0..nr] gets the
+ * prefix sums of C's row block counts and the total is returned; then call
+ * again with c_idx / c_val of that size.  Columns come out sorted; every
+ * block is summed in a fixed order (independent of the thread count). */
+long long wnfea_host_bsr_spgemm(int nr, int ra, int ca, int cb, int nc, const int *a_ptr, const int *a_idx,
+                                const double *a_val, const int *b_ptr, const int *b_idx, const double *b_val,
+                                int *c_ptr, int *c_idx, double *c_val, int threads);
+
+/* T = A^T (t_ptr[nc + 1], t_idx / t_val[nnzb], blocks transposed). */
+int wnfea_host_bsr_transpose(int nr, int nc, int ra, int ca, const int *a_ptr, const int *a_idx,
+                             const double *a_val, int *t_ptr, int *t_idx, double *t_val);
+
+/* In place A = (A + A^T) / 2 for a square block CSR with a structurally
+ * symmetric pattern and sorted columns; zero diagonal entries become 1 (the
+ * reference's fix of vanished coarse DOFs, amg_preconditioner.py:288-293).
+ * Returns the number of diagonal entries set, or -1 if a mirror block is
+ * missing. */
+long long wnfea_host_bsr_symmetrize(int n, int b, const int *ptr, const int *idx, double *val, int threads);
+
+/* y = A x for a square block CSR (b <= 8) on host threads. */
+int wnfea_host_bsr_spmv(int n, int b, const int *ptr, const int *idx, const double *val, const double *x, double *y,
+                        int threads);
 
 /* ---------------------------------------------------------------- block matrices */
 

@@ -56,6 +56,39 @@ def test_tangent_and_linear_stiffness_match_oracle(lattice):
     assert abs(Al - Alo).max() <= 1e-12 * abs(Alo).max()
 
 
+def test_consistent_tangent_is_the_derivative_of_the_forces(lattice):
+    """The default element tangent (h_rel = 0) is the exact derivative of the
+    co-rotational forces: at a deformed state it equals the oracle's central
+    differences extrapolated to h -> 0 (Richardson on h = 1e-4 and 5e-5) to
+    1e-8, and at U = 0 (no axial force, so no geometric stiffness) the
+    reference's linear stiffness of every member whose frame both codes build
+    alike (+x and +z chords, right-handed) to round-off."""
+    import torch
+    from oracle import beam_oracle as B
+    from wnfea_amd.solver.beam_solver import BeamSystem
+    model, props = lattice
+    bs = BeamSystem(model)
+    assert bs.h_rel == 0.0
+    U = 1e-3 * np.random.default_rng(2).standard_normal(6 * len(model.mesh_nodes))
+    Kt = bs.element_tangents(torch.as_tensor(U, device="cuda")).cpu().numpy()
+    K1 = B.fd_tangent(model.mesh_nodes, model.mesh_elements, props, U, h_rel=1e-4)
+    K2 = B.fd_tangent(model.mesh_nodes, model.mesh_elements, props, U, h_rel=5e-5)
+    R = (4.0 * K2 - K1) / 3.0
+    assert np.abs(Kt - R).max() <= 1e-8 * np.abs(R).max()
+    K0 = bs.element_tangents(torch.zeros(6 * len(model.mesh_nodes), dtype=torch.float64, device="cuda")).cpu().numpy()
+    conn = np.asarray(model.mesh_elements)
+    d = model.mesh_nodes[conn[:, 1]] - model.mesh_nodes[conn[:, 0]]
+    sel = ((d[:, 0] > 0) & (np.abs(d[:, 1]) < 1e-12) & (np.abs(d[:, 2]) < 1e-12)) | \
+          ((d[:, 2] > 0) & (np.abs(d[:, 0]) < 1e-12) & (np.abs(d[:, 1]) < 1e-12))
+    assert sel.sum() > len(conn) // 3
+    Kl = B.ke_linear(model.mesh_nodes, conn[sel], props[sel])
+    assert np.abs(K0[sel] - Kl).max() <= 1e-12 * np.abs(Kl).max()
+    # central differences stay available (h_rel > 0) and agree to O(h^2)
+    bs.h_rel = 1e-6
+    Kfd = bs.element_tangents(torch.as_tensor(U, device="cuda")).cpu().numpy()
+    assert np.abs(Kfd - Kt).max() <= 1e-6 * np.abs(Kt).max()
+
+
 def test_newton_matches_oracle(lattice):
     from oracle import beam_oracle as B
     from wnfea_amd.solver.beam_solver import BeamSystem

@@ -36,6 +36,38 @@ def test_ke_matches_oracle():
     assert np.allclose(Ke, Kb[:3], rtol=0, atol=0)
 
 
+def test_ke_matrix_core_variant_matches_oracle():
+    """The matrix-core Ke kernel (v_mfma_f64_16x16x4f64 contraction) gives the
+    oracle's Ke to FP64 rounding, exactly symmetric, like the vector-ALU
+    kernel; the process default is restored afterwards."""
+    from oracle import oracle as O
+    from wnfea_amd import _lib
+    from wnfea_amd.elements import c3d10
+    L = _lib.load()
+    X = _random_tets(320, seed=7)
+    # the midside jitter inverts the odd small element at a Gauss point: keep
+    # the valid ones, 301 of them (not a multiple of the 4 elements per block)
+    a, b = (5 - 5 ** 0.5) / 20, (5 + 3 * 5 ** 0.5) / 20
+    gauss = [(a, a, a), (b, a, a), (a, b, a), (a, a, b)]
+    ok = [i for i in range(len(X)) if min(np.linalg.det(O.shape_derivs(*p)[1] @ X[i]) for p in gauss) > 0]
+    X = X[ok[:301]]
+    assert len(X) == 301
+    E = np.random.default_rng(8).uniform(1e9, 3e11, len(X))
+    nu = np.random.default_rng(9).uniform(0.1, 0.45, len(X))
+    K0 = c3d10.element_stiffness_c3d10(X, E, nu)
+    try:
+        assert L.wnfea_c3d10_ke_variant(1) == 0
+        K1 = c3d10.element_stiffness_c3d10(X, E, nu)
+    finally:
+        assert L.wnfea_c3d10_ke_variant(0) == 0
+    assert L.wnfea_c3d10_ke_variant(2) < 0
+    for i in range(len(X)):
+        Ko = O.element_stiffness(X[i], E[i], nu[i])
+        assert np.abs(K1[i] - Ko).max() <= 1e-12 * np.abs(Ko).max(), i
+        assert np.array_equal(K1[i], K1[i].T)
+    assert np.abs(K1 - K0).max() <= 1e-12 * np.abs(K0).max()
+
+
 def test_fint_and_stress_match_oracle():
     from oracle import oracle as O
     from wnfea_amd.elements import c3d10

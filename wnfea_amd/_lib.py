@@ -28,6 +28,7 @@ SIGNATURES = {
     "wnfea_abi_version": (_i, []),
     "wnfea_device_info": (_i, [_pll]),
     "wnfea_c3d10_ke": (_i, [_i, _p, _p, _p, _p, _p, _p, _p]),
+    "wnfea_c3d10_ke_variant": (_i, [_i]),
     "wnfea_c3d10_fint": (_i, [_i, _i, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p]),
     "wnfea_c3d10_stress": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "wnfea_adjoint_grad": (_i, [_i, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
@@ -41,6 +42,10 @@ SIGNATURES = {
     "wnfea_node_incidence": (_i, [_i, _i, _i, _p, _p, _p]),
     "wnfea_sa_aggregate": (_ll, [_i, _p, _p, _p]),
     "wnfea_sa_tentative": (_i, [_i, _i, _i, _p, _i, _p, _p, _p, _p, _p]),
+    "wnfea_host_bsr_spgemm": (_ll, [_i, _i, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i]),
+    "wnfea_host_bsr_transpose": (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
+    "wnfea_host_bsr_symmetrize": (_ll, [_i, _i, _p, _p, _p, _i]),
+    "wnfea_host_bsr_spmv": (_i, [_i, _i, _p, _p, _p, _p, _p, _i]),
     "wnfea_bsr_create": (_p, [_i, _i, _i, _i, _p, _p, _i, _i]),
     "wnfea_bsr_destroy": (None, [_p]),
     "wnfea_bsr_info": (_i, [_p, _pll]),

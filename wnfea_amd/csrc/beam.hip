@@ -1,6 +1,8 @@
 // Co-rotational 3D beam kernels for gfx950 (FP64): internal forces, the
-// element tangent by central differences of those forces, and the linear
-// 12x12 beam stiffness.  6 DOF per node [ux uy uz rx ry rz].
+// consistent element tangent (the exact derivative of those forces, by
+// forward-mode differentiation with dual numbers; central differences on
+// request), and the linear 12x12 beam stiffness.  6 DOF per node
+// [ux uy uz rx ry rz].
 //
 // Internal forces follow the vectorised co-rotational formulation of
 // wnfea/solver/fast_kernels.py:451-537 (the path the reference's solver runs):
@@ -18,22 +20,62 @@
 // The linear stiffness follows build_element_stiffness_3d_beam and
 // build_transformation_matrix (wnfea/solver/assembler.py:13-148), including
 // its axis-aligned frame rules.
+#include <type_traits>
 #include "common.hpp"
 
 namespace wn {
 
-struct V3 {
-  double x, y, z;
+using ::cos;
+using ::sin;
+using ::sqrt;
+
+// Forward-mode dual number: a value and its derivative along one direction.
+struct Dual {
+  double v, d;
+  __host__ __device__ Dual() : v(0.0), d(0.0) {}
+  __host__ __device__ Dual(double a) : v(a), d(0.0) {}
+  __host__ __device__ Dual(double a, double b) : v(a), d(b) {}
 };
-__device__ __forceinline__ V3 v3(double a, double b, double c) { return V3{a, b, c}; }
-__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ V3 operator*(double s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
-__device__ __forceinline__ double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V3 cross(V3 a, V3 b) {
-  return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+__host__ __device__ __forceinline__ Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__host__ __device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__host__ __device__ __forceinline__ Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__host__ __device__ __forceinline__ Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__host__ __device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
 }
-__device__ __forceinline__ double norm(V3 a) { return sqrt(dot(a, a)); }
+__host__ __device__ __forceinline__ Dual sqrt(Dual a) {
+  const double r = ::sqrt(a.v);
+  return Dual(r, a.d / (2.0 * r));
+}
+__host__ __device__ __forceinline__ Dual sin(Dual a) { return Dual(::sin(a.v), ::cos(a.v) * a.d); }
+__host__ __device__ __forceinline__ Dual cos(Dual a) { return Dual(::cos(a.v), -::sin(a.v) * a.d); }
+__host__ __device__ __forceinline__ double val(double a) { return a; }
+__host__ __device__ __forceinline__ double val(Dual a) { return a.v; }
+
+template <typename S>
+struct V3T {
+  S x, y, z;
+};
+using V3 = V3T<double>;
+template <typename S>
+__host__ __device__ __forceinline__ V3T<S> v3(S a, S b, S c) { return V3T<S>{a, b, c}; }
+template <typename S>
+__host__ __device__ __forceinline__ V3T<S> lift(V3 a) { return V3T<S>{S(a.x), S(a.y), S(a.z)}; }
+template <typename S>
+__host__ __device__ __forceinline__ V3T<S> operator+(V3T<S> a, V3T<S> b) { return v3<S>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <typename S>
+__host__ __device__ __forceinline__ V3T<S> operator-(V3T<S> a, V3T<S> b) { return v3<S>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename S>
+__host__ __device__ __forceinline__ V3T<S> operator*(S s, V3T<S> a) { return v3<S>(s * a.x, s * a.y, s * a.z); }
+template <typename S>
+__host__ __device__ __forceinline__ S dot(V3T<S> a, V3T<S> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename S>
+__host__ __device__ __forceinline__ V3T<S> cross(V3T<S> a, V3T<S> b) {
+  return v3<S>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <typename S>
+__host__ __device__ __forceinline__ S norm(V3T<S> a) { return sqrt(dot(a, a)); }
 
 struct BeamGeom {
   V3 X1, X2;
@@ -44,8 +86,8 @@ __device__ __forceinline__ BeamGeom load_geom(int e, const int* __restrict__ con
                                               const double* __restrict__ props) {
   const int n1 = conn[2 * e], n2 = conn[2 * e + 1];
   BeamGeom g;
-  g.X1 = v3(nodes[3 * n1], nodes[3 * n1 + 1], nodes[3 * n1 + 2]);
-  g.X2 = v3(nodes[3 * n2], nodes[3 * n2 + 1], nodes[3 * n2 + 2]);
+  g.X1 = v3<double>(nodes[3 * n1], nodes[3 * n1 + 1], nodes[3 * n1 + 2]);
+  g.X2 = v3<double>(nodes[3 * n2], nodes[3 * n2 + 1], nodes[3 * n2 + 2]);
   const double* p = props + 6 * (size_t)e;
   g.E = p[0];
   g.G = p[1];
@@ -57,58 +99,77 @@ __device__ __forceinline__ BeamGeom load_geom(int e, const int* __restrict__ con
 }
 
 // f[12] = B^T q of one element for the element displacement u[12]
-// ([u1 theta1 u2 theta2]).
-__device__ void corot_force(const BeamGeom& g, const double* u, double* f) {
+// ([u1 theta1 u2 theta2]).  S = double: the reference's forces.  S = Dual:
+// the same forces and their derivative along the direction carried by u's
+// dual parts.  The rotation taking e1 to r1 is the reference's Rodrigues
+// form, cos(phi) e2 + sin(phi) (a x e2) + (a.e2)(1 - cos(phi)) a with
+// a = (e1 x r1)/|e1 x r1| and phi = atan2(|e1 x r1|, e1.r1), skipped when
+// |e1 x r1| <= 1e-12.  With c = e1 x r1, |c| = sin(phi), e1.r1 = cos(phi)
+// (unit vectors) it equals cos(phi) e2 + c x e2 + (c.e2)/(1 + cos(phi)) c,
+// which is smooth through c = 0: the differentiated form uses it, so the
+// tangent at an unrotated chord is the derivative of the rotation there
+// (the limit the reference's branch approaches), not the zero derivative
+// of its skipped branch.
+template <typename S>
+__host__ __device__ void corot_force(const BeamGeom& g, const S* u, S* f) {
   const V3 v0 = g.X2 - g.X1;
   const double L0 = norm(v0);
   const V3 e1 = (1.0 / fmax(L0, 1e-15)) * v0;
-  const V3 ref = fabs(e1.z) < 0.9 ? v3(0.0, 0.0, 1.0) : v3(0.0, 1.0, 0.0);
-  V3 e2 = cross(ref, e1);
-  e2 = (1.0 / norm(e2)) * e2;
-  V3 e3 = cross(e1, e2);
-  e3 = (1.0 / norm(e3)) * e3;
-  const V3 u1 = v3(u[0], u[1], u[2]), th1 = v3(u[3], u[4], u[5]);
-  const V3 u2 = v3(u[6], u[7], u[8]), th2 = v3(u[9], u[10], u[11]);
-  const V3 du = u2 - u1;
-  const V3 v = v0 + du;
-  const double L = norm(v);
-  V3 r1 = (1.0 / fmax(L, 1e-15)) * v;
-  const V3 cr = cross(e1, r1);
-  const double sphi = norm(cr), cphi = dot(e1, r1);
-  V3 r2 = e2, r3 = e3;
-  if (sphi > 1e-12) {
-    const V3 ax = (1.0 / sphi) * cr;
-    const double phi = atan2(sphi, cphi);
-    const double cp = cos(phi), sp = sin(phi);
-    r2 = cp * e2 + sp * cross(ax, e2) + (dot(ax, e2) * (1.0 - cp)) * ax;
-    r3 = cp * e3 + sp * cross(ax, e3) + (dot(ax, e3) * (1.0 - cp)) * ax;
+  const V3 ref = fabs(e1.z) < 0.9 ? v3<double>(0.0, 0.0, 1.0) : v3<double>(0.0, 1.0, 0.0);
+  V3 e2d = cross(ref, e1);
+  e2d = (1.0 / norm(e2d)) * e2d;
+  V3 e3d = cross(e1, e2d);
+  e3d = (1.0 / norm(e3d)) * e3d;
+  const V3T<S> e2 = lift<S>(e2d), e3 = lift<S>(e3d), E1 = lift<S>(e1);
+  const V3T<S> u1 = v3<S>(u[0], u[1], u[2]), th1 = v3<S>(u[3], u[4], u[5]);
+  const V3T<S> u2 = v3<S>(u[6], u[7], u[8]), th2 = v3<S>(u[9], u[10], u[11]);
+  const V3T<S> du = u2 - u1;
+  const V3T<S> v = lift<S>(v0) + du;
+  const S L = norm(v);
+  V3T<S> r1 = (S(1.0) / (val(L) > 1e-15 ? L : S(1e-15))) * v;
+  const V3T<S> cr = cross(E1, r1);
+  const S cphi = dot(E1, r1);
+  V3T<S> r2 = e2, r3 = e3;
+  if constexpr (std::is_same<S, double>::value) {
+    const double sphi = norm(cr);
+    if (sphi > 1e-12) {
+      const V3 ax = (1.0 / sphi) * cr;
+      const double phi = atan2(sphi, cphi);
+      const double cp = cos(phi), sp = sin(phi);
+      r2 = cp * e2 + sp * cross(ax, e2) + (dot(ax, e2) * (1.0 - cp)) * ax;
+      r3 = cp * e3 + sp * cross(ax, e3) + (dot(ax, e3) * (1.0 - cp)) * ax;
+    }
+  } else {
+    const S k = S(1.0) / (S(1.0) + cphi);
+    r2 = cphi * e2 + cross(cr, e2) + (dot(cr, e2) * k) * cr;
+    r3 = cphi * e3 + cross(cr, e3) + (dot(cr, e3) * k) * cr;
   }
-  const double tw = 0.5 * dot(th1 + th2, r1);
-  const double ct = cos(tw), st = sin(tw);
-  const V3 r2t = ct * r2 + st * cross(r1, r2);
-  const V3 r3t = ct * r3 + st * cross(r1, r3);
-  r1 = (1.0 / norm(r1)) * r1;
+  const S tw = S(0.5) * dot(th1 + th2, r1);
+  const S ct = cos(tw), st = sin(tw);
+  const V3T<S> r2t = ct * r2 + st * cross(r1, r2);
+  r1 = (S(1.0) / norm(r1)) * r1;
   r2 = r2t - dot(r2t, r1) * r1;
-  r2 = (1.0 / norm(r2)) * r2;
-  r3 = cross(r1, r2);
-  r3 = (1.0 / norm(r3)) * r3;
-  (void)r3t;  // the reference recomputes r3 from r1 x r2 after the twist
-  const double ul = (2.0 * dot(v0, du) + dot(du, du)) / (L + L0);
-  const double thy1 = dot(r2, th1) + dot(r3, du) / L;
-  const double thz1 = dot(r3, th1) - dot(r2, du) / L;
-  const double thx = dot(r1, th2 - th1);
-  const double thy2 = dot(r2, th2) + dot(r3, du) / L;
-  const double thz2 = dot(r3, th2) - dot(r2, du) / L;
-  const double N = (g.E * g.A / L0) * ul;
-  const double My1 = (g.E * g.Iy / L0) * (4.0 * thy1 + 2.0 * thy2);
-  const double My2 = (g.E * g.Iy / L0) * (2.0 * thy1 + 4.0 * thy2);
-  const double Mz1 = (g.E * g.Iz / L0) * (4.0 * thz1 + 2.0 * thz2);
-  const double Mz2 = (g.E * g.Iz / L0) * (2.0 * thz1 + 4.0 * thz2);
-  const double Tx = (g.G * g.J / L0) * thx;
-  const double iL = 1.0 / L;
-  const V3 f1 = (-N) * r1 - ((My1 + My2) * iL) * r3 + ((Mz1 + Mz2) * iL) * r2;
-  const V3 m1 = (-Tx) * r1 + My1 * r2 + Mz1 * r3;
-  const V3 m2 = Tx * r1 + My2 * r2 + Mz2 * r3;
+  r2 = (S(1.0) / norm(r2)) * r2;
+  r3 = cross(r1, r2);  // the reference recomputes r3 from r1 x r2 after the twist
+  r3 = (S(1.0) / norm(r3)) * r3;
+  const V3T<S> V0 = lift<S>(v0);
+  const S ul = (S(2.0) * dot(V0, du) + dot(du, du)) / (L + S(L0));
+  const S iL = S(1.0) / L;
+  const S thy1 = dot(r2, th1) + dot(r3, du) * iL;
+  const S thz1 = dot(r3, th1) - dot(r2, du) * iL;
+  const S thx = dot(r1, th2 - th1);
+  const S thy2 = dot(r2, th2) + dot(r3, du) * iL;
+  const S thz2 = dot(r3, th2) - dot(r2, du) * iL;
+  const S N = S(g.E * g.A / L0) * ul;
+  const S ky = S(g.E * g.Iy / L0), kz = S(g.E * g.Iz / L0);
+  const S My1 = ky * (S(4.0) * thy1 + S(2.0) * thy2);
+  const S My2 = ky * (S(2.0) * thy1 + S(4.0) * thy2);
+  const S Mz1 = kz * (S(4.0) * thz1 + S(2.0) * thz2);
+  const S Mz2 = kz * (S(2.0) * thz1 + S(4.0) * thz2);
+  const S Tx = S(g.G * g.J / L0) * thx;
+  const V3T<S> f1 = (-N) * r1 - ((My1 + My2) * iL) * r3 + ((Mz1 + Mz2) * iL) * r2;
+  const V3T<S> m1 = (-Tx) * r1 + My1 * r2 + Mz1 * r3;
+  const V3T<S> m2 = Tx * r1 + My2 * r2 + Mz2 * r3;
   f[0] = f1.x;
   f[1] = f1.y;
   f[2] = f1.z;
@@ -140,14 +201,16 @@ __global__ void beam_force_kernel(int M, const int* __restrict__ conn, const dou
   const BeamGeom g = load_geom(e, conn, nodes, props);
   double u[12], f[12];
   load_u(e, conn, U, u);
-  corot_force(g, u, f);
+  corot_force<double>(g, u, f);
 #pragma unroll
   for (int k = 0; k < 12; ++k) fe[12 * (size_t)e + k] = f[k];
 }
 
-// Column j of the element tangent d f / d u_j by central differences:
+// Column j of the element tangent d f / d u_j, one thread per (element,
+// column).  h_rel <= 0: the exact derivative (forces evaluated once in dual
+// numbers seeded with e_j).  h_rel > 0: central differences
 // Kt[e][i][j] = (f_i(u + h e_j) - f_i(u - h e_j)) / 2h, h = h_rel * L0 for
-// translations and h_rel for rotations.  One thread per (element, column).
+// translations and h_rel for rotations.
 __global__ void beam_tangent_kernel(int M, const int* __restrict__ conn, const double* __restrict__ nodes,
                                     const double* __restrict__ props, const double* __restrict__ U, double h_rel,
                                     double* __restrict__ Kt) {
@@ -155,17 +218,27 @@ __global__ void beam_tangent_kernel(int M, const int* __restrict__ conn, const d
   if (t >= 12LL * M) return;
   const int e = (int)(t / 12), j = (int)(t - 12LL * e);
   const BeamGeom g = load_geom(e, conn, nodes, props);
-  double u[12], fp[12], fm[12];
+  double u[12];
   load_u(e, conn, U, u);
+  double* out = Kt + 144 * (size_t)e;
+  if (h_rel <= 0.0) {
+    Dual ud[12], fd[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) ud[k] = Dual(u[k], k == j ? 1.0 : 0.0);
+    corot_force<Dual>(g, ud, fd);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) out[12 * i + j] = fd[i].d;
+    return;
+  }
+  double fp[12], fm[12];
   const double L0 = norm(g.X2 - g.X1);
   const double h = ((j % 6) < 3 ? h_rel * L0 : h_rel);
   const double uj = u[j];
   u[j] = uj + h;
-  corot_force(g, u, fp);
+  corot_force<double>(g, u, fp);
   u[j] = uj - h;
-  corot_force(g, u, fm);
+  corot_force<double>(g, u, fm);
   const double inv = 0.5 / h;
-  double* out = Kt + 144 * (size_t)e;
 #pragma unroll
   for (int i = 0; i < 12; ++i) out[12 * i + j] = (fp[i] - fm[i]) * inv;
 }

@@ -22,6 +22,14 @@ long long contributions(int n_nodes, int M, int npe, const int* conn, const int*
                         long long* cptr, int* codes, long long cap);
 void node_incidence(int n_nodes, int M, int npe, const int* conn, int* ptr, int* codes);
 long long sa_aggregate(int n, const int* ptr, const int* ind, int* agg);
+long long bsr_spgemm(int nr, int ra, int ca, int cb, int nc, const int* a_ptr, const int* a_idx, const double* a_val,
+                     const int* b_ptr, const int* b_idx, const double* b_val, int* c_ptr, int* c_idx, double* c_val,
+                     int threads);
+void bsr_transpose(int nr, int nc, int ra, int ca, const int* a_ptr, const int* a_idx, const double* a_val,
+                   int* t_ptr, int* t_idx, double* t_val);
+long long bsr_symmetrize(int n, int b, const int* ptr, const int* idx, double* val, int threads);
+void bsr_spmv_host(int n, int b, const int* ptr, const int* idx, const double* val, const double* x, double* y,
+                   int threads);
 int sa_tentative(int n_nodes, int b, int k, const int* agg, int n_agg, const double* B, int* p_rowptr, int* p_colidx,
                  double* p_vals, double* Bc);
 // solver.hip
@@ -99,6 +107,9 @@ int wnfea_c3d10_ke(int M, const int* conn, const double* nodes, const double* E,
   return launch_ke(M, conn, nodes, E, nu, Ke, bad, S(stream));
 }
 
+// Ke kernel of wnfea_c3d10_ke for the process: 0 vector ALU, 1 matrix cores
+int wnfea_c3d10_ke_variant(int variant) { return set_ke_variant(variant); }
+
 int wnfea_c3d10_fint(int M, int n_nodes, const int* conn, const double* nodes, const double* E, const double* nu,
                      const double* U, int stride, const int* n2e_ptr, const int* n2e, double* fe_work, double* F,
                      int accumulate, void* stream) {
@@ -142,7 +153,7 @@ int wnfea_beam_forces(int M, int n_nodes, const int* conn, const double* nodes, 
 
 int wnfea_beam_tangent(int M, const int* conn, const double* nodes, const double* props, const double* U,
                        double h_rel, double* Kt, void* stream) {
-  if (M < 0 || !(h_rel > 0.0) || (M > 0 && (!conn || !nodes || !props || !U || !Kt))) return kBadArg;
+  if (M < 0 || h_rel != h_rel || (M > 0 && (!conn || !nodes || !props || !U || !Kt))) return kBadArg;
   return launch_beam_tangent(M, conn, nodes, props, U, h_rel, Kt, S(stream));
 }
 
@@ -188,6 +199,35 @@ int wnfea_sa_tentative(int n_nodes, int b, int k, const int* agg, int n_agg, con
   if (n_nodes < 0 || b <= 0 || k <= 0 || n_agg < 0 || !agg || !B || !p_rowptr || !p_colidx || !p_vals || !Bc)
     return kBadArg;
   return sa_tentative(n_nodes, b, k, agg, n_agg, B, p_rowptr, p_colidx, p_vals, Bc);
+}
+
+// host block-sparse kernels of the smoothed-aggregation set-up (host pointers)
+long long wnfea_host_bsr_spgemm(int nr, int ra, int ca, int cb, int nc, const int* a_ptr, const int* a_idx,
+                                const double* a_val, const int* b_ptr, const int* b_idx, const double* b_val,
+                                int* c_ptr, int* c_idx, double* c_val, int threads) {
+  if (nr < 0 || nc < 0 || ra <= 0 || ca <= 0 || cb <= 0 || !a_ptr || !b_ptr || !c_ptr) return kBadArg;
+  if (c_idx && (!c_val || !a_val || !b_val)) return kBadArg;
+  return bsr_spgemm(nr, ra, ca, cb, nc, a_ptr, a_idx, a_val, b_ptr, b_idx, b_val, c_ptr, c_idx, c_val, threads);
+}
+
+int wnfea_host_bsr_transpose(int nr, int nc, int ra, int ca, const int* a_ptr, const int* a_idx,
+                             const double* a_val, int* t_ptr, int* t_idx, double* t_val) {
+  if (nr < 0 || nc < 0 || ra <= 0 || ca <= 0 || !a_ptr || !t_ptr) return kBadArg;
+  bsr_transpose(nr, nc, ra, ca, a_ptr, a_idx, a_val, t_ptr, t_idx, t_val);
+  return kOk;
+}
+
+long long wnfea_host_bsr_symmetrize(int n, int b, const int* ptr, const int* idx, double* val, int threads) {
+  if (n < 0 || b <= 0 || !ptr) return kBadArg;
+  const long long r = bsr_symmetrize(n, b, ptr, idx, val, threads);
+  return r < 0 ? kBadArg : r;
+}
+
+int wnfea_host_bsr_spmv(int n, int b, const int* ptr, const int* idx, const double* val, const double* x, double* y,
+                        int threads) {
+  if (n < 0 || b <= 0 || b > 8 || !ptr || !x || !y) return kBadArg;
+  bsr_spmv_host(n, b, ptr, idx, val, x, y, threads);
+  return kOk;
 }
 
 // ------------------------------------------------------------ block matrices

@@ -104,6 +104,7 @@ __global__ __launch_bounds__(256) void c3d10_ke_kernel(int M, const int* __restr
   __shared__ double sX[kElemPerBlock][10][3];
   __shared__ double sG[kElemPerBlock][4][10][3];  // w detJ folded in separately
   __shared__ double sW[kElemPerBlock][4];
+  __shared__ __align__(16) double sK[kElemPerBlock][900];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int e = blockIdx.x * kElemPerBlock + w;
   const bool live = e < M;
@@ -125,45 +126,202 @@ __global__ __launch_bounds__(256) void c3d10_ke_kernel(int M, const int* __restr
     }
   }
   __syncthreads();
-  if (!live || lane >= 55) return;
-  // lane -> upper-triangle pair (a, b), a <= b
-  int a = 0, rem = lane;
-  while (rem >= 10 - a) { rem -= 10 - a; ++a; }
-  const int b = a + rem;
-  const double Ev = E[e], nv = nu[e];
-  const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
-  const double lam = f * nv, mu = Ev / (2.0 * (1.0 + nv));
-  double K[3][3] = {};
+  // lane -> upper-triangle pair (a, b), a <= b; the 55 pair blocks (and their
+  // mirror images) are assembled in LDS, then the wavefront stores its
+  // element's 900 values as contiguous 16-byte chunks (a lane's own 3x3 blocks
+  // would be 24-byte pieces scattered over 30 rows)
+  if (live && lane < 55) {
+    int a = 0, rem = lane;
+    while (rem >= 10 - a) { rem -= 10 - a; ++a; }
+    const int b = a + rem;
+    const double Ev = E[e], nv = nu[e];
+    const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
+    const double lam = f * nv, mu = Ev / (2.0 * (1.0 + nv));
+    double K[3][3] = {};
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    const double wd = sW[w][g];
-    const double ga[3] = {sG[w][g][a][0], sG[w][g][a][1], sG[w][g][a][2]};
-    const double gb[3] = {sG[w][g][b][0], sG[w][g][b][1], sG[w][g][b][2]};
-    const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+    for (int g = 0; g < 4; ++g) {
+      const double wd = sW[w][g];
+      const double ga[3] = {sG[w][g][a][0], sG[w][g][a][1], sG[w][g][a][2]};
+      const double gb[3] = {sG[w][g][b][0], sG[w][g][b][1], sG[w][g][b][2]};
+      const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        double v = lam * ga[r] * gb[c] + mu * ga[c] * gb[r];
-        if (r == c) v += mu * dot;
-        K[r][c] = fma(wd, v, K[r][c]);
-      }
-  }
-  double* out = Ke + (size_t)e * 900;
-  if (a == b) {  // diagonal block: mirror so that rounding cannot break symmetry
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) out[(3 * a + r) * 30 + 3 * a + c] = 0.5 * (K[r][c] + K[c][r]);
-    return;
-  }
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      out[(3 * a + r) * 30 + 3 * b + c] = K[r][c];
-      out[(3 * b + c) * 30 + 3 * a + r] = K[r][c];
+        for (int c = 0; c < 3; ++c) {
+          double v = lam * ga[r] * gb[c] + mu * ga[c] * gb[r];
+          if (r == c) v += mu * dot;
+          K[r][c] = fma(wd, v, K[r][c]);
+        }
     }
+    double* out = sK[w];
+    if (a == b) {  // diagonal block: mirror so that rounding cannot break symmetry
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[(3 * a + r) * 30 + 3 * a + c] = 0.5 * (K[r][c] + K[c][r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          out[(3 * a + r) * 30 + 3 * b + c] = K[r][c];
+          out[(3 * b + c) * 30 + 3 * a + r] = K[r][c];
+        }
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  // 900 doubles = 450 16-byte chunks; Ke + 900 e is 16-byte aligned
+  using D2 = double __attribute__((ext_vector_type(2)));
+  const D2* src = reinterpret_cast<const D2*>(sK[w]);
+  D2* dst = reinterpret_cast<D2*>(Ke + (size_t)e * 900);
+  for (int k = lane; k < 450; k += 64) __builtin_nontemporal_store(src[k], dst + k);
+}
+
+// The same Ke with the contraction on the matrix cores:
+//   Ke = sum_g w_g B_g^T D B_g = A Bd,  A[i][k] = B_g[s][i],  Bd[k][j] = w_g (D B_g)[s][j],
+// k = 6 g + s over the 4 Gauss points and the 6 strain components (xx, yy, zz,
+// xy, yz, xz; engineering shears), i, j the 30 DOFs padded to 32.  One
+// wavefront per element: the operands A (32 x 24) and Bd (24 x 32) are built
+// in LDS by the whole wavefront, then v_mfma_f64_16x16x4f64 runs on the three
+// 16x16 tiles (0,0), (0,1), (1,1) of the upper triangle, six K-steps of 4
+// each.  Operand and result lanes (checked on the MI355X): lane l supplies
+// A[16 I + l % 16][4 t + l / 16] and Bd[4 t + l / 16][16 J + l % 16], and
+// result r of lane l is D[4 r + l / 16][l % 16].  The results overwrite the
+// operands' LDS; the tile (0,1) is mirrored into (1,0), and the diagonal
+// tiles are symmetrised (0.5 (K + K^T), as the reference does) while the
+// element is stored.
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double strain_b(const double* gr, int s, int c) {
+  // B_g[s][3 a + c] for the node gradient gr = (gx, gy, gz)
+  switch (s) {
+    case 0: return c == 0 ? gr[0] : 0.0;
+    case 1: return c == 1 ? gr[1] : 0.0;
+    case 2: return c == 2 ? gr[2] : 0.0;
+    case 3: return c == 0 ? gr[1] : (c == 1 ? gr[0] : 0.0);
+    case 4: return c == 1 ? gr[2] : (c == 2 ? gr[1] : 0.0);
+    default: return c == 0 ? gr[2] : (c == 2 ? gr[0] : 0.0);
+  }
+}
+
+// A (32 rows x 24, row stride 25) then Bd (24 rows x 32, row stride 33): the
+// odd strides spread a wavefront's operand reads and transposed writes over
+// the LDS banks.  Two elements (wavefronts) per block keep the LDS per block
+// at 27 KB.
+constexpr int kKeAS = 25, kKeBS = 33;
+constexpr int kKeOps = 32 * kKeAS + 24 * kKeBS;
+constexpr int kKeMfElem = 2;
+
+__global__ __launch_bounds__(64 * kKeMfElem) void c3d10_ke_mfma_kernel(int M, const int* __restrict__ conn,
+                                                            const double* __restrict__ nodes,
+                                                            const double* __restrict__ E,
+                                                            const double* __restrict__ nu, double* __restrict__ Ke,
+                                                            int* __restrict__ bad_flag) {
+  __shared__ double sX[kKeMfElem][10][3];
+  __shared__ double sG[kKeMfElem][4][10][3];
+  __shared__ double sW[kKeMfElem][4];
+  __shared__ __align__(16) double sOp[kKeMfElem][kKeOps];  // operands, then the 32 x 32 result
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int e = blockIdx.x * kKeMfElem + w;
+  const bool live = e < M;
+  if (live && lane < 30) {
+    const int node = conn[(size_t)e * 10 + lane / 3];
+    sX[w][lane / 3][lane % 3] = nodes[(size_t)node * 3 + lane % 3];
+  }
+  __syncthreads();
+  if (live && lane < 40) {
+    const int g = lane / 10, a = lane % 10;
+    double grad[3];
+    const double det = node_gradient(sX[w], g, a, grad);
+    sG[w][g][a][0] = grad[0];
+    sG[w][g][a][1] = grad[1];
+    sG[w][g][a][2] = grad[2];
+    if (a == 0) {
+      sW[w][g] = det / 24.0;
+      if (!(det > 0.0)) atomicMax(bad_flag, e + 1);
+    }
+  }
+  __syncthreads();
+  double* A = sOp[w];
+  double* Bd = sOp[w] + 32 * kKeAS;
+  if (live) {
+    const double Ev = E[e], nv = nu[e];
+    const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
+    const double lam = f * nv, mu = Ev / (2.0 * (1.0 + nv)), l2m = lam + 2.0 * mu;
+    // A[i][k] and Bd[k][i] for i = DOF, k = 6 g + s: 768 entries each, 12 per lane
+    for (int q = lane; q < 32 * 24; q += 64) {
+      const int i = q / 24, k = q % 24, g = k / 6, sc = k % 6;
+      double av = 0.0, bv = 0.0;
+      if (i < 30) {
+        const double* gi = sG[w][g][i / 3];
+        const int ci = i % 3;
+        av = strain_b(gi, sc, ci);
+        double db;
+        if (sc < 3) {
+          const double b0 = strain_b(gi, 0, ci), b1 = strain_b(gi, 1, ci), b2 = strain_b(gi, 2, ci);
+          db = l2m * av + lam * (b0 + b1 + b2 - av);
+        } else {
+          db = mu * av;
+        }
+        bv = sW[w][g] * db;
+      }
+      A[i * kKeAS + k] = av;
+      Bd[k * kKeBS + i] = bv;
+    }
+  }
+  __syncthreads();
+  mf_d4 acc[3];
+  if (live) {
+    const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int tile = 0; tile < 3; ++tile) {
+      const int I = tile == 2 ? 1 : 0, J = tile == 0 ? 0 : 1;
+      acc[tile] = mf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < 6; ++t) {
+        const int k = 4 * t + lk;
+        acc[tile] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(16 * I + lr) * kKeAS + k], Bd[k * kKeBS + 16 * J + lr],
+                                                         acc[tile], 0, 0, 0);
+      }
+    }
+  }
+  __syncthreads();  // every lane is done with the operands: reuse their LDS for the result
+  if (live) {
+    const int lr = lane & 15, lk = lane >> 4;
+    double* K2 = sOp[w];
+#pragma unroll
+    for (int tile = 0; tile < 3; ++tile) {
+      const int I = tile == 2 ? 1 : 0, J = tile == 0 ? 0 : 1;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * I + 4 * r + lk, col = 16 * J + lr;
+        K2[row * 32 + col] = acc[tile][r];
+        if (I != J) K2[col * 32 + row] = acc[tile][r];
+      }
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  // symmetrise and store the 30 x 30 element as 16-byte chunks
+  using D2 = double __attribute__((ext_vector_type(2)));
+  D2* dst = reinterpret_cast<D2*>(Ke + (size_t)e * 900);
+  const double* k2 = sOp[w];
+  for (int q = lane; q < 450; q += 64) {
+    const int i0 = 2 * q, r0 = i0 / 30, c0 = i0 % 30;  // c0 even, so c0 + 1 < 30 is in the same row
+    D2 v;
+    v.x = 0.5 * (k2[r0 * 32 + c0] + k2[c0 * 32 + r0]);
+    v.y = 0.5 * (k2[r0 * 32 + c0 + 1] + k2[(c0 + 1) * 32 + r0]);
+    __builtin_nontemporal_store(v, dst + q);
+  }
+}
+
+// Ke variant: 0 vector ALU (default), 1 matrix cores
+static int g_ke_variant = 0;
+int set_ke_variant(int v) {
+  if (v != 0 && v != 1) return kBadArg;
+  g_ke_variant = v;
+  return kOk;
 }
 
 // Per-element internal forces fe[e][30] = Ke(e) u_e, matrix-free (no Ke), plus
@@ -514,26 +672,34 @@ __global__ __launch_bounds__(256) void c3d10_pa_elem_kernel(int M, const int* __
     c4[k] = T(4) * L[k];
     cc[k] = c4[k] - T(1);
   }
+  // Loads in two rounds, each issued together: first everything that does
+  // not depend on another load (the connectivity of this lane's nodes g,
+  // g + 4, g + 8, the quadrature data, the moduli), then the nodal values
+  // the connectivity addresses.  A lane without a third node (g >= 2) reads
+  // node 9's entries and drops them, which keeps the code free of branches
+  // (left to itself the compiler serialised the third node's chain and the
+  // moduli behind the rest: four exposed memory latencies instead of two).
+  int nd[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) nd[j] = conn[(size_t)e * 10 + (g + 4 * j < 10 ? g + 4 * j : 9)];
   // quadrature data of this point (affine: of the element, shared by its 4 lanes)
   T Ji[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) Ji[k] = Affine ? qd[(size_t)k * M + e] : qd[((size_t)k * M + e) * 4 + g];
   const T wd = Affine ? qd[(size_t)9 * M + e] : qd[((size_t)9 * M + e) * 4 + g];
+  const double Ev = E[e], nv = nu[e];
+  __builtin_amdgcn_sched_group_barrier(0x020, 15, 0);
   // lane g loads the scaled values of nodes g, g + 4, g + 8 only; the quad
   // shares them by DPP broadcast (a third of the memory instructions)
   T own[3][3];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    const int i = g + 4 * j;
-    if (i < 10) {
-      const int n3 = 3 * conn[(size_t)e * 10 + i];
+    const int n3 = 3 * nd[j];
+    const T keep = g + 4 * j < 10 ? T(1) : T(0);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) own[j][c] = U[n3 + c] * sc[n3 + c];
-    } else {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) own[j][c] = T(0);
-    }
+    for (int c = 0; c < 3; ++c) own[j][c] = keep * (U[n3 + c] * sc[n3 + c]);
   }
+  __builtin_amdgcn_sched_group_barrier(0x020, 18, 0);
   T Q[4][3];
 #pragma unroll
   for (int i = 0; i < 10; ++i) {
@@ -559,7 +725,6 @@ __global__ __launch_bounds__(256) void c3d10_pa_elem_kernel(int M, const int* __
 #pragma unroll
     for (int r = 0; r < 3; ++r) H[r][c] = Ji[3 * r] * g0 + Ji[3 * r + 1] * g1 + Ji[3 * r + 2] * g2;
   }
-  const double Ev = E[e], nv = nu[e];
   const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
   const T c11 = T(f * (1.0 - nv)), c12 = T(f * nv), c44 = T(Ev / (2.0 * (1.0 + nv)));
   const T e0 = H[0][0], e1 = H[1][1], e2 = H[2][2];
@@ -631,40 +796,58 @@ __global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_kernel(
   const int lb = slice_of_block(nb, 1);
   const int c = lb * kGatherWaves + w;
   const bool live = lb < nb && c < n_chunks;
-  // chunk c: nodes [a.x, b.x), incidences [a.y, b.y)
-  const int2 a = live ? chunk[c] : make_int2(0, 0);
-  const int2 b = live ? chunk[c + 1] : make_int2(0, 0);
+  // chunk c: nodes [a.x, b.x), incidences [a.y, b.y).  Every load below is
+  // unconditional with a clamped index (node 0 and incidence 0 always exist)
+  // and its value masked afterwards: the three dependent rounds (chunk ->
+  // node operands and incidence codes -> element values) are then each
+  // issued together, where loads under per-lane branches were waited on one
+  // by one (eight serial latencies per wavefront).
+  const int cc0 = live ? c : 0;
+  const int2 a0 = chunk[cc0], b0 = chunk[cc0 + 1];
+  const int2 a = live ? a0 : make_int2(0, 0);
+  const int2 b = live ? b0 : make_int2(0, 0);
   const int k0 = a.y, k1 = b.y;
   const bool big = k1 - k0 > kChunkInc;  // then the chunk is one node
   // this lane's node and its operands, loaded before the incidences
   const int n = a.x + l;
   const bool has = n < b.x;
-  int j0 = 0, j1 = 0, m = 0;
+  const int nn = has ? n : 0;
+  const int p0 = n2e_ptr[nn], p1 = n2e_ptr[nn + 1];
+  const int mraw = fix ? fix[nn] : 0;
+  double ur[3], fr[3];
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    ur[cc] = u[(size_t)nn * 3 + cc];
+    fr[cc] = mode == 2 ? f[(size_t)nn * 3 + cc] : 0.0;
+  }
+  int code[kIncPerLane];
+#pragma unroll
+  for (int q = 0; q < kIncPerLane; ++q) {
+    const int k = k0 + l + 64 * q;
+    code[q] = n2e[(!big && k < k1) ? k : 0];  // e * 10 + local
+  }
+  double v[kIncPerLane][3];
+#pragma unroll
+  for (int q = 0; q < kIncPerLane; ++q) {
+    const double* p = fe + ((size_t)(code[q] % 10) * M + code[q] / 10) * 3;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) v[q][cc] = p[cc];
+  }
+  const int j0 = has ? p0 - k0 : 0, j1 = has ? p1 - k0 : 0, m = has ? mraw : 0;
   double uu[3] = {0.0, 0.0, 0.0}, ff[3] = {0.0, 0.0, 0.0};
   if (has) {
-    j0 = n2e_ptr[n] - k0;
-    j1 = n2e_ptr[n + 1] - k0;
-    m = fix ? fix[n] : 0;
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) {
-      if (mode == 1 || ((m >> cc) & 1)) uu[cc] = u[(size_t)n * 3 + cc];
-      if (mode == 2) ff[cc] = f[(size_t)n * 3 + cc];
+      if (mode == 1 || ((m >> cc) & 1)) uu[cc] = ur[cc];
+      if (mode == 2) ff[cc] = fr[cc];
     }
   }
 #pragma unroll
   for (int q = 0; q < kIncPerLane; ++q) {
     const int k = k0 + l + 64 * q;
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-    if (!big && k < k1) {
-      const int code = n2e[k];  // e * 10 + local
-      const double* p = fe + ((size_t)(code % 10) * M + code / 10) * 3;
-      v0 = p[0];
-      v1 = p[1];
-      v2 = p[2];
-    }
-    sv[w][l + 64 * q][0] = v0;
-    sv[w][l + 64 * q][1] = v1;
-    sv[w][l + 64 * q][2] = v2;
+    const bool ok = !big && k < k1;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) sv[w][l + 64 * q][cc] = ok ? v[q][cc] : 0.0;
   }
   __syncthreads();
   double dot = 0.0;
@@ -726,41 +909,50 @@ __global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_op_kernel(
   const int lb = slice_of_block(nb, 1);
   const int c = lb * kGatherWaves + w;
   const bool live = lb < nb && c < n_chunks;
-  const int2 a = live ? chunk[c] : make_int2(0, 0);
-  const int2 b = live ? chunk[c + 1] : make_int2(0, 0);
+  // branch-free loads with clamped indices, as in mf_gather_kernel
+  const int cc0 = live ? c : 0;
+  const int2 a0 = chunk[cc0], b0 = chunk[cc0 + 1];
+  const int2 a = live ? a0 : make_int2(0, 0);
+  const int2 b = live ? b0 : make_int2(0, 0);
   const int k0 = a.y, k1 = b.y;
   const bool big = k1 - k0 > kChunkInc;
   const int n = a.x + l;
   const bool has = n < b.x;
-  int j0 = 0, j1 = 0, m = 0;
-  T s3[3] = {T(1), T(1), T(1)}, xf[3] = {T(0), T(0), T(0)};
+  const int nn = has ? n : 0;
+  const int p0 = n2e_ptr[nn], p1 = n2e_ptr[nn + 1];
+  const int mraw = fix ? fix[nn] : 0;
+  T s3[3] = {T(1), T(1), T(1)}, xr[3];
   typename Op::Pre pre[3];
-  if (has) {
-    j0 = n2e_ptr[n] - k0;
-    j1 = n2e_ptr[n + 1] - k0;
-    m = fix ? fix[n] : 0;
 #pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-      const long long k = (long long)n * 3 + cc;
-      op.load(k, pre[cc]);
-      if (sc) s3[cc] = sc[k];
-      if ((m >> cc) & 1) xf[cc] = x[k];
-    }
+  for (int cc = 0; cc < 3; ++cc) {
+    const long long k = (long long)nn * 3 + cc;
+    op.load(k, pre[cc]);
+    if (sc) s3[cc] = sc[k];
+    xr[cc] = x[k];
   }
+  int code[kIncPerLane];
 #pragma unroll
   for (int q = 0; q < kIncPerLane; ++q) {
     const int k = k0 + l + 64 * q;
-    T v0 = T(0), v1 = T(0), v2 = T(0);
-    if (!big && k < k1) {
-      const int code = n2e[k];  // e * 10 + local
-      const T* p = fe + ((size_t)(code % 10) * M + code / 10) * 3;
-      v0 = p[0];
-      v1 = p[1];
-      v2 = p[2];
-    }
-    sv[w][l + 64 * q][0] = v0;
-    sv[w][l + 64 * q][1] = v1;
-    sv[w][l + 64 * q][2] = v2;
+    code[q] = n2e[(!big && k < k1) ? k : 0];  // e * 10 + local
+  }
+  T v[kIncPerLane][3];
+#pragma unroll
+  for (int q = 0; q < kIncPerLane; ++q) {
+    const T* p = fe + ((size_t)(code[q] % 10) * M + code[q] / 10) * 3;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) v[q][cc] = p[cc];
+  }
+  const int j0 = has ? p0 - k0 : 0, j1 = has ? p1 - k0 : 0, m = has ? mraw : 0;
+  T xf[3];
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) xf[cc] = (m >> cc) & 1 ? xr[cc] : T(0);
+#pragma unroll
+  for (int q = 0; q < kIncPerLane; ++q) {
+    const int k = k0 + l + 64 * q;
+    const bool ok = !big && k < k1;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) sv[w][l + 64 * q][cc] = ok ? v[q][cc] : T(0);
   }
   __syncthreads();
   if (!has) return;
@@ -810,8 +1002,12 @@ int launch_ke(int M, const int* conn, const double* nodes, const double* E, cons
   if (M <= 0) return kOk;
   int r = upload_element_consts(s);
   if (r) return r;
-  hipLaunchKernelGGL(c3d10_ke_kernel, dim3(ceil_div(M, kElemPerBlock)), dim3(256), 0, s, M, conn, nodes, E, nu, Ke,
-                     bad);
+  if (g_ke_variant == 1)
+    hipLaunchKernelGGL(c3d10_ke_mfma_kernel, dim3(ceil_div(M, kKeMfElem)), dim3(64 * kKeMfElem), 0, s, M, conn, nodes,
+                       E, nu, Ke, bad);
+  else
+    hipLaunchKernelGGL(c3d10_ke_kernel, dim3(ceil_div(M, kElemPerBlock)), dim3(256), 0, s, M, conn, nodes, E, nu, Ke,
+                       bad);
   return static_cast<int>(hipGetLastError());
 }
 

@@ -3,11 +3,18 @@
 //   * the node graph / 3x3 block pattern of a C3D10 mesh with the Dirichlet
 //     pattern rule,
 //   * the per-block contribution lists that make GPU assembly a deterministic
-//     gather, and the node -> (element, local node) incidence lists.
+//     gather, and the node -> (element, local node) incidence lists;
+//   * the block-sparse products of the smoothed-aggregation set-up (A P and
+//     R (A P)), the block transpose, the in-place symmetrisation of a
+//     Galerkin operator and the block SpMV of its eigenvalue estimate, on
+//     host threads (one contiguous range of block rows each; every output
+//     entry is summed in a fixed order, so results do not depend on the
+//     thread count).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace wn {
@@ -292,6 +299,203 @@ int sa_tentative(int n_nodes, int b, int k, const int* agg, int n_agg, const dou
       for (int c = 0; c < k; ++c) Bc[((size_t)a * k + r) * k + c] = R[r * k + c];
   }
   return 0;
+}
+
+// ---------------------------------------------------------------- threaded block-sparse set-up kernels
+// Runs fn(row_begin, row_end) on `threads` host threads over [0, n) (0 = one
+// per hardware thread, at most 64).
+template <class F>
+static void parallel_rows(int n, int threads, F fn) {
+  int t = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+  t = std::max(1, std::min({t, 64, std::max(1, n / 64)}));
+  if (t == 1) {
+    fn(0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const int per = (n + t - 1) / t;
+  for (int k = 0; k < t; ++k) {
+    const int a = k * per, b = std::min(n, a + per);
+    if (a < b) pool.emplace_back(fn, a, b);
+  }
+  for (auto& th : pool) th.join();
+}
+
+// C = A B for block-CSR matrices (row-major blocks): A has nr block rows of
+// ra x ca blocks, B has ca x cb blocks and nc block columns.  With c_idx ==
+// nullptr only the structure pass runs: c_ptr[0..nr] receives the prefix sums
+// of the block counts of C's rows, and the total is returned.  With c_idx
+// and c_val (c_ptr[nr] entries) the products are accumulated: each row of C
+// in a dense accumulator over B's block columns, its columns sorted, each
+// block summed over A's row in storage order.
+long long bsr_spgemm(int nr, int ra, int ca, int cb, int nc, const int* a_ptr, const int* a_idx, const double* a_val,
+                     const int* b_ptr, const int* b_idx, const double* b_val, int* c_ptr, int* c_idx, double* c_val,
+                     int threads) {
+  const int bbA = ra * ca, bbB = ca * cb, bbC = ra * cb;
+  if (!c_idx) {
+    std::vector<int> cnt(nr, 0);
+    parallel_rows(nr, threads, [&](int r0, int r1) {
+      std::vector<int> mark(nc, -1);
+      for (int i = r0; i < r1; ++i) {
+        int c = 0;
+        for (int ka = a_ptr[i]; ka < a_ptr[i + 1]; ++ka) {
+          const int j = a_idx[ka];
+          for (int kb = b_ptr[j]; kb < b_ptr[j + 1]; ++kb) {
+            const int col = b_idx[kb];
+            if (mark[col] != i) {
+              mark[col] = i;
+              ++c;
+            }
+          }
+        }
+        cnt[i] = c;
+      }
+    });
+    long long tot = 0;
+    c_ptr[0] = 0;
+    for (int i = 0; i < nr; ++i) {
+      tot += cnt[i];
+      c_ptr[i + 1] = (int)tot;
+    }
+    return tot;
+  }
+  parallel_rows(nr, threads, [&](int r0, int r1) {
+    std::vector<int> pos(nc, -1);
+    std::vector<int> cols;
+    std::vector<double> acc;
+    for (int i = r0; i < r1; ++i) {
+      cols.clear();
+      for (int ka = a_ptr[i]; ka < a_ptr[i + 1]; ++ka) {
+        const int j = a_idx[ka];
+        for (int kb = b_ptr[j]; kb < b_ptr[j + 1]; ++kb) {
+          const int col = b_idx[kb];
+          if (pos[col] < 0) {
+            pos[col] = 0;
+            cols.push_back(col);
+          }
+        }
+      }
+      std::sort(cols.begin(), cols.end());
+      for (size_t q = 0; q < cols.size(); ++q) pos[cols[q]] = (int)q;
+      acc.assign(cols.size() * bbC, 0.0);
+      for (int ka = a_ptr[i]; ka < a_ptr[i + 1]; ++ka) {
+        const int j = a_idx[ka];
+        const double* Ab = a_val + (size_t)ka * bbA;
+        for (int kb = b_ptr[j]; kb < b_ptr[j + 1]; ++kb) {
+          const double* Bb = b_val + (size_t)kb * bbB;
+          double* Cb = acc.data() + (size_t)pos[b_idx[kb]] * bbC;
+          for (int r = 0; r < ra; ++r)
+            for (int m = 0; m < ca; ++m) {
+              const double av = Ab[r * ca + m];
+              if (av == 0.0) continue;
+              for (int c = 0; c < cb; ++c) Cb[r * cb + c] += av * Bb[m * cb + c];
+            }
+        }
+      }
+      const int base = c_ptr[i];
+      for (size_t q = 0; q < cols.size(); ++q) {
+        c_idx[base + q] = cols[q];
+        std::memcpy(c_val + (size_t)(base + q) * bbC, acc.data() + q * bbC, sizeof(double) * bbC);
+        pos[cols[q]] = -1;
+      }
+    }
+  });
+  return c_ptr[nr];
+}
+
+// T = A^T for a block-CSR A (nr block rows, nc block columns, ra x ca blocks):
+// t_ptr[nc + 1], t_idx / t_val[nnzb] (rows of T sorted by column, blocks
+// transposed).
+void bsr_transpose(int nr, int nc, int ra, int ca, const int* a_ptr, const int* a_idx, const double* a_val,
+                   int* t_ptr, int* t_idx, double* t_val) {
+  const long long nnzb = a_ptr[nr];
+  std::vector<int> cnt(nc + 1, 0);
+  for (long long k = 0; k < nnzb; ++k) cnt[a_idx[k] + 1]++;
+  for (int c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+  std::memcpy(t_ptr, cnt.data(), sizeof(int) * (nc + 1));
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int i = 0; i < nr; ++i)
+    for (int k = a_ptr[i]; k < a_ptr[i + 1]; ++k) {
+      const int c = a_idx[k], dst = fill[c]++;
+      t_idx[dst] = i;
+      const double* src = a_val + (size_t)k * ra * ca;
+      double* out = t_val + (size_t)dst * ra * ca;
+      for (int r = 0; r < ra; ++r)
+        for (int q = 0; q < ca; ++q) out[q * ra + r] = src[r * ca + q];
+    }
+}
+
+// In place A <- (A + A^T) / 2 for a square block-CSR matrix with a
+// structurally symmetric pattern and sorted columns; each pair of mirror
+// blocks is averaged once.  Zero diagonal entries are then set to one (a
+// coarse DOF that no fine DOF interpolates).  Returns the number of entries
+// so set, or -1 when a mirror block is missing.
+long long bsr_symmetrize(int n, int b, const int* ptr, const int* idx, double* val, int threads) {
+  const int bb = b * b;
+  std::vector<long long> fixed(std::max(1, n), 0);
+  std::vector<char> bad(std::max(1, n), 0);
+  parallel_rows(n, threads, [&](int r0, int r1) {
+    for (int i = r0; i < r1; ++i) {
+      for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const int j = idx[k];
+        if (j < i) continue;
+        double* A = val + (size_t)k * bb;
+        if (j == i) {
+          for (int r = 0; r < b; ++r)
+            for (int c = r + 1; c < b; ++c) {
+              const double m = 0.5 * (A[r * b + c] + A[c * b + r]);
+              A[r * b + c] = m;
+              A[c * b + r] = m;
+            }
+          for (int r = 0; r < b; ++r)
+            if (A[r * b + r] == 0.0) {
+              A[r * b + r] = 1.0;
+              fixed[i]++;
+            }
+          continue;
+        }
+        const int* lo = idx + ptr[j];
+        const int* hi = idx + ptr[j + 1];
+        const int* at = std::lower_bound(lo, hi, i);
+        if (at == hi || *at != i) {
+          bad[i] = 1;
+          continue;
+        }
+        double* T = val + (size_t)(ptr[j] + (at - lo)) * bb;
+        // block (i, j) and block (j, i) = its transpose: average them
+        for (int r = 0; r < b; ++r)
+          for (int c = 0; c < b; ++c) {
+            const double m = 0.5 * (A[r * b + c] + T[c * b + r]);
+            A[r * b + c] = m;
+            T[c * b + r] = m;
+          }
+      }
+    }
+  });
+  long long tot = 0;
+  for (int i = 0; i < n; ++i) {
+    if (bad[i]) return -1;
+    tot += fixed[i];
+  }
+  return tot;
+}
+
+// y = A x for a square block-CSR matrix (b x b blocks) on host threads.
+void bsr_spmv_host(int n, int b, const int* ptr, const int* idx, const double* val, const double* x, double* y,
+                   int threads) {
+  const int bb = b * b;
+  parallel_rows(n, threads, [&](int r0, int r1) {
+    for (int i = r0; i < r1; ++i) {
+      double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const double* A = val + (size_t)k * bb;
+        const double* xv = x + (size_t)idx[k] * b;
+        for (int r = 0; r < b; ++r)
+          for (int c = 0; c < b; ++c) acc[r] += A[r * b + c] * xv[c];
+      }
+      for (int r = 0; r < b; ++r) y[(size_t)i * b + r] = acc[r];
+    }
+  });
 }
 
 }  // namespace wn

@@ -75,6 +75,7 @@ struct SpmvTune {
 
 // -------- element kernels (element.hip)
 int upload_element_consts(hipStream_t s);
+int set_ke_variant(int v);
 int launch_ke(int M, const int* conn, const double* nodes, const double* E, const double* nu, double* Ke, int* bad,
               hipStream_t s);
 // Matrix-free C3D10 operator with the assembled matrix's Dirichlet rule.

@@ -52,6 +52,103 @@ def rigid_body_modes(coords: np.ndarray, dofs_per_node: int = 3) -> np.ndarray:
     return B
 
 
+def _bsr_parts(A: sp.bsr_matrix):
+    r, c = A.blocksize
+    return (np.ascontiguousarray(A.indptr, dtype=np.int32), np.ascontiguousarray(A.indices, dtype=np.int32),
+            np.ascontiguousarray(A.data, dtype=np.float64).reshape(-1, r, c))
+
+
+def _spgemm(A: sp.bsr_matrix, B: sp.bsr_matrix) -> sp.bsr_matrix:
+    """A B for block-CSR matrices on host threads (wnfea_host_bsr_spgemm):
+    sorted block columns, every block summed in a fixed order."""
+    L = _lib.load()
+    ra, ca = A.blocksize
+    cb2, cb = B.blocksize
+    if ca != cb2 or A.shape[1] != B.shape[0]:
+        raise ValueError("block shapes of A and B do not chain")
+    nr, nc = A.shape[0] // ra, B.shape[1] // cb
+    ap, ai, av = _bsr_parts(A)
+    bp, bi, bv = _bsr_parts(B)
+    cp = np.zeros(nr + 1, dtype=np.int32)
+    tot = L.wnfea_host_bsr_spgemm(nr, ra, ca, cb, nc, _lib.hptr(ap), _lib.hptr(ai), _lib.hptr(av), _lib.hptr(bp),
+                                  _lib.hptr(bi), _lib.hptr(bv), _lib.hptr(cp), None, None, 0)
+    if tot < 0:
+        raise _lib.HipError(f"wnfea_host_bsr_spgemm failed ({tot})")
+    ci = np.empty(max(tot, 1), dtype=np.int32)
+    cv = np.empty((max(tot, 1), ra, cb))
+    tot2 = L.wnfea_host_bsr_spgemm(nr, ra, ca, cb, nc, _lib.hptr(ap), _lib.hptr(ai), _lib.hptr(av), _lib.hptr(bp),
+                                   _lib.hptr(bi), _lib.hptr(bv), _lib.hptr(cp), _lib.hptr(ci), _lib.hptr(cv), 0)
+    if tot2 != tot:
+        raise _lib.HipError(f"wnfea_host_bsr_spgemm failed ({tot2})")
+    return sp.bsr_matrix((cv[:tot], ci[:tot], cp), shape=(A.shape[0], B.shape[1]))
+
+
+def _transpose(A: sp.bsr_matrix) -> sp.bsr_matrix:
+    """A^T as a block-CSR matrix with transposed blocks (host kernel)."""
+    L = _lib.load()
+    ra, ca = A.blocksize
+    nr, nc = A.shape[0] // ra, A.shape[1] // ca
+    ap, ai, av = _bsr_parts(A)
+    nnzb = int(ap[-1])
+    tp = np.zeros(nc + 1, dtype=np.int32)
+    ti = np.empty(max(nnzb, 1), dtype=np.int32)
+    tv = np.empty((max(nnzb, 1), ca, ra))
+    _lib.check(L.wnfea_host_bsr_transpose(nr, nc, ra, ca, _lib.hptr(ap), _lib.hptr(ai), _lib.hptr(av), _lib.hptr(tp),
+                                          _lib.hptr(ti), _lib.hptr(tv)), "wnfea_host_bsr_transpose")
+    return sp.bsr_matrix((tv[:nnzb], ti[:nnzb], tp), shape=(A.shape[1], A.shape[0]))
+
+
+def _symmetrize_galerkin(Ac: sp.bsr_matrix) -> tuple[sp.bsr_matrix, int]:
+    """(Ac + Ac^T) / 2 in place, zero diagonal entries set to one (the
+    reference's fix of coarse DOFs no fine DOF interpolates,
+    amg_preconditioner.py:288-293); returns (Ac, entries fixed)."""
+    L = _lib.load()
+    b = Ac.blocksize[0]
+    ap, ai, av = _bsr_parts(Ac)
+    fixed = L.wnfea_host_bsr_symmetrize(Ac.shape[0] // b, b, _lib.hptr(ap), _lib.hptr(ai), _lib.hptr(av), 0)
+    if fixed < 0:
+        raise _lib.HipError("wnfea_host_bsr_symmetrize: pattern not structurally symmetric")
+    return sp.bsr_matrix((av, ai, ap), shape=Ac.shape), int(fixed)
+
+
+def _matvec(A: sp.spmatrix):
+    """x -> A x: the library's threaded host block SpMV for a square BSR with
+    blocks of at most 8, else scipy's."""
+    if not (sp.isspmatrix_bsr(A) and A.blocksize[0] == A.blocksize[1] and A.blocksize[0] <= 8):
+        return lambda x: A @ x
+    L = _lib.load()
+    b = A.blocksize[0]
+    ap, ai, av = _bsr_parts(A)
+    n = A.shape[0] // b
+
+    def mv(x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _lib.check(L.wnfea_host_bsr_spmv(n, b, _lib.hptr(ap), _lib.hptr(ai), _lib.hptr(av), _lib.hptr(x),
+                                         _lib.hptr(y), 0), "wnfea_host_bsr_spmv")
+        return y
+    return mv
+
+
+def _smooth_prolongator(A: sp.bsr_matrix, P0: sp.bsr_matrix, dinv: np.ndarray, w: float) -> sp.bsr_matrix:
+    """P = P0 - w D^-1 A P0 on the pattern of A P0 (which holds P0's blocks:
+    every stored node row has its diagonal block)."""
+    AP = _spgemm(A, P0)
+    b, k = P0.blocksize
+    nb, nc = P0.shape[0] // b, P0.shape[1] // k
+    rows_ap = np.repeat(np.arange(nb, dtype=np.int64), np.diff(AP.indptr))
+    vals = AP.data * ((-w) * dinv.reshape(nb, b)[rows_ap])[:, :, None]
+    rows_p0 = np.repeat(np.arange(nb, dtype=np.int64), np.diff(P0.indptr))
+    keys_ap = rows_ap * nc + AP.indices
+    keys_p0 = rows_p0 * nc + P0.indices
+    pos = np.searchsorted(keys_ap, keys_p0)
+    if len(pos) and (pos.max() >= len(keys_ap) or not np.array_equal(keys_ap[pos], keys_p0)):
+        # a node row without its diagonal block: add P0 in scipy instead
+        return (P0 - w * (sp.diags(dinv) @ AP)).tobsr(blocksize=(b, k))
+    vals[pos] += P0.data
+    return sp.bsr_matrix((vals, AP.indices, AP.indptr), shape=P0.shape)
+
+
 def _power_lmax(A: sp.spmatrix, dinv: np.ndarray, iters: int = 15, seed: int = 0) -> float:
     """Upper estimate of the largest eigenvalue of D^-1 A (dinv > 0): Lanczos
     on the similar symmetric matrix D^-1/2 A D^-1/2, largest Ritz value plus
@@ -59,6 +156,7 @@ def _power_lmax(A: sp.spmatrix, dinv: np.ndarray, iters: int = 15, seed: int = 0
     steps where plain power iteration stalls on clustered eigenvalues; the
     Chebyshev smoother needs a value that is not below the true one."""
     n = A.shape[0]
+    mv = _matvec(A)
     dh = np.sqrt(np.abs(dinv))
     v = np.random.default_rng(seed).standard_normal(n)
     v /= np.linalg.norm(v)
@@ -66,7 +164,7 @@ def _power_lmax(A: sp.spmatrix, dinv: np.ndarray, iters: int = 15, seed: int = 0
     alpha, beta = [], []
     steps = max(2, min(iters + 5, n))
     for k in range(steps):
-        w = dh * (A @ (dh * V[-1]))
+        w = dh * mv(dh * V[-1])
         a = float(V[-1] @ w)
         w -= a * V[-1]
         if k:
@@ -168,28 +266,18 @@ def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None
         _lib.check(L.wnfea_sa_tentative(n, b, k, _lib.hptr(agg), na, _lib.hptr(Bc_in), _lib.hptr(p_rowptr),
                                         _lib.hptr(p_colidx), _lib.hptr(p_vals), _lib.hptr(Bc)), "wnfea_sa_tentative")
         P0 = sp.bsr_matrix((p_vals[:n_blk].reshape(-1, b, k), p_colidx[:n_blk], p_rowptr), shape=(n * b, na * k))
-        # P = P0 - (omega / lmax) D^-1 A P0
-        AP = (A @ P0).tobsr(blocksize=(b, k))
-        Dinv = sp.diags(dinv)
-        P = (P0 - (omega / lmax) * (Dinv @ AP)).tobsr(blocksize=(b, k))
-        P.sum_duplicates()
-        P.sort_indices()
-        R = P.T.tobsr(blocksize=(k, b))
-        R.sort_indices()
-        Ac = (R @ (A @ P).tobsr(blocksize=(b, k))).tobsr(blocksize=(k, k))
-        Ac = (0.5 * (Ac + Ac.T)).tobsr(blocksize=(k, k))
+        # P = P0 - (omega / lmax) D^-1 A P0;  Ac = R A P with R = P^T, on host
+        # threads (wnfea_host_bsr_*), symmetrised
+        P = _smooth_prolongator(A, P0, dinv, omega / lmax)
+        R = _transpose(P)
+        Ac = _spgemm(R, _spgemm(A, P))
         # An aggregate that cannot carry all k near-null-space modes (one or
         # two solid nodes, collinear nodes) leaves zero columns in P, hence
-        # zero rows and columns in Ac.  Give them a unit diagonal (the
-        # reference does the same, amg_preconditioner.py:288-293): they
-        # receive no restricted residual and prolong nothing, and the
+        # zero rows and columns in Ac: the symmetrisation gives them a unit
+        # diagonal (the reference does the same, amg_preconditioner.py:288-293),
+        # so they receive no restricted residual and prolong nothing, and the
         # coarser levels and the dense coarse solve stay nonsingular.
-        dAc = Ac.diagonal()
-        vanished = dAc == 0.0
-        if vanished.any():
-            Ac = (Ac + sp.diags(vanished.astype(np.float64))).tobsr(blocksize=(k, k))
-        Ac.sum_duplicates()
-        Ac.sort_indices()
+        Ac, _ = _symmetrize_galerkin(Ac)
         levels.append(AMGLevelData(A=A, dinv=dinv, lmax=lmax, P=P, R=R, n_aggregates=na))
         A, B = Ac, Bc
         th *= theta_decay
@@ -203,17 +291,40 @@ def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None
 
 
 def _coarse_inverse(Acd: np.ndarray) -> np.ndarray:
-    """Symmetric inverse of the coarsest operator by Cholesky.  A matrix that
-    is only semidefinite in floating point gets the reference's relative
-    1e-12 diagonal shift (amg_preconditioner.py:314) before a second
-    factorisation; an indefinite one falls back to LU."""
-    import scipy.linalg as sla
+    """Symmetric inverse of the coarsest operator by Cholesky, in FP64 on the
+    GPU when one is visible (rocSOLVER through torch.linalg: the 9 978-row
+    I-beam level factorises in a fraction of a second there), else on the
+    host (scipy).  A matrix that is only semidefinite in floating point gets
+    the reference's relative 1e-12 diagonal shift (amg_preconditioner.py:314)
+    before a second factorisation; an indefinite one falls back to LU."""
     n = len(Acd)
     if n == 0:
         return np.zeros((0, 0))
-    for shift in (0.0, 1e-12 * max(float(np.abs(np.diag(Acd)).max()), 1e-300)):
+    shift = 1e-12 * max(float(np.abs(np.diag(Acd)).max()), 1e-300)
+    try:
+        import torch
+        on_gpu = torch.cuda.is_available()
+    except ImportError:
+        on_gpu = False
+    if on_gpu:
+        A = torch.as_tensor(np.ascontiguousarray(Acd, dtype=np.float64), device="cuda")
+        inv = None
+        for sh in (0.0, shift):
+            M = A if sh == 0.0 else A + sh * torch.eye(n, dtype=A.dtype, device=A.device)
+            Lc, info = torch.linalg.cholesky_ex(M)
+            if int(info.item()) == 0:
+                inv = torch.cholesky_inverse(Lc)
+                break
+        if inv is None:
+            inv = torch.linalg.inv(A)
+        out = (0.5 * (inv + inv.T)).cpu().numpy()
+        del A, inv
+        torch.cuda.empty_cache()
+        return out
+    import scipy.linalg as sla
+    for sh in (0.0, shift):
         try:
-            c = sla.cho_factor(Acd + shift * np.eye(n) if shift else Acd, lower=True, check_finite=False)
+            c = sla.cho_factor(Acd + sh * np.eye(n) if sh else Acd, lower=True, check_finite=False)
             inv = sla.cho_solve(c, np.eye(n), check_finite=False)
             break
         except np.linalg.LinAlgError:

@@ -37,7 +37,7 @@ class NonLinearConvergenceError(RuntimeError):
 class BeamSystem:
     """Device-resident 6-DOF beam frame: geometry, tangent matrix, AMG."""
 
-    def __init__(self, model: FEAModel, sigma: int = 4096, h_rel: float = 1e-6):
+    def __init__(self, model: FEAModel, sigma: int = 4096, h_rel: float = 0.0):
         import torch
         from ..mesh.lattice import beam_props
         from .bsr_matrix import DeviceBSR
