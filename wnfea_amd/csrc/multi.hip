@@ -37,6 +37,7 @@ __global__ __launch_bounds__(64 * W) void sell_spmm_kernel(SellView<VT> A, const
   if (guard_skip(flags, guard)) return;
   using AT = typename AccT<VT>::type;
   constexpr int BB = BR * BC;
+  constexpr int kLoads = 1 + (BB * (int)sizeof(VT) + 15) / 16 + BC * ((NR * (int)sizeof(XT) + 15) / 16);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int s = slice_of_block(A.n_slices, xcd_remap);
@@ -66,6 +67,11 @@ __global__ __launch_bounds__(64 * W) void sell_spmm_kernel(SellView<VT> A, const
 #pragma unroll
           for (int k = 0; k < NR; ++k) acc[r * NR + k] = fma(AT(v[r * BC + q]), AT(xv[k]), acc[r * NR + k]);
       }
+      // all of the block's loads (column index, values, the NR-wide x rows)
+      // in flight together before its FMAs; left to itself the compiler
+      // reuses the x registers and waits between the x loads
+      __builtin_amdgcn_sched_group_barrier(0x020, kLoads, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 2 * BB * NR, 0);
     }
     if constexpr (W > 1) {
       __shared__ AT red[W - 1][BR * NR][64];
