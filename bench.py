@@ -914,6 +914,19 @@ def main() -> int:
         t_other = time.perf_counter() - t0
         others[f"mixed_{other}_vcycle"] = {"seconds": round(t_other, 5), "iterations": ito, "rel_res": relo}
         log(f"[bench] mixed with {other} V-cycle: {t_other * 1e3:.1f} ms iters {ito} rel {relo:.2e}")
+    # the mixed solve with its coarsest solve on the FP64 inverse (FP64 products,
+    # FP32 vectors), the precision the reference reserves for the coarse solve
+    amg.set_coarse_precision("fp64")
+    solve_mixed(system, amg, tol=1e-8, amg_precision="fp32", operator=main_op)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        uc, itc, _, relc, _ = solve_mixed(system, amg, tol=1e-8, amg_precision="fp32", operator=main_op)
+    torch.cuda.synchronize()
+    t_c64 = (time.perf_counter() - t0) / max(1, args.steps // 2)
+    amg.set_coarse_precision("fp32")
+    others["mixed_fp32_vcycle_fp64_coarse_solve"] = {"seconds": round(t_c64, 5), "iterations": itc, "rel_res": relc}
+    log(f"[bench] mixed with the FP64 coarsest solve: {t_c64 * 1e3:.1f} ms iters {itc} rel {relc:.2e}")
     # the same two solves with the other operator for CG
     alt = {"operator": alt_name}
     for name, prec in (("mixed", args.amg_precision), ("fp64", "fp64")):

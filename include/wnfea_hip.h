@@ -343,6 +343,13 @@ int wnfea_amg_tune_precision(void *h, int prec, int waves, int xcd_remap, int un
  * and arithmetic). */
 int wnfea_amg_vcycle(void *h, int vt, const void *r, void *z, void *stream);
 
+/* Coarsest-level solve of the FP32 / FP16 cycles: fp64 != 0 applies the
+ * FP64 dense inverse (FP64 products, FP32 vectors in and out; needs FP64
+ * storage), 0 (default) its FP32 copy.  No reference counterpart (the
+ * reference factorises its coarse level in the cycle's working precision,
+ * wnfea/solver/amg_preconditioner.py:304-330). */
+int wnfea_amg_set_coarse_fp64(void *h, int fp64);
+
 /* reps Chebyshev smoothing steps of level lvl (0 <= lvl < last) on the
  * level's own cycle vectors, with its assembled values in FP64 (vt 0) or
  * FP32 (vt 1): the smoother kernel of the V-cycle alone, so that its time per

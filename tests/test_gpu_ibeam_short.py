@@ -11,7 +11,9 @@ section, on shortened beams that a sparse direct factorization covers.
   their gradients against the CPU FP64 reference gradient (oracle element
   matrices, exact solutions by sparse LU) to 1e-5, and the compliance
   gradient against central differences (h = 1e-4 E_e, FP64 forward solves)
-  on 20 seeded elements (seed 3) to 1e-4 each."""
+  on 20 seeded elements (seed 3) to 1e-4 each.
+* The V-cycle of a 3-level hierarchy (30 layers; a 6x6-block level 1 with
+  its own smoothing) against the FP64 numpy mirror of the same hierarchy."""
 import numpy as np
 import pytest
 
@@ -112,3 +114,42 @@ def test_compliance_gradient_vs_central_differences_20_elements(c3_short):
         h = 1e-4 * E[e]
         fd = prob.compliance_central_difference(E, int(e), h)
         assert abs(fd - g[e]) <= 1e-4 * abs(g[e]), (int(e), fd, g[e])
+
+
+@pytest.mark.timeout(300)
+def test_three_level_vcycle_matches_numpy_mirror():
+    """The V-cycle on a 3-level hierarchy of the 30-layer I-beam (107 055 ->
+    13 080 -> 1 044 rows: a 6x6-block level 1 with its own Chebyshev
+    smoothing, defect and 6x6 transfers) equals the FP64 numpy mirror of the
+    same hierarchy (tests/vcycle_mirror.py) to 1e-12 in FP64, and to 1e-10
+    with the finest level's smoothing and defect on the matrix-free operator
+    (equal to the assembled one to FP64 rounding); the FP32 cycle agrees to
+    FP32 accuracy."""
+    import torch
+    from vcycle_mirror import vcycle
+    from wnfea_amd.solver.solid_solver import MatrixFreeOperator, assemble_c3d10_bsr, build_amg
+    mr, meta, o2n = _short_ibeam(30)
+    system = assemble_c3d10_bsr(mr, want_fp64_equilibrated=True)
+    amg = build_amg(system, storage=("fp64", "fp32"), theta=0.1, theta_decay=0.4)
+    op = MatrixFreeOperator(system)
+    try:
+        assert [lv.A.shape[0] for lv in amg.levels] == [107055, 13080, 1044]
+        assert amg.levels[1].A.blocksize == (6, 6)
+        n = amg.levels[0].A.shape[0]
+        r = np.random.default_rng(4).standard_normal(n)
+        z_ref = vcycle(amg.levels, amg.coarse_inv, r, deg=amg.level_degrees, ratio=amg.ratio)
+        z = amg.vcycle(torch.as_tensor(r, device="cuda")).cpu().numpy()
+        assert _rel(z, z_ref) <= 1e-12
+        z32 = amg.vcycle(torch.as_tensor(r, device="cuda").float()).double().cpu().numpy()
+        assert _rel(z32, z_ref) <= 1e-4
+        # the FP32 cycle with its coarsest solve on the FP64 inverse
+        amg.set_coarse_precision("fp64")
+        z32c = amg.vcycle(torch.as_tensor(r, device="cuda").float()).double().cpu().numpy()
+        amg.set_coarse_precision("fp32")
+        assert _rel(z32c, z_ref) <= 1e-4 and not np.array_equal(z32c, z32)
+        op.attach_smoother(amg)
+        zmf = amg.vcycle(torch.as_tensor(r, device="cuda")).cpu().numpy()
+        assert _rel(zmf, z_ref) <= 1e-10
+    finally:
+        op.close()
+        amg.close()

@@ -66,6 +66,7 @@ SIGNATURES = {
     "wnfea_amg_tune_precision": (_i, [_p, _i, _i, _i, _i]),
     "wnfea_amg_vcycle": (_i, [_p, _i, _p, _p, _p]),
     "wnfea_amg_level_cheb": (_i, [_p, _i, _i, _i, _p]),
+    "wnfea_amg_set_coarse_fp64": (_i, [_p, _i]),
     "wnfea_residual_noalloc": (_i, [_p, _p, _p, _p, _p, _p]),
     "wnfea_solver_create": (_p, []),
     "wnfea_solver_destroy": (None, [_p]),

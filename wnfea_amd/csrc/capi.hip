@@ -38,6 +38,7 @@ struct AmgLevel;
 struct Solver;
 int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s);
 int amg_level_cheb(Amg& g, int l, int vt, int reps, hipStream_t s);
+int amg_set_coarse_fp64(Amg& g, int fp64);
 template <typename T>
 int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, int amg_vt, const double* scale,
             const T* b, T* x, double rtol, int max_iter, int check_every, const SpmvTune& tune0, hipStream_t s,
@@ -417,6 +418,12 @@ int wnfea_amg_vcycle(void* h, int vt, const void* r, void* z, void* stream) {
   return amg_apply(*static_cast<Amg*>(h), vt, r, z, nullptr, S(stream));
 }
 
+// FP32 cycles: coarsest solve with the FP64 inverse (fp64 != 0) or its FP32 copy
+int wnfea_amg_set_coarse_fp64(void* h, int fp64) {
+  if (!h) return kBadArg;
+  return amg_set_coarse_fp64(*static_cast<Amg*>(h), fp64);
+}
+
 // reps Chebyshev smoothing steps of one level on its own vectors (vt 0 FP64,
 // 1 FP32 values): the cycle's smoother kernel alone, for timing
 int wnfea_amg_level_cheb(void* h, int lvl, int vt, int reps, void* stream) {
@@ -540,6 +547,7 @@ int wnfea_mf_affine(void* h) {
   if (!h) return kBadArg;
   return static_cast<WnMf*>(h)->op.affine ? 1 : 0;
 }
+
 
 // y = K u (device FP64 vectors of 3 * n_nodes)
 int wnfea_mf_apply(void* h, const double* u, double* y, void* stream) {

@@ -15,6 +15,7 @@
 // is algebraically B_a^T D B_b, so each of the 55 lanes owning an upper
 // block pair (a <= b) produces a full 3x3 block and also writes its transpose:
 // Ke comes out exactly symmetric, as the reference's 0.5 (Ke + Ke^T).
+#include <type_traits>
 #include "common.hpp"
 #include "wn.hpp"
 

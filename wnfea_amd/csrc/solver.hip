@@ -116,10 +116,11 @@ __device__ __forceinline__ double dpp_row16_sum(double v) {
 
 // Thread t of the block holds V = 16 B / sizeof(T) consecutive columns
 // col0 = (t % CPR) * V of NI rows t / CPR + RPI * m: every load instruction
-// of the block covers 4 KiB of the tile contiguously.
-template <typename T>
+// of the block covers 4 KiB of the tile contiguously.  T: tile values, XT:
+// the vector (an FP32 cycle may apply the FP64 inverse).
+template <typename T, typename XT = T>
 __global__ __launch_bounds__(256) void sym_tile_gemv_kernel(int n, const T* __restrict__ tiles,
-                                                           const int2* __restrict__ tij, const T* __restrict__ x,
+                                                           const int2* __restrict__ tij, const XT* __restrict__ x,
                                                            double* __restrict__ part, const int* skip) {
   if (skip && *skip) return;
   constexpr int V = 16 / sizeof(T);
@@ -216,6 +217,15 @@ __global__ __launch_bounds__(kTile * kRedGroups) void sym_tile_reduce_kernel(int
 
 template <typename T>
 static int coarse_apply(const Amg& g, const T* b, T* x, const int* skip, hipStream_t s) {
+  // an FP32 cycle with coarse_fp64 set applies the FP64 inverse (FP64
+  // products and partial sums; only the vectors stay FP32)
+  if (std::is_same<T, float>::value && g.coarse_fp64 && g.tiles64 && g.tpart && g.tij) {
+    hipLaunchKernelGGL((sym_tile_gemv_kernel<double, T>), dim3((unsigned)g.ntiles), dim3(256), 0, s, g.n_coarse,
+                       g.tiles64, g.tij, b, g.tpart, skip);
+    hipLaunchKernelGGL(sym_tile_reduce_kernel<T>, dim3(g.nt), dim3(kTile * kRedGroups), 0, s, g.n_coarse, g.nt,
+                       g.tpart, x, skip);
+    return static_cast<int>(hipGetLastError());
+  }
   const T* tiles = coarse_tiles<T>(g);
   if (tiles && g.tpart && g.tij) {
     hipLaunchKernelGGL(sym_tile_gemv_kernel<T>, dim3((unsigned)g.ntiles), dim3(256), 0, s, g.n_coarse, tiles, g.tij,
@@ -324,6 +334,12 @@ int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream
     default:
       return kBadMode;
   }
+}
+
+int amg_set_coarse_fp64(Amg& g, int fp64) {
+  if (fp64 && !g.tiles64) return kNotReady;
+  g.coarse_fp64 = fp64 ? 1 : 0;
+  return kOk;
 }
 
 // `reps` Chebyshev steps (the k = 1 coefficients) of level l < last on the

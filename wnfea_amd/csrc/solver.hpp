@@ -38,6 +38,7 @@ struct Amg {
   float* tiles32 = nullptr;
   int2* tij = nullptr;
   double* tpart = nullptr;
+  int coarse_fp64 = 0;  // FP32 cycles apply the FP64 inverse tiles instead of the FP32 copy
   SpmvTune tune;
   // optional: the finest level's smoothing and defect products matrix-free
   MfSmooth mfs;

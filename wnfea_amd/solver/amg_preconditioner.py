@@ -419,6 +419,14 @@ class SmoothedAggregationAMG:
                                                      int(bool(xcd_remap)), int(unroll)),
                    "wnfea_amg_tune_precision")
 
+    def set_coarse_precision(self, precision: str) -> None:
+        """Coarsest solve of the FP32 / FP16 cycles: "fp64" applies the FP64
+        dense inverse (needs "fp64" storage), "fp32" (default) its FP32 copy."""
+        if precision not in ("fp32", "fp64"):
+            raise ValueError(f"unknown coarse precision {precision!r}")
+        _lib.check(self.lib.wnfea_amg_set_coarse_fp64(self.handle, int(precision == "fp64")),
+                   "wnfea_amg_set_coarse_fp64")
+
     def stats(self) -> dict:
         out = []
         for lv in self.levels:
