@@ -67,11 +67,16 @@ __global__ __launch_bounds__(64 * W) void sell_spmm_kernel(SellView<VT> A, const
 #pragma unroll
           for (int k = 0; k < NR; ++k) acc[r * NR + k] = fma(AT(v[r * BC + q]), AT(xv[k]), acc[r * NR + k]);
       }
-      // all of the block's loads (column index, values, the NR-wide x rows)
-      // in flight together before its FMAs; left to itself the compiler
-      // reuses the x registers and waits between the x loads
-      __builtin_amdgcn_sched_group_barrier(0x020, kLoads, 0);
-      __builtin_amdgcn_sched_group_barrier(0x002, 2 * BB * NR, 0);
+      // FP32 3x3 blocks: all of the block's loads (column index, values, the
+      // NR-wide x rows) in flight together before its FMAs; left to itself
+      // the compiler reuses the x registers and waits between the x loads
+      // (I-beam, 8 columns: 277 -> 258 us).  The FP64 and 6x6 forms are left
+      // to the compiler: the hint raised their registers and slowed the FP64
+      // product (370 -> 467 us).
+      if constexpr (sizeof(XT) == 4 && BB <= 9) {
+        __builtin_amdgcn_sched_group_barrier(0x020, kLoads, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 2 * BB * NR, 0);
+      }
     }
     if constexpr (W > 1) {
       __shared__ AT red[W - 1][BR * NR][64];
