@@ -200,7 +200,8 @@ int wnfea_sa_tentative(int n_nodes, int b, int k, const int *agg, int n_agg, con
                        int *p_colidx, double *p_vals, double *Bc);
 
 /* Host block-sparse kernels of the smoothed-aggregation set-up (host
- * pointers, host threads; `threads` 0 = one per hardware thread).  They
+ * pointers, host threads; `threads` 0 = OMP_NUM_THREADS when set, else one
+ * per hardware thread).  They
  * replace the scipy sparse products, transposes and matrix-vector products
  * of the reference's setup (wnfea/solver/amg_preconditioner.py:241-293,
  * its Galerkin product R A P at :283-293).

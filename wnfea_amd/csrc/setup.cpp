@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -302,11 +303,21 @@ int sa_tentative(int n_nodes, int b, int k, const int* agg, int n_agg, const dou
 }
 
 // ---------------------------------------------------------------- threaded block-sparse set-up kernels
-// Runs fn(row_begin, row_end) on `threads` host threads over [0, n) (0 = one
-// per hardware thread, at most 64).
+// Host threads to use when the caller asks for 0: OMP_NUM_THREADS when set
+// (a job's CPU share), else one per hardware thread.
+static int default_threads() {
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int v = std::atoi(e);
+    if (v > 0) return v;
+  }
+  return (int)std::thread::hardware_concurrency();
+}
+
+// Runs fn(row_begin, row_end) on `threads` host threads over [0, n) (0 = the
+// default above, at most 64).
 template <class F>
 static void parallel_rows(int n, int threads, F fn) {
-  int t = threads > 0 ? threads : (int)std::thread::hardware_concurrency();
+  int t = threads > 0 ? threads : default_threads();
   t = std::max(1, std::min({t, 64, std::max(1, n / 64)}));
   if (t == 1) {
     fn(0, n);
