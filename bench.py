@@ -332,7 +332,26 @@ def mf_bytes(n_elements: int, n_nodes: int, s: int) -> dict:
     return {"elem": elem, "gather_cheb": gather}
 
 
-def mf_only(system, model, reps: int, log) -> int:
+def mf_patch_bytes(n_elements: int, n_nodes: int, n_local: int, n_patch: int, stride: int, s: int,
+                   folded: bool) -> dict:
+    """Algorithmic bytes of the patch layout's two kernels with s-byte values.
+    The patch kernel reads per element its 16-bit local connectivity (20 B),
+    quadrature record (folded FP32: J^-1 and the weighted moduli, 12 values;
+    FP64: J^-1 and det J / 24, 10 values, plus the element id and E, nu,
+    20 B) and slot codes (20 B), per patch its stride-padded local-node ids
+    and list starts (6 B each), every node's value and scaling once (6
+    values), and writes one partial sum (3 values) per local node.  The node
+    kernel reads per node its 16-byte head and Dirichlet byte, per local node
+    its partial sum (3 values), and the epilogue's operands: a Chebyshev step
+    reads b, D^-1, x, d, s (15 values) and writes x', d' (6); the FP64
+    product reads u (3) and writes y (3)."""
+    per_elem = 20 + 20 + (12 * s if folded else 10 * s + 20)
+    patch = n_elements * per_elem + n_patch * stride * 6 + n_local * 3 * s + n_nodes * 6 * s
+    node = n_nodes * (16 + 1) + n_local * 3 * s
+    return {"patch": patch, "node_cheb": node + n_nodes * 21 * s, "node_apply": node + n_nodes * 6 * s}
+
+
+def mf_only(system, model, reps: int, log, patch: int = -1) -> int:
     """Matrix-free kernels alone on the resident I-beam: `reps` FP64 operator
     applications (element kernel + gather), then `reps` FP32 V-cycles with the
     finest level's smoothing and defect matrix-free and `reps` with them on
@@ -341,6 +360,8 @@ def mf_only(system, model, reps: int, log) -> int:
     import torch
     from wnfea_amd.solver.solid_solver import MatrixFreeOperator, build_amg
     op = MatrixFreeOperator(system)
+    if patch >= 0:
+        op.set_patch(patch)
     amg = build_amg(system, storage=("fp64", "fp32"))
     u = torch.randn(system.n_dofs, dtype=torch.float64, device=system.K.device)
     r = torch.randn(system.n_dofs, dtype=torch.float32, device=system.K.device)
@@ -371,9 +392,13 @@ def mf_only(system, model, reps: int, log) -> int:
         c.copy_(a)
     torch.cuda.synchronize()
     ne, nn = system.mesh["n_elements"], system.n_nodes
+    pinfo = op.patch_info
+    pb = {p: mf_patch_bytes(ne, nn, pinfo["local_nodes"], pinfo["patches"], pinfo["stride"], s, folded=(s == 4))
+          for p, s in (("fp32", 4), ("fp64", 8))}
     print(json.dumps({"mf64_apply_us": t_apply * 1e6, "vcycle32_matrix_free_us": t_mf * 1e6,
-                      "vcycle32_assembled_us": t_asm * 1e6, "affine": op.affine,
+                      "vcycle32_assembled_us": t_asm * 1e6, "affine": op.affine, "layout": pinfo,
                       "bytes_fp32": mf_bytes(ne, nn, 4), "bytes_fp64": mf_bytes(ne, nn, 8),
+                      "patch_bytes_fp32": pb["fp32"], "patch_bytes_fp64": pb["fp64"],
                       "copy_bytes_each_way": 4 * n}), flush=True)
     op.close()
     amg.close()
@@ -764,6 +789,9 @@ def main() -> int:
                          "for rocprofv3 --pmc")
     ap.add_argument("--no-ir", action="store_true", help="skip the restart-IR (FP32 Krylov) scheme")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--mf-patch", type=int, default=-1,
+                    help="elements per patch of the matrix-free operator's patch layout (0: per-element "
+                         "contributions and node gather; default: the library's layout)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the solve path has no multi-GPU implementation (it would shard "
@@ -820,7 +848,7 @@ def main() -> int:
     if args.spmv_only:
         return spmv_only(system, info, args.spmv_reps, log)
     if args.mf_only:
-        return mf_only(system, model, args.spmv_reps, log)
+        return mf_only(system, model, args.spmv_reps, log, args.mf_patch)
     t0 = time.perf_counter()
     # strength thresholds per workload (sweep on the I-beam: 0.1 on the finest
     # level and 0.04 below it take 44 instead of 50 iterations)
@@ -838,6 +866,9 @@ def main() -> int:
     # every result is verified with the assembled FP64 K
     from wnfea_amd.solver.solid_solver import MatrixFreeOperator
     mf_op = MatrixFreeOperator(system)
+    if args.mf_patch >= 0:
+        mf_op.set_patch(args.mf_patch)
+    log(f"[bench] matrix-free layout {mf_op.patch_info}")
     main_op = mf_op if args.operator == "matrix_free" else None
     alt_op = None if args.operator == "matrix_free" else mf_op
     alt_name = "assembled" if args.operator == "matrix_free" else "matrix_free"
@@ -1056,9 +1087,14 @@ def main() -> int:
     torch.cuda.synchronize()
     t_mf = st_ev.elapsed_time(en_ev) / 1e3 / args.spmv_reps
     ne_mf = system.mesh["n_elements"]
-    b_mf = mf_bytes(ne_mf, nb, 8)["elem"] + ne_mf * (40 + 30 * 8) + nb * (4 + 1 + 3 * 8)
+    pinfo = mf_op.patch_info
+    if pinfo["elems"]:
+        pb = mf_patch_bytes(ne_mf, nb, pinfo["local_nodes"], pinfo["patches"], pinfo["stride"], 8, folded=False)
+        b_mf = pb["patch"] + pb["node_apply"]
+    else:
+        b_mf = mf_bytes(ne_mf, nb, 8)["elem"] + ne_mf * (40 + 30 * 8) + nb * (4 + 1 + 3 * 8)
     mf_info = {"apply_fp64_us": round(t_mf * 1e6, 2), "algorithmic_bytes": b_mf,
-               "gbs": round(b_mf / t_mf / 1e9, 1), "affine": mf_op.affine}
+               "gbs": round(b_mf / t_mf / 1e9, 1), "affine": mf_op.affine, "layout": pinfo}
     log(f"[bench] matrix-free FP64 apply {t_mf * 1e6:.1f} us {b_mf / t_mf / 1e9:.0f} GB/s algorithmic")
     # FP64 reference solution (device FP64 AMG-PCG to the FP64 residual floor)
     # and the device von Mises fields of the mixed solution, for the oracle check

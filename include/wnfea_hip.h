@@ -83,6 +83,25 @@ void wnfea_mf_destroy(void *handle);
  * handle.  No reference counterpart. */
 int wnfea_mf_affine(void *handle);
 
+/* Patch layout of the operator's products (affine meshes; built by
+ * wnfea_mf_create with 64 elements per patch when it fits): elements in
+ * Morton order of their centroids, cut into patches of `elems` (a multiple
+ * of 64, at most 256); one workgroup per patch stages the patch's nodal
+ * values and element forces in LDS and writes one partial sum per (patch,
+ * node), and a node-ordered kernel adds each node's partials in patch order.
+ * elems == 0 drops the layout (per-element contributions and a node gather,
+ * as the reference's c3d10_internal_forces_kernel + gather,
+ * wnfea/solver/native/hip_kernels.cpp:283-425).  -3 for a curved mesh, -2
+ * when an FP64 patch would not fit a workgroup's LDS.  Hierarchies attached
+ * before use the per-element path until attached again.  No reference
+ * counterpart. */
+int wnfea_mf_set_patch(void *handle, int elems, void *stream);
+
+/* out[0..5] (host) = elements per patch (0: no patch layout), patches, most
+ * local nodes of one patch, local nodes of all patches, affine (1/0), the
+ * per-patch stride of the local-node arrays.  No reference counterpart. */
+int wnfea_mf_info(void *handle, long long *out);
+
 /* y = K u, device FP64 vectors of 3 * n_nodes (matrix-free, deterministic). */
 int wnfea_mf_apply(void *handle, const double *u, double *y, void *stream);
 
@@ -229,6 +248,33 @@ long long wnfea_host_bsr_symmetrize(int n, int b, const int *ptr, const int *idx
 /* y = A x for a square block CSR (b <= 8) on host threads. */
 int wnfea_host_bsr_spmv(int n, int b, const int *ptr, const int *idx, const double *val, const double *x, double *y,
                         int threads);
+
+/* Host set-up of the matrix-free patch layout (host pointers; what
+ * wnfea_mf_set_patch builds and uploads).  order[M] = the elements sorted by
+ * the Morton code of their corner centroids (21 bits per axis on the mesh's
+ * bounding cube; ties in element order).  No reference counterpart (the
+ * reference's matrix-free kernel walks elements in mesh order,
+ * wnfea/solver/native/hip_kernels.cpp:283-425). */
+int wnfea_host_morton_order(int M, const int *conn, const double *nodes, int *order);
+
+/* Patches of `elems` consecutive slots of `order` (slot s holds element
+ * order[s]; a patch's local nodes are its elements' distinct nodes,
+ * ascending).  wnfea_host_mf_patch_sizes returns the number of local nodes
+ * of all patches and *lmax, the most of one patch.  wnfea_host_mf_patches
+ * fills, for stride > lmax: pnode[p stride + j] the global node of local
+ * node j of patch p (-1 past the last); pconn[10 M] the local node of corner
+ * i of slot s; pslot[10 M] per patch the local nodes' slot codes
+ * 10 (s - p elems) + i, each list ascending, local node j's list from
+ * poff[p stride + j] to poff[p stride + j + 1] (entries past the last node
+ * hold the end); nhead[4 n_nodes] node n's partial slots p stride + j in
+ * patch order, the first three (-1 where absent) and their count, or with
+ * more than three -(k + 1), where novf[k] is the count of the rest and
+ * novf[k + 1 ...] the rest (novf needs room for the local-node count + 1).
+ * Returns the length of novf. */
+long long wnfea_host_mf_patch_sizes(int M, int n_nodes, const int *conn, const int *order, int elems, int *lmax);
+long long wnfea_host_mf_patches(int M, int n_nodes, const int *conn, const int *order, int elems, int stride,
+                                int *pnode, unsigned short *pconn, unsigned short *poff, unsigned short *pslot,
+                                int *nhead, int *novf);
 
 /* ---------------------------------------------------------------- block matrices */
 

@@ -46,6 +46,9 @@ SIGNATURES = {
     "wnfea_host_bsr_transpose": (_i, [_i, _i, _i, _i, _p, _p, _p, _p, _p, _p]),
     "wnfea_host_bsr_symmetrize": (_ll, [_i, _i, _p, _p, _p, _i]),
     "wnfea_host_bsr_spmv": (_i, [_i, _i, _p, _p, _p, _p, _p, _i]),
+    "wnfea_host_morton_order": (_i, [_i, _p, _p, _p]),
+    "wnfea_host_mf_patch_sizes": (_ll, [_i, _i, _p, _p, _i, _pi]),
+    "wnfea_host_mf_patches": (_ll, [_i, _i, _p, _p, _i, _i, _p, _p, _p, _p, _p, _p]),
     "wnfea_bsr_create": (_p, [_i, _i, _i, _i, _p, _p, _i, _i]),
     "wnfea_bsr_destroy": (None, [_p]),
     "wnfea_bsr_info": (_i, [_p, _pll]),
@@ -81,6 +84,8 @@ SIGNATURES = {
     "wnfea_pcg_mf": (_i, [_p, _p, _i, _p, _p, _i, _p, _p, _p, _d, _i, _i, _pi, _pd, _p]),
     "wnfea_amg_attach_mf": (_i, [_p, _p, _p, _p]),
     "wnfea_mf_affine": (_i, [_p]),
+    "wnfea_mf_set_patch": (_i, [_p, _i, _p]),
+    "wnfea_mf_info": (_i, [_p, _pll]),
     "wnfea_pcg_multi": (_i, [_p, _p, _p, _i, _p, _i, _p, _p, _d, _i, _i, _pi, _pd, _p]),
     "wnfea_bsr_spmm": (_i, [_p, _i, _i, _p, _p, _p]),
 }

@@ -231,6 +231,29 @@ int wnfea_host_bsr_spmv(int n, int b, const int* ptr, const int* idx, const doub
   return kOk;
 }
 
+// host set-up of the matrix-free operator's patch layout (host pointers)
+int wnfea_host_morton_order(int M, const int* conn, const double* nodes, int* order) {
+  if (M < 0 || (M > 0 && (!conn || !nodes || !order))) return kBadArg;
+  morton_order(M, conn, nodes, order);
+  return kOk;
+}
+
+long long wnfea_host_mf_patch_sizes(int M, int n_nodes, const int* conn, const int* order, int elems, int* lmax) {
+  if (M < 0 || n_nodes < 0 || elems <= 0 || !lmax || (M > 0 && (!conn || !order))) return kBadArg;
+  const long long r = mf_patch_sizes(M, n_nodes, conn, order, elems, lmax);
+  return r < 0 ? kBadArg : r;
+}
+
+long long wnfea_host_mf_patches(int M, int n_nodes, const int* conn, const int* order, int elems, int stride,
+                                int* pnode, unsigned short* pconn, unsigned short* poff, unsigned short* pslot,
+                                int* nhead, int* novf) {
+  if (M < 0 || n_nodes < 0 || elems <= 0 || !nhead ||
+      (M > 0 && (!conn || !order || !pnode || !pconn || !poff || !pslot || !novf)))
+    return kBadArg;
+  const long long r = mf_patches(M, n_nodes, conn, order, elems, stride, pnode, pconn, poff, pslot, nhead, novf);
+  return r < 0 ? kBadArg : r;
+}
+
 // ------------------------------------------------------------ block matrices
 void* wnfea_bsr_create(int br, int bc, int n_rows, int n_cols, const int* rowptr, const int* colidx, int sigma,
                        int lanes) {
@@ -546,6 +569,32 @@ void wnfea_mf_destroy(void* h) {
 int wnfea_mf_affine(void* h) {
   if (!h) return kBadArg;
   return static_cast<WnMf*>(h)->op.affine ? 1 : 0;
+}
+
+// Patch layout of the operator's products: elems > 0 (a multiple of 64, at
+// most 256) rebuilds it with that many elements per patch, 0 drops it (the
+// per-element contributions and node gather are used instead).  Affine
+// meshes only (-3 otherwise); -2 when one patch of the FP64 product would
+// not fit a workgroup's LDS.  Hierarchies attached before keep running, on
+// the per-element path, until attached again.
+int wnfea_mf_set_patch(void* h, int elems, void* stream) {
+  if (!h) return kBadArg;
+  return mf_set_patch(static_cast<WnMf*>(h)->op, elems, S(stream));
+}
+
+// out[0..5] = elements per patch (0: no patch layout), patches, most local
+// nodes of one patch, local nodes of all patches, affine (1/0), the
+// per-patch stride of the local-node arrays
+int wnfea_mf_info(void* h, long long* out) {
+  if (!h || !out) return kBadArg;
+  const MfOp& op = static_cast<WnMf*>(h)->op;
+  out[0] = op.patch;
+  out[1] = op.n_patch;
+  out[2] = op.lmax;
+  out[3] = op.n_local;
+  out[4] = op.affine;
+  out[5] = op.stride;
+  return kOk;
 }
 
 

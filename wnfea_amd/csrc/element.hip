@@ -15,7 +15,9 @@
 // is algebraically B_a^T D B_b, so each of the 55 lanes owning an upper
 // block pair (a <= b) produces a full 3x3 block and also writes its transpose:
 // Ke comes out exactly symmetric, as the reference's 0.5 (Ke + Ke^T).
+#include <algorithm>
 #include <type_traits>
+#include <utility>
 #include "common.hpp"
 #include "wn.hpp"
 
@@ -583,11 +585,13 @@ __global__ void qdata_affine_kernel(long long n, const double* __restrict__ qd, 
 }
 
 // v from the lane given by a quad permutation (DPP, within groups of 4
-// lanes).  The FP32 form takes 0 for the old value, which lets the compiler
-// fold the permutation into the consuming add (v_add_f32_dpp).
+// lanes; every lane reads a lane of its own quad, so all are written).  The
+// FP32 form has no old value (bound_ctrl set), which lets the compiler fold
+// the permutation into the consuming add or multiply (v_add_f32_dpp) without
+// first clearing the destination.
 template <int Ctrl>
 __device__ __forceinline__ float quad_perm(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), Ctrl, 0xF, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), Ctrl, 0xF, 0xF, true));
 }
 template <int Ctrl>
 __device__ __forceinline__ double quad_perm(double v) {
@@ -634,6 +638,103 @@ __device__ __forceinline__ T quad_sum(T v) {
 __device__ constexpr int kEdgeP[6] = {0, 1, 2, 0, 1, 2};
 __device__ constexpr int kEdgeQ[6] = {1, 2, 0, 3, 3, 3};
 
+// The element forces of one quad of lanes (lane g: Gauss point g; see
+// c3d10_pa_elem_kernel): from node i's scaled values (get(i, u) fills u[3]),
+// the point's inverse Jacobian Ji and the weighted moduli w11 = w c11,
+// w12 = w c12, w44 = w c44 (w = det J / 24), the quad-summed forces of the
+// lane's nodes g, g + 4, g + 8 (out[j] for node g + 4 j; out[2] stays 0
+// for g >= 2).
+template <typename T, typename Get>
+__device__ __forceinline__ void pa_forces(int g, const T (&c4)[4], const T (&cc)[4], const T (&Ji)[9], T w11, T w12,
+                                          T w44, Get get, T (&out)[3][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[j][c] = T(0);
+  T Q[4][3];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    T u[3];
+    get(i, u);
+    if (i < 4) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Q[i][c] = cc[i] * u[c];  // corners come first: initialise
+    } else {
+      const int p = kEdgeP[i - 4], q = kEdgeQ[i - 4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        Q[p][c] += c4[q] * u[c];
+        Q[q][c] += c4[p] * u[c];
+      }
+    }
+  }
+  T H[3][3];  // H = J^-1 G,  G[a][c] = Q[a+1][c] - Q[0][c]
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T g0 = Q[1][c] - Q[0][c], g1 = Q[2][c] - Q[0][c], g2 = Q[3][c] - Q[0][c];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) H[r][c] = Ji[3 * r] * g0 + Ji[3 * r + 1] * g1 + Ji[3 * r + 2] * g2;
+  }
+  const T e0 = H[0][0], e1 = H[1][1], e2 = H[2][2];
+  const T s0 = w11 * e0 + w12 * (e1 + e2);
+  const T s1 = w11 * e1 + w12 * (e0 + e2);
+  const T s2 = w11 * e2 + w12 * (e0 + e1);
+  const T s3 = w44 * (H[1][0] + H[0][1]), s4 = w44 * (H[2][1] + H[1][2]), s5 = w44 * (H[2][0] + H[0][2]);
+  // S = [[s0 s3 s5] [s3 s1 s4] [s5 s4 s2]];  R_{k+1} = column k of S J^-1, R_0 = -(R_1 + R_2 + R_3)
+  T R[4][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    R[k + 1][0] = s0 * Ji[k] + s3 * Ji[3 + k] + s5 * Ji[6 + k];
+    R[k + 1][1] = s3 * Ji[k] + s1 * Ji[3 + k] + s4 * Ji[6 + k];
+    R[k + 1][2] = s5 * Ji[k] + s4 * Ji[3 + k] + s2 * Ji[6 + k];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) R[0][c] = -(R[1][c] + R[2][c] + R[3][c]);
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    T fi[3];
+    if (i < 4) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fi[c] = R[i][c] * cc[i];
+    } else {
+      const int p = kEdgeP[i - 4], q = kEdgeQ[i - 4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) fi[c] = R[p][c] * c4[q] + R[q][c] * c4[p];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fi[c] = quad_sum(fi[c]);
+    if ((i & 3) == g) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) out[i >> 2][c] = fi[c];
+    }
+  }
+}
+
+// Weighted isotropic moduli (w c11, w c12, w c44) from E, nu (in FP64, rounded to T).
+template <typename T>
+__device__ __forceinline__ void weighted_moduli(T wd, double Ev, double nv, T& w11, T& w12, T& w44) {
+  const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
+  w11 = wd * T(f * (1.0 - nv));
+  w12 = wd * T(f * nv);
+  w44 = wd * T(Ev / (2.0 * (1.0 + nv)));
+}
+
+// 4 L_k and 4 L_k - 1 at Gauss point g of the 4-point rule: L_k = b for
+// k = g, else a, with a = (5 - sqrt 5) / 20 and b = (5 + 3 sqrt 5) / 20
+// (= 1 - 3 a); the products with 4 are exact, so every constant is one
+// selection between two literals.
+template <typename T>
+__device__ __forceinline__ void gauss_point_consts(int g, T (&c4)[4], T (&cc)[4]) {
+  constexpr double ga = 0.13819660112501051517954131656343619;  // (5 - sqrt 5) / 20
+  constexpr double gb = 0.58541019662496845446137605030969143;  // (5 + 3 sqrt 5) / 20
+  constexpr T a4 = T(4.0 * ga), b4 = T(4.0 * gb), a4m = T(4.0 * ga) - T(1), b4m = T(4.0 * gb) - T(1);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    c4[k] = k == g ? b4 : a4;
+    cc[k] = k == g ? b4m : a4m;
+  }
+}
+
 // fe = K_e (sc .* u)_e from the quadrature data, stored [i][e][3] for local
 // node i, where sc is a per-DOF scaling that also carries the Dirichlet rule
 // (0 on a fixed DOF: its column of the operator is zero).  Four lanes per
@@ -663,16 +764,8 @@ __global__ __launch_bounds__(256) void c3d10_pa_elem_kernel(int M, const int* __
   const long long t = (long long)lb * blockDim.x + threadIdx.x;
   const int e = (int)(t >> 2), g = (int)(t & 3);
   if (e >= M) return;  // whole quads leave together
-  // barycentric coordinates of Gauss point g: (a, a, a) and its permutations with one b
-  const double ga = (5.0 - sqrt(5.0)) / 20.0, gb = (5.0 + 3.0 * sqrt(5.0)) / 20.0;
-  const double p0 = g == 1 ? gb : ga, p1 = g == 2 ? gb : ga, p2 = g == 3 ? gb : ga;
-  const T L[4] = {T(1.0 - p0 - p1 - p2), T(p0), T(p1), T(p2)};
   T c4[4], cc[4];  // 4 L_k and 4 L_k - 1
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    c4[k] = T(4) * L[k];
-    cc[k] = c4[k] - T(1);
-  }
+  gauss_point_consts<T>(g, c4, cc);
   // Loads in two rounds, each issued together: first everything that does
   // not depend on another load (the connectivity of this lane's nodes g,
   // g + 4, g + 8, the quadrature data, the moduli), then the nodal values
@@ -701,68 +794,24 @@ __global__ __launch_bounds__(256) void c3d10_pa_elem_kernel(int M, const int* __
     for (int c = 0; c < 3; ++c) own[j][c] = keep * (U[n3 + c] * sc[n3 + c]);
   }
   __builtin_amdgcn_sched_group_barrier(0x020, 18, 0);
-  T Q[4][3];
+  T w11, w12, w44;
+  weighted_moduli<T>(wd, Ev, nv, w11, w12, w44);
+  T out[3][3];
+  pa_forces<T>(g, c4, cc, Ji, w11, w12, w44,
+               [&](int i, T (&u)[3]) {
 #pragma unroll
-  for (int i = 0; i < 10; ++i) {
-    T u[3];
+                 for (int c = 0; c < 3; ++c) u[c] = quad_bcast(own[i >> 2][c], i & 3);
+               },
+               out);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) u[c] = quad_bcast(own[i >> 2][c], i & 3);
-    if (i < 4) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) Q[i][c] = cc[i] * u[c];  // corners come first: initialise
-    } else {
-      const int p = kEdgeP[i - 4], q = kEdgeQ[i - 4];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        Q[p][c] += c4[q] * u[c];
-        Q[q][c] += c4[p] * u[c];
-      }
-    }
-  }
-  T H[3][3];  // H = J^-1 G,  G[a][c] = Q[a+1][c] - Q[0][c]
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const T g0 = Q[1][c] - Q[0][c], g1 = Q[2][c] - Q[0][c], g2 = Q[3][c] - Q[0][c];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) H[r][c] = Ji[3 * r] * g0 + Ji[3 * r + 1] * g1 + Ji[3 * r + 2] * g2;
-  }
-  const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
-  const T c11 = T(f * (1.0 - nv)), c12 = T(f * nv), c44 = T(Ev / (2.0 * (1.0 + nv)));
-  const T e0 = H[0][0], e1 = H[1][1], e2 = H[2][2];
-  const T s0 = wd * (c11 * e0 + c12 * (e1 + e2));
-  const T s1 = wd * (c11 * e1 + c12 * (e0 + e2));
-  const T s2 = wd * (c11 * e2 + c12 * (e0 + e1));
-  const T s3 = wd * c44 * (H[1][0] + H[0][1]), s4 = wd * c44 * (H[2][1] + H[1][2]),
-          s5 = wd * c44 * (H[2][0] + H[0][2]);
-  // S = [[s0 s3 s5] [s3 s1 s4] [s5 s4 s2]];  R_{k+1} = column k of S J^-1, R_0 = -(R_1 + R_2 + R_3)
-  T R[4][3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    R[k + 1][0] = s0 * Ji[k] + s3 * Ji[3 + k] + s5 * Ji[6 + k];
-    R[k + 1][1] = s3 * Ji[k] + s1 * Ji[3 + k] + s4 * Ji[6 + k];
-    R[k + 1][2] = s5 * Ji[k] + s4 * Ji[3 + k] + s2 * Ji[6 + k];
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) R[0][c] = -(R[1][c] + R[2][c] + R[3][c]);
-#pragma unroll
-  for (int i = 0; i < 10; ++i) {
-    T fi[3];
-    if (i < 4) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) fi[c] = R[i][c] * cc[i];
-    } else {
-      const int p = kEdgeP[i - 4], q = kEdgeQ[i - 4];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) fi[c] = R[p][c] * c4[q] + R[q][c] * c4[p];
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) fi[c] = quad_sum(fi[c]);
-    if ((i & 3) == g) {
+  for (int j = 0; j < 3; ++j) {
+    const int i = g + 4 * j;
+    if (i < 10) {
       // layout [local node][element][3]
       T* o = fe + ((size_t)i * M + e) * 3;
-      o[0] = fi[0];
-      o[1] = fi[1];
-      o[2] = fi[2];
+      o[0] = out[j][0];
+      o[1] = out[j][1];
+      o[2] = out[j][2];
     }
   }
 }
@@ -981,6 +1030,316 @@ __global__ __launch_bounds__(64 * kGatherWaves) void mf_gather_op_kernel(
   }
 }
 
+// ---------------------------------------------------------------- patch layout
+// One workgroup of 4 P lanes per patch of P consecutive element slots
+// (elements sorted along a Morton curve, so a patch is a compact piece of the
+// mesh), one quad of lanes per element (lane g: Gauss point g):
+//  1. loads addressed by the patch index alone are issued first: each
+//     element's 16-bit local connectivity and quadrature data, the patch's
+//     slot codes (into LDS) and its local nodes' global ids and list starts
+//     (fixed stride per patch); then those addressed through them: the
+//     scaled nodal values sc .* U of the local nodes (into LDS, four entries
+//     per node; up to three nodes per lane, loads issued together) and,
+//     unfolded, the element moduli;
+//  2. each quad computes its element's forces as c3d10_pa_elem_kernel does,
+//     every lane reading the element's ten nodal values from LDS, and keeps
+//     them in LDS;
+//  3. each local node adds its element forces in slot order (slot codes
+//     ascending, from LDS) and writes the sum to its partial slot
+//     p stride + j (contiguous per patch).
+// The quadrature data is one record per slot: Ji and det J / 24 ([M][10],
+// the moduli read through the slot's element id), or Folded, Ji and the
+// weighted moduli w c11, w c12, w c44 ([M][12], made when a smoother is
+// attached).
+// The element forces never leave the chip; a node interior to a patch has a
+// single partial, a node on a patch boundary one per patch that touches it.
+constexpr int kStageSlots = 3;  // local nodes per lane in step 1: 3 * 4 P >= the stride
+
+__device__ __forceinline__ void lds_node(const float* xs, int li, float (&u)[3]) {
+  const float4 v = *reinterpret_cast<const float4*>(xs + 4 * li);
+  u[0] = v.x;
+  u[1] = v.y;
+  u[2] = v.z;
+}
+__device__ __forceinline__ void lds_node(const double* xs, int li, double (&u)[3]) {
+  const double2 v = *reinterpret_cast<const double2*>(xs + 4 * li);
+  u[0] = v.x;
+  u[1] = v.y;
+  u[2] = xs[4 * li + 2];
+}
+__device__ __forceinline__ void lds_store_node(float* xs, int j, const float (&v)[3]) {
+  *reinterpret_cast<float4*>(xs + 4 * j) = make_float4(v[0], v[1], v[2], 0.0f);
+}
+__device__ __forceinline__ void lds_store_node(double* xs, int j, const double (&v)[3]) {
+  *reinterpret_cast<double2*>(xs + 4 * j) = make_double2(v[0], v[1]);
+  xs[4 * j + 2] = v[2];
+}
+
+template <typename T, int P, bool Folded>
+__global__ __launch_bounds__(4 * P) void mf_patch_kernel(
+    int n_patch, int M, int stride, const int* __restrict__ pnode, const unsigned short* __restrict__ pconn,
+    const unsigned short* __restrict__ poff, const unsigned short* __restrict__ pslot, const int* __restrict__ pe,
+    const T* __restrict__ qdp, const double* __restrict__ E, const double* __restrict__ nu,
+    const T* __restrict__ sc, const T* __restrict__ U, T* __restrict__ part, const int* skip) {
+  constexpr int NT = 4 * P;
+  if (skip && *skip) return;
+  extern __shared__ __align__(16) unsigned char patch_lds[];
+  T* xs = reinterpret_cast<T*>(patch_lds);                              // [stride][4] scaled nodal values
+  T* fl = xs + 4 * stride;                                               // [P][10][3] element forces
+  unsigned short* ls = reinterpret_cast<unsigned short*>(fl + 30 * P);  // [10 P] slot codes
+  unsigned short* lo = ls + 10 * P;                                       // [stride] list starts
+  const int p = slice_of_block(n_patch, 1);  // XCD-contiguous patch ranges
+  if (p >= n_patch) return;
+  const int t = (int)threadIdx.x;
+  // Lane offsets are unsigned 32-bit from uniform per-array bases (the
+  // set-up keeps 12 M and 3 n_patch stride below 2^31), so the loads take
+  // the scalar-base + vector-offset form and share their offsets.
+  const int e0 = p * P, ne = min(P, M - e0);
+  const size_t pb = (size_t)p * stride;
+  // step 1, first round: addressed by the patch index
+  const int el = t >> 2, g = t & 3;
+  const bool live = el < ne;
+  const unsigned su = (unsigned)(e0 + (live ? el : 0));  // clamped: the loads stay unconditional
+  int lc[10];
+  {
+    const unsigned short* pc = pconn + (size_t)su * 10;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) lc[i] = pc[i];
+  }
+  // the slot's quadrature record: 16-byte aligned (48 B folded FP32, 80 B FP64)
+  constexpr int QW = Folded ? 12 : 10;
+  T qv[QW];
+  {
+    const T* q = qdp + (size_t)su * QW;
+#pragma unroll
+    for (int k = 0; k < QW; ++k) qv[k] = q[k];
+  }
+  T Ji[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Ji[k] = qv[k];
+  T w11 = T(0), w12 = T(0), w44 = T(0), wd = T(0);
+  int ee = 0;
+  if constexpr (Folded) {
+    w11 = qv[QW - 3];
+    w12 = qv[QW - 2];
+    w44 = qv[QW - 1];
+  } else {
+    wd = qv[9];
+    ee = pe[su];
+  }
+  const unsigned short* psl = pslot + (size_t)e0 * 10;
+  unsigned short code[3];  // slot codes t, t + NT, t + 2 NT (10 P < 3 NT)
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const unsigned q = (unsigned)(t + k * NT);
+    code[k] = psl[q < (unsigned)(10 * ne) ? q : 0u];
+  }
+  const int* pn = pnode + pb;
+  const unsigned short* po = poff + pb;
+  int gn[kStageSlots];
+  unsigned short off[kStageSlots];
+#pragma unroll
+  for (int k = 0; k < kStageSlots; ++k) {
+    const unsigned j = (unsigned)min(t + k * NT, stride - 1);
+    gn[k] = pn[j];
+    off[k] = po[j];
+  }
+  // second round: addressed through the first
+  double Ev = 0.0, nv = 0.0;
+  if constexpr (!Folded) {
+    Ev = E[ee];
+    nv = nu[ee];
+  }
+  T xv[kStageSlots][3];
+#pragma unroll
+  for (int k = 0; k < kStageSlots; ++k) {
+    const size_t n3 = 3 * (size_t)max(gn[k], 0);
+    const T* uk = U + n3;
+    const T* sk = sc + n3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xv[k][c] = uk[c] * sk[c];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if (t + k * NT < 10 * ne) ls[t + k * NT] = code[k];
+#pragma unroll
+  for (int k = 0; k < kStageSlots; ++k) {
+    const int j = t + k * NT;
+    if (j < stride) {
+      lo[j] = off[k];
+      lds_store_node(xs, j, xv[k]);
+    }
+  }
+  __syncthreads();
+  // step 2: the element forces
+  if constexpr (!Folded) weighted_moduli<T>(wd, Ev, nv, w11, w12, w44);
+  T c4[4], cc[4];
+  gauss_point_consts<T>(g, c4, cc);
+  T out[3][3];
+  pa_forces<T>(g, c4, cc, Ji, w11, w12, w44, [&](int i, T (&u)[3]) { lds_node(xs, lc[i], u); }, out);
+  if (live) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int i = g + 4 * j;
+      if (i < 10) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) fl[(el * 10 + i) * 3 + c] = out[j][c];
+      }
+    }
+  }
+  __syncthreads();
+  // step 3: per local node, its element forces in slot order (the entry
+  // after the last node holds the end of its list)
+#pragma unroll
+  for (int k = 0; k < kStageSlots; ++k) {
+    const int j = t + k * NT;
+    if (j < stride && gn[k] >= 0) {
+      const int o0 = lo[j], o1 = lo[j + 1];
+      T a0 = T(0), a1 = T(0), a2 = T(0);
+      for (int q = o0; q < o1; ++q) {
+        const int f = 3 * ls[q];
+        a0 += fl[f];
+        a1 += fl[f + 1];
+        a2 += fl[f + 2];
+      }
+      T* o = part + pb * 3;
+      const unsigned j3 = 3u * (unsigned)j;
+      o[j3] = a0;
+      o[j3 + 1] = a1;
+      o[j3 + 2] = a2;
+    }
+  }
+}
+
+// Folded FP32 quadrature records of the patch layout: per slot the rounded
+// inverse Jacobian and w c11, w c12, w c44 (formed in FP64 from the slot's
+// element moduli), [M][12].
+__global__ void qdata_fold_kernel(int M, const int* __restrict__ pe, const double* __restrict__ qdp,
+                                  const double* __restrict__ E, const double* __restrict__ nu,
+                                  float* __restrict__ out) {
+  for (int sl = blockIdx.x * blockDim.x + threadIdx.x; sl < M; sl += gridDim.x * blockDim.x) {
+    const double* q = qdp + (size_t)sl * 10;
+    float* o = out + (size_t)sl * 12;
+    for (int k = 0; k < 9; ++k) o[k] = (float)q[k];
+    const double wd = q[9], Ev = E[pe[sl]], nv = nu[pe[sl]];
+    const double f = Ev / ((1.0 + nv) * (1.0 - 2.0 * nv));
+    o[9] = (float)(wd * f * (1.0 - nv));
+    o[10] = (float)(wd * f * nv);
+    o[11] = (float)(wd * Ev / (2.0 * (1.0 + nv)));
+  }
+}
+
+// Node-ordered completion of the patch layout: node n's partial sums (its
+// head: the first three slots and the count, the rest in an overflow list;
+// all in patch order) are added, the assembled operator's Dirichlet rule
+// applied (a fixed DOF k gives s_k^2 x_k, the unit diagonal of the scaled
+// operator) and the epilogue Op consumes a_k = s_k (K (sc .* x))_k as in
+// mf_gather_op_kernel (s = 1 without a scaling).  One lane per node; every
+// operand and result streams in node order, and the partial sums are one
+// dependent load behind the head.  Ops with dot products store one partial
+// per block of the padded grid (zero for blocks without nodes).
+constexpr int kNodeThreads = 256;
+
+template <typename T, typename Op>
+__global__ __launch_bounds__(kNodeThreads) void mf_node_kernel(int n_nodes, const int4* __restrict__ nhead,
+                                                              const int* __restrict__ novf, const T* __restrict__ part,
+                                                              const unsigned char* __restrict__ fix,
+                                                              const T* __restrict__ sc, const T* __restrict__ x,
+                                                              Op op, const int* skip) {
+  if (skip && *skip) return;
+  const int nblk = (n_nodes + kNodeThreads - 1) / kNodeThreads;
+  const int lb = slice_of_block(nblk, 1);
+  const int n = lb * kNodeThreads + (int)threadIdx.x;
+  const bool has = lb < nblk && n < n_nodes;
+  const int nn = has ? n : 0;
+  const int4 h = nhead[nn];
+  const int mraw = fix ? fix[nn] : 0;
+  T s3[3] = {T(1), T(1), T(1)}, xr[3];
+  typename Op::Pre pre[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const long long k = (long long)nn * 3 + c;
+    op.load(k, pre[c]);
+    if (sc) s3[c] = sc[k];
+    xr[c] = x[k];
+  }
+  const int hi[3] = {h.x, h.y, h.z};
+  T v[3][3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[q][c] = part[(size_t)max(hi[q], 0) * 3 + c];
+  T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (hi[q] >= 0) acc[c] = q == 0 ? v[q][c] : acc[c] + v[q][c];
+  if (h.w < 0) {  // more than three partials
+    const int k = -h.w - 1, cnt = novf[k];
+    for (int q = 1; q <= cnt; ++q) {
+      const T* w = part + (size_t)novf[k + q] * 3;
+      acc[0] += w[0];
+      acc[1] += w[1];
+      acc[2] += w[2];
+    }
+  }
+  double dots[2] = {0.0, 0.0};
+  if (has) {
+    const int m = mraw;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T ku = (m >> c) & 1 ? s3[c] * xr[c] : acc[c];
+      op.out((long long)n * 3 + c, s3[c] * ku, dots, pre[c]);
+    }
+  }
+  if constexpr (Op::kDots > 0) {
+    if (op.partials) {
+      __shared__ double sm[kNodeThreads / 64];
+      double vv[Op::kDots];
+#pragma unroll
+      for (int k = 0; k < Op::kDots; ++k) vv[k] = block_sum(dots[k], sm);
+      store_block_partials<Op::kDots>(vv, op.partials);
+    }
+  }
+}
+
+// The modes of mf_apply as an epilogue (mode 0: y = K u; 1: y = K u and the
+// dot u.y; 2: y = f - K u and the dot y.y).
+struct OpMfMode {
+  static constexpr int kDots = 1;
+  const double* u;
+  const double* f;
+  double* y;
+  int mode;
+  double* partials;  // null: no dot
+  struct Pre { double u, f; };
+  __device__ __forceinline__ void load(long long k, Pre& p) const {
+    p.u = u[k];
+    p.f = mode == 2 ? f[k] : 0.0;
+  }
+  __device__ __forceinline__ void out(long long k, double a, double (&d)[2], const Pre& p) const {
+    if (mode == 2) {
+      const double r = p.f - a;
+      y[k] = r;
+      d[0] += r * r;
+    } else {
+      y[k] = a;
+      if (mode == 1) d[0] += p.u * a;
+    }
+  }
+};
+
+// qdp[s][k] = qd[k][pe[s]] (affine quadrature data as one record per slot)
+__global__ void qdata_permute_kernel(int M, const int* __restrict__ pe, const double* __restrict__ qd,
+                                     double* __restrict__ qdp) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < 10LL * M;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long sl = i / 10, k = i % 10;
+    qdp[i] = qd[k * M + pe[sl]];
+  }
+}
+
 // out_k = 0 for a DOF fixed in node_fix (bit k % 3 of node k / 3), else
 // scale_k (1 without a scale): the element kernel's input scaling.
 template <typename T>
@@ -1046,6 +1405,86 @@ std::vector<int> mf_chunks(const std::vector<int>& ptr) {  // first node of each
   return c;
 }
 
+// ---------------------------------------------------------------- patch layout (host set-up)
+static void mf_patch_release(MfOp& op) {
+  for (void* p : {(void*)op.pe, (void*)op.pnode, (void*)op.pconn, (void*)op.poff, (void*)op.pslot, (void*)op.nhead,
+                  (void*)op.novf, (void*)op.qdp, (void*)op.part})
+    if (p) (void)hipFree(p);
+  op.pe = op.pnode = op.novf = nullptr;
+  op.nhead = nullptr;
+  op.pconn = op.poff = op.pslot = nullptr;
+  op.qdp = op.part = nullptr;
+  op.patch = op.n_patch = op.lmax = op.stride = op.n_local = 0;
+  ++op.gen;
+}
+
+// LDS of one patch workgroup: staged nodal values and element forces
+static size_t patch_lds_bytes(const MfOp& op, size_t elem) {
+  return (4 * (size_t)op.stride + 30 * (size_t)op.patch) * elem + 2 * (10 * (size_t)op.patch + (size_t)op.stride);
+}
+constexpr size_t kPatchLdsMax = 64 * 1024;
+constexpr int kDefaultPatch = 64;
+
+int mf_set_patch(MfOp& op, int elems, hipStream_t s) {
+  if (elems < 0 || elems > 256 || elems % 64) return kBadArg;
+  mf_patch_release(op);
+  if (elems == 0) return kOk;
+  if (!op.affine || op.M <= 0 || !op.qd || op.order.size() != (size_t)op.M) return kBadMode;
+  const int M = op.M, N = op.n_nodes, P = elems;
+  std::vector<int> conn((size_t)M * 10);
+  WN_HIP(hipMemcpy(conn.data(), op.conn, sizeof(int) * conn.size(), hipMemcpyDeviceToHost));
+  const std::vector<int>& order = op.order;
+  int lmax = 0;
+  const long long nl = mf_patch_sizes(M, N, conn.data(), order.data(), P, &lmax);
+  if (nl < 0) return kBadArg;
+  op.patch = P;
+  op.n_patch = ceil_div(M, P);
+  op.lmax = lmax;
+  op.stride = (lmax + 1 + 3) / 4 * 4;
+  op.n_local = (int)nl;
+  // the FP64 patch must fit one workgroup's LDS, and step 1 of the patch
+  // kernel stages at most kStageSlots local nodes per lane
+  if (patch_lds_bytes(op, sizeof(double)) > kPatchLdsMax || op.stride > kStageSlots * 4 * P ||
+      12LL * M >= (1LL << 31) || (long long)op.n_patch * op.stride * 3 >= (1LL << 31)) {
+    mf_patch_release(op);
+    return kBadSize;
+  }
+  const size_t cap = (size_t)op.n_patch * op.stride;
+  std::vector<int> pnode(cap), nhead((size_t)N * 4), novf((size_t)nl + 1);
+  std::vector<unsigned short> pconn((size_t)M * 10), pslot((size_t)M * 10), poff(cap);
+  const long long nov = mf_patches(M, N, conn.data(), order.data(), P, op.stride, pnode.data(), pconn.data(),
+                                   poff.data(), pslot.data(), nhead.data(), novf.data());
+  if (nov < 0) {
+    mf_patch_release(op);
+    return kBadArg;
+  }
+  novf.resize((size_t)std::max<long long>(nov, 1));
+  auto up = [&](auto*& dst, const auto& v) -> int {
+    using E = typename std::decay_t<decltype(v)>::value_type;
+    WN_HIP(hipMalloc(&dst, sizeof(E) * std::max<size_t>(v.size(), 1)));
+    WN_HIP(hipMemcpy(dst, v.data(), sizeof(E) * v.size(), hipMemcpyHostToDevice));
+    return kOk;
+  };
+  int r;
+  if ((r = up(op.pe, order)) || (r = up(op.pnode, pnode)) || (r = up(op.pconn, pconn)) || (r = up(op.poff, poff)) ||
+      (r = up(op.pslot, pslot)) || (r = up(op.nhead, nhead)) || (r = up(op.novf, novf))) {
+    mf_patch_release(op);
+    return r;
+  }
+  hipError_t e = hipMalloc(&op.qdp, sizeof(double) * 10 * (size_t)M);
+  if (e == hipSuccess) e = hipMalloc(&op.part, sizeof(double) * 3 * cap);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(qdata_permute_kernel, dim3(vec_grid(10LL * M)), dim3(256), 0, s, M, op.pe, op.qd, op.qdp);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    mf_patch_release(op);
+    return static_cast<int>(e);
+  }
+  return kOk;
+}
+
 int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s) {
   mf_release(op);
   if (op.n_nodes <= 0) return kBadArg;
@@ -1090,10 +1529,25 @@ int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s) {
     op.qd = qa;
     op.affine = 1;
   }
-  return static_cast<int>(hipGetLastError());
+  WN_HIP(hipGetLastError());
+  if (op.affine) {  // the patch layout, when one patch's FP64 workgroup fits the LDS
+    std::vector<int> conn((size_t)op.M * 10);
+    std::vector<double> X((size_t)op.n_nodes * 3);
+    WN_HIP(hipMemcpy(conn.data(), op.conn, sizeof(int) * conn.size(), hipMemcpyDeviceToHost));
+    WN_HIP(hipMemcpy(X.data(), nodes, sizeof(double) * X.size(), hipMemcpyDeviceToHost));
+    for (int v : conn)
+      if (v < 0 || v >= op.n_nodes) return kBadArg;
+    op.order.resize((size_t)op.M);
+    morton_order(op.M, conn.data(), X.data(), op.order.data());
+    r = mf_set_patch(op, kDefaultPatch, s);
+    if (r != kOk && r != kBadSize) return r;
+  }
+  return kOk;
 }
 
 void mf_release(MfOp& op) {
+  mf_patch_release(op);
+  op.order.clear();
   for (void* p : {(void*)op.qd, (void*)op.fe, (void*)op.mask, (void*)op.chunk})
     if (p) (void)hipFree(p);
   op.qd = op.fe = op.mask = nullptr;
@@ -1102,13 +1556,48 @@ void mf_release(MfOp& op) {
   op.n_chunks = 0;
 }
 
-// gather grid: chunk blocks padded to a multiple of the 8 XCDs
-int mf_grid(const MfOp& op) { return 8 * ceil_div(std::max(1, ceil_div(op.n_chunks, kGatherWaves)), 8); }
+// blocks of the node-ordered kernel that completes the patch layout
+static int node_grid(int n_nodes) { return 8 * ceil_div(std::max(1, ceil_div(n_nodes, kNodeThreads)), 8); }
+static int patch_grid(int n_patch) { return 8 * ceil_div(std::max(1, n_patch), 8); }
+
+// the patch kernel for the layout's patch size
+template <typename T, bool Folded>
+static void launch_patch(const MfOp& op, const T* qdp, const T* sc, const T* U, T* part, const int* skip,
+                         hipStream_t s) {
+  const dim3 grid(patch_grid(op.n_patch));
+  const size_t lds = patch_lds_bytes(op, sizeof(T));
+#define WN_PATCH_LAUNCH(PP)                                                                                         \
+  hipLaunchKernelGGL((mf_patch_kernel<T, PP, Folded>), grid, dim3(4 * (PP)), lds, s, op.n_patch, op.M, op.stride,   \
+                     op.pnode, op.pconn, op.poff, op.pslot, op.pe, qdp, op.E, op.nu, sc, U, part, skip)
+  switch (op.patch) {
+    case 64: WN_PATCH_LAUNCH(64); break;
+    case 128: WN_PATCH_LAUNCH(128); break;
+    case 192: WN_PATCH_LAUNCH(192); break;
+    default: WN_PATCH_LAUNCH(256); break;
+  }
+#undef WN_PATCH_LAUNCH
+}
+
+// blocks of the per-element path's node gather, padded to a multiple of the 8 XCDs
+static int gather_grid(const MfOp& op) { return 8 * ceil_div(std::max(1, ceil_div(op.n_chunks, kGatherWaves)), 8); }
+
+// blocks of the final kernel of mf_apply (the node kernel of the patch
+// layout, or the gather)
+int mf_grid(const MfOp& op) { return op.patch ? node_grid(op.n_nodes) : gather_grid(op); }
 static int elem_grid(int M) { return 8 * ceil_div(ceil_div(4L * M, 256), 8); }
 
 int mf_apply(const MfOp& op, const double* u, const double* f, double* y, double* partials, int mode,
              const int* flags, int guard, hipStream_t s) {
   if (!op.chunk) return kNotReady;
+  if (op.patch) {
+    if (!op.mask) return kNotReady;
+    const int* skip = guard ? flags : nullptr;
+    launch_patch<double, false>(op, op.qdp, op.mask, u, op.part, skip, s);
+    const OpMfMode o{u, f, y, mode, mode ? partials : nullptr};
+    hipLaunchKernelGGL((mf_node_kernel<double, OpMfMode>), dim3(node_grid(op.n_nodes)), dim3(kNodeThreads), 0, s,
+                       op.n_nodes, op.nhead, op.novf, op.part, op.fix, nullptr, u, o, skip);
+    return static_cast<int>(hipGetLastError());
+  }
   if (op.M > 0) {
     if (!op.qd || !op.fe || !op.mask) return kNotReady;
     if (op.affine)
@@ -1118,7 +1607,7 @@ int mf_apply(const MfOp& op, const double* u, const double* f, double* y, double
       hipLaunchKernelGGL((c3d10_pa_elem_kernel<double, false>), dim3(elem_grid(op.M)), dim3(256), 0, s, op.M,
                          op.conn, op.qd, op.E, op.nu, op.mask, u, op.fe, flags, guard);
   }
-  hipLaunchKernelGGL(mf_gather_kernel, dim3(mf_grid(op)), dim3(64 * kGatherWaves), 0, s, op.n_chunks, op.M, op.chunk,
+  hipLaunchKernelGGL(mf_gather_kernel, dim3(gather_grid(op)), dim3(64 * kGatherWaves), 0, s, op.n_chunks, op.M, op.chunk,
                      op.n2e_ptr, op.n2e, op.fe, op.fix, u, f, y, partials, mode, flags, guard);
   return static_cast<int>(hipGetLastError());
 }
@@ -1150,7 +1639,8 @@ int launch_adjoint_dot(int M, const int* conn, const double* E, const double* fe
 
 // ---------------------------------------------------------------- V-cycle smoothing on the matrix-free operator
 void mf_smooth_release(MfSmooth& m) {
-  for (void* p : {(void*)m.s64e, (void*)m.s32, (void*)m.s32e, (void*)m.qd32, (void*)m.fe32})
+  for (void* p : {(void*)m.s64e, (void*)m.s32, (void*)m.s32e, (void*)m.qd32, (void*)m.fe32, (void*)m.qdp32,
+                  (void*)m.part32})
     if (p) (void)hipFree(p);
   m = MfSmooth{};
 }
@@ -1173,7 +1663,14 @@ int mf_smooth_attach(MfSmooth& m, const MfOp* op, const double* scale, int want3
     hipLaunchKernelGGL(dof_scale_kernel<float>, dim3(vec_grid(n)), dim3(256), 0, s, n, scale, op->fix, m.s32e);
     if (op->M > 0)
       hipLaunchKernelGGL(round_to_float_kernel, dim3(vec_grid(nq)), dim3(256), 0, s, nq, op->qd, m.qd32);
+    if (op->patch) {
+      WN_HIP(hipMalloc(&m.qdp32, sizeof(float) * 12 * (size_t)op->M));
+      WN_HIP(hipMalloc(&m.part32, sizeof(float) * 3 * (size_t)op->n_patch * op->stride));
+      hipLaunchKernelGGL(qdata_fold_kernel, dim3(vec_grid(op->M)), dim3(256), 0, s, op->M, op->pe, op->qdp, op->E,
+                         op->nu, m.qdp32);
+    }
   }
+  m.gen = op->gen;
   WN_HIP(hipGetLastError());
   m.op = op;
   m.s64 = scale;
@@ -1197,18 +1694,36 @@ template <typename T> static T* mf_fe(const MfSmooth& m);
 template <> double* mf_fe<double>(const MfSmooth& m) { return m.op->fe; }
 template <> float* mf_fe<float>(const MfSmooth& m) { return m.fe32; }
 
+// the patch layout's FP64 buffers are the operator's; FP32 uses the
+// attachment's copies, made for the operator's layout at attach time
+template <typename T> static const T* mf_qdp(const MfSmooth& m);
+template <> const double* mf_qdp<double>(const MfSmooth& m) { return m.op->patch ? m.op->qdp : nullptr; }
+template <> const float* mf_qdp<float>(const MfSmooth& m) {
+  return m.op->patch && m.gen == m.op->gen ? m.qdp32 : nullptr;
+}
+template <typename T> static T* mf_part(const MfSmooth& m);
+template <> double* mf_part<double>(const MfSmooth& m) { return m.op->part; }
+template <> float* mf_part<float>(const MfSmooth& m) { return m.part32; }
+
 template <typename T, typename Op>
 static int mf_smooth_run(const MfSmooth& m, const T* x, const Op& o, const int* skip, hipStream_t s) {
   if (!mf_smooth_ready<T>(m)) return kNotReady;
   const MfOp& op = *m.op;
   const T* sc = mf_scale<T>(m);
+  if (const T* qdp = mf_qdp<T>(m)) {
+    T* part = mf_part<T>(m);
+    launch_patch<T, std::is_same<T, float>::value>(op, qdp, mf_scale_in<T>(m), x, part, skip, s);
+    hipLaunchKernelGGL((mf_node_kernel<T, Op>), dim3(node_grid(op.n_nodes)), dim3(kNodeThreads), 0, s, op.n_nodes,
+                       op.nhead, op.novf, part, op.fix, sc, x, o, skip);
+    return static_cast<int>(hipGetLastError());
+  }
   if (op.M > 0 && op.affine)
     hipLaunchKernelGGL((c3d10_pa_elem_kernel<T, true>), dim3(elem_grid(op.M)), dim3(256), 0, s, op.M,
                        op.conn, mf_qd<T>(m), op.E, op.nu, mf_scale_in<T>(m), x, mf_fe<T>(m), skip, skip ? 1 : 0);
   else if (op.M > 0)
     hipLaunchKernelGGL((c3d10_pa_elem_kernel<T, false>), dim3(elem_grid(op.M)), dim3(256), 0, s, op.M,
                        op.conn, mf_qd<T>(m), op.E, op.nu, mf_scale_in<T>(m), x, mf_fe<T>(m), skip, skip ? 1 : 0);
-  hipLaunchKernelGGL((mf_gather_op_kernel<T, Op>), dim3(mf_grid(op)), dim3(64 * kGatherWaves), 0, s, op.n_chunks,
+  hipLaunchKernelGGL((mf_gather_op_kernel<T, Op>), dim3(gather_grid(op)), dim3(64 * kGatherWaves), 0, s, op.n_chunks,
                      op.M, op.chunk, op.n2e_ptr, op.n2e, mf_fe<T>(m), op.fix, sc, x, o, skip);
   return static_cast<int>(hipGetLastError());
 }

@@ -9,13 +9,17 @@
 //     Galerkin operator and the block SpMV of its eigenvalue estimate, on
 //     host threads (one contiguous range of block rows each; every output
 //     entry is summed in a fixed order, so results do not depend on the
-//     thread count).
+//     thread count);
+//   * the patch layout of the matrix-free C3D10 operator (elements in Morton
+//     order, cut into patches; local connectivity, per-node slot lists and
+//     each node's partial sums).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
+#include <utility>
 #include <vector>
 
 namespace wn {
@@ -507,6 +511,169 @@ void bsr_spmv_host(int n, int b, const int* ptr, const int* idx, const double* v
       for (int r = 0; r < b; ++r) y[(size_t)i * b + r] = acc[r];
     }
   });
+}
+
+// ---------------------------------------------------------------- matrix-free patch layout
+// Morton order of C3D10 elements: the centroid of each element's four
+// corners, quantised to 21 bits per axis on the mesh's bounding cube, bits
+// interleaved (x lowest); ties keep element order.
+static unsigned long long spread3(unsigned long long v) {  // bit i -> bit 3 i (21 bits)
+  v &= 0x1fffffULL;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+void morton_order(int M, const int* conn, const double* X, int* order) {
+  std::vector<double> c((size_t)M * 3);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int e = 0; e < M; ++e)
+    for (int a = 0; a < 3; ++a) {
+      double v = 0.0;
+      for (int i = 0; i < 4; ++i) v += X[(size_t)conn[(size_t)e * 10 + i] * 3 + a];
+      v *= 0.25;
+      c[(size_t)e * 3 + a] = v;
+      lo[a] = std::min(lo[a], v);
+      hi[a] = std::max(hi[a], v);
+    }
+  double ext = 0.0;
+  for (int a = 0; a < 3; ++a) ext = std::max(ext, hi[a] - lo[a]);
+  if (!(ext > 0.0)) ext = 1.0;
+  const double top = double((1 << 21) - 1), q = top / ext;
+  std::vector<std::pair<unsigned long long, int>> key((size_t)M);
+  for (int e = 0; e < M; ++e) {
+    unsigned long long k = 0;
+    for (int a = 0; a < 3; ++a) {
+      const double t = std::min(std::max((c[(size_t)e * 3 + a] - lo[a]) * q, 0.0), top);
+      k |= spread3((unsigned long long)t) << a;
+    }
+    key[(size_t)e] = {k, e};
+  }
+  std::sort(key.begin(), key.end());
+  for (int e = 0; e < M; ++e) order[e] = key[(size_t)e].second;
+}
+
+// Patch layout: slot s holds element order[s]; patch p the slots
+// [p P, min(M, (p + 1) P)).  A patch's local nodes are the distinct global
+// nodes of its elements, ascending.  mf_patch_sizes returns the number of
+// local nodes of all patches and the most of one patch (lmax).
+// mf_patches fills, for a per-patch stride >= lmax + 1:
+//   pnode[p stride + j]  global node of local node j (-1 past the last one);
+//   pconn[10 s + i]      local node of corner i of slot s;
+//   pslot[10 p P ...]    the local nodes' slot codes 10 (s - p P) + i, each
+//                        node's list ascending, local node j's from
+//                        poff[p stride + j] to poff[p stride + j + 1] (the
+//                        entries past the last node hold 10 ne);
+//   nhead[4 n]           node n's partial slots p stride + j in patch order:
+//                        the first three (-1 where absent) and their count,
+//                        or, with more than three, -(k + 1) where
+//                        novf[k] = count - 3 and novf[k + 1 ...] the rest.
+// Returns the length of novf, or -1 on a corner outside [0, N), a repeated
+// element, P outside [1, 6553] or stride <= lmax.
+static bool mf_patch_check(int M, int N, const int* order, int P) {
+  if (M < 0 || N < 0 || P < 1 || P > 6553) return false;
+  std::vector<char> seen((size_t)M, 0);
+  for (int sl = 0; sl < M; ++sl) {
+    const int e = order[sl];
+    if (e < 0 || e >= M || seen[(size_t)e]) return false;
+    seen[(size_t)e] = 1;
+  }
+  return true;
+}
+
+// local nodes of patch p (ascending) into ln; false on a corner outside [0, N)
+static bool patch_nodes(int M, int N, const int* conn, const int* order, int P, int p, std::vector<int>& ln) {
+  const int s0 = p * P, s1 = std::min(M, s0 + P);
+  ln.clear();
+  for (int sl = s0; sl < s1; ++sl)
+    for (int i = 0; i < 10; ++i) {
+      const int g = conn[(size_t)order[sl] * 10 + i];
+      if (g < 0 || g >= N) return false;
+      ln.push_back(g);
+    }
+  std::sort(ln.begin(), ln.end());
+  ln.erase(std::unique(ln.begin(), ln.end()), ln.end());
+  return true;
+}
+
+long long mf_patch_sizes(int M, int N, const int* conn, const int* order, int P, int* lmax) {
+  if (!mf_patch_check(M, N, order, P)) return -1;
+  const int n_patch = (M + P - 1) / P;
+  std::vector<int> ln;
+  long long nl = 0;
+  int lm = 0;
+  for (int p = 0; p < n_patch; ++p) {
+    if (!patch_nodes(M, N, conn, order, P, p, ln)) return -1;
+    nl += (long long)ln.size();
+    lm = std::max(lm, (int)ln.size());
+  }
+  *lmax = lm;
+  return nl;
+}
+
+long long mf_patches(int M, int N, const int* conn, const int* order, int P, int stride, int* pnode,
+                     unsigned short* pconn, unsigned short* poff, unsigned short* pslot, int* nhead, int* novf) {
+  if (!mf_patch_check(M, N, order, P) || stride < 2) return -1;
+  const int n_patch = (M + P - 1) / P;
+  std::vector<int> loc((size_t)N, -1), ln, cnt;
+  std::vector<std::vector<int>> parts;  // per node, only while building
+  std::vector<int> count((size_t)N, 0);
+  for (int p = 0; p < n_patch; ++p) {
+    const int s0 = p * P, s1 = std::min(M, s0 + P), ne = s1 - s0;
+    if (!patch_nodes(M, N, conn, order, P, p, ln)) return -1;
+    const int L = (int)ln.size();
+    if (L >= stride) return -1;
+    int* pn = pnode + (size_t)p * stride;
+    unsigned short* po = poff + (size_t)p * stride;
+    for (int j = 0; j < L; ++j) {
+      loc[(size_t)ln[(size_t)j]] = j;
+      pn[j] = ln[(size_t)j];
+    }
+    for (int j = L; j < stride; ++j) {
+      pn[j] = -1;
+      po[j] = (unsigned short)(10 * ne);
+    }
+    for (int sl = s0; sl < s1; ++sl)
+      for (int i = 0; i < 10; ++i)
+        pconn[(size_t)sl * 10 + i] = (unsigned short)loc[(size_t)conn[(size_t)order[sl] * 10 + i]];
+    // counting sort of the slot codes by local node: each list ascending
+    cnt.assign((size_t)L + 1, 0);
+    const unsigned short* pc = pconn + (size_t)s0 * 10;
+    for (int q = 0; q < 10 * ne; ++q) ++cnt[(size_t)pc[q] + 1];
+    for (int j = 0; j < L; ++j) cnt[(size_t)j + 1] += cnt[(size_t)j];
+    for (int j = 0; j < L; ++j) po[j] = (unsigned short)cnt[(size_t)j];
+    for (int q = 0; q < 10 * ne; ++q) pslot[(size_t)s0 * 10 + cnt[(size_t)pc[q]]++] = (unsigned short)q;
+    for (int j = 0; j < L; ++j) {
+      const int g = ln[(size_t)j];
+      loc[(size_t)g] = -1;
+      const int slot = p * stride + j;
+      int* h = nhead + (size_t)g * 4;
+      if (count[(size_t)g] < 3) {
+        h[count[(size_t)g]] = slot;
+      } else {
+        if (parts.empty()) parts.resize((size_t)N);
+        parts[(size_t)g].push_back(slot);
+      }
+      ++count[(size_t)g];
+    }
+  }
+  long long nov = 0;
+  for (int n = 0; n < N; ++n) {
+    int* h = nhead + (size_t)n * 4;
+    const int c = count[(size_t)n];
+    for (int k = c; k < 3; ++k) h[k] = -1;
+    if (c <= 3) {
+      h[3] = c;
+    } else {
+      h[3] = -(int)(nov + 1);
+      novf[nov++] = c - 3;
+      for (int v : parts[(size_t)n]) novf[nov++] = v;
+    }
+  }
+  return nov;
 }
 
 }  // namespace wn

@@ -406,6 +406,17 @@ static void key_tune(std::vector<uint64_t>& k, const SpmvTune& t) {
   key_push(k, t.flags);
 }
 
+// every pointer and size the matrix-free kernels take (both layouts)
+static void key_mf(std::vector<uint64_t>& k, const MfOp& m) {
+  for (const void* p : {(const void*)m.qd, (const void*)m.fe, (const void*)m.mask, (const void*)m.conn,
+                        (const void*)m.fix, (const void*)m.E, (const void*)m.nu, (const void*)m.n2e,
+                        (const void*)m.n2e_ptr, (const void*)m.chunk, (const void*)m.pe, (const void*)m.pnode,
+                        (const void*)m.pconn, (const void*)m.poff, (const void*)m.pslot, (const void*)m.nhead,
+                        (const void*)m.novf, (const void*)m.qdp, (const void*)m.part})
+    key_push(k, p);
+  for (int v : {m.M, m.n_nodes, m.n_chunks, m.affine, m.patch, m.n_patch, m.stride, m.n_local, m.gen}) key_push(k, v);
+}
+
 static void key_amg(std::vector<uint64_t>& k, const Amg* g) {
   key_push(k, g);
   if (!g) return;
@@ -413,8 +424,11 @@ static void key_amg(std::vector<uint64_t>& k, const Amg* g) {
   for (const void* p : {(const void*)g->inv, (const void*)g->inv32, (const void*)g->tiles64, (const void*)g->tiles32,
                         (const void*)g->tij, (const void*)g->tpart, (const void*)g->mfs.op,
                         (const void*)g->mfs.s64, (const void*)g->mfs.s64e, (const void*)g->mfs.s32,
-                        (const void*)g->mfs.s32e, (const void*)g->mfs.qd32, (const void*)g->mfs.fe32})
+                        (const void*)g->mfs.s32e, (const void*)g->mfs.qd32, (const void*)g->mfs.fe32,
+                        (const void*)g->mfs.qdp32, (const void*)g->mfs.part32})
     key_push(k, p);
+  key_push(k, g->mfs.gen);
+  if (g->mfs.op) key_mf(k, *g->mfs.op);
   key_tune(k, g->tune);
   for (const auto& lp : g->L) {
     const AmgLevel& l = *lp;
@@ -545,12 +559,7 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
       key_push(key, p);
     key_push(key, n);
     if (mf) {
-      // every pointer and size the matrix-free kernels take
-      for (const void* p : {(const void*)mf->qd, (const void*)mf->fe, (const void*)mf->mask, (const void*)mf->conn,
-                            (const void*)mf->fix, (const void*)mf->E, (const void*)mf->nu, (const void*)mf->n2e,
-                            (const void*)mf->n2e_ptr, (const void*)mf->chunk})
-        key_push(key, p);
-      for (int v : {mf->M, mf->n_nodes, mf->n_chunks, mf->affine}) key_push(key, v);
+      key_mf(key, *mf);
     } else {
       key_matrix(key, A);
     }

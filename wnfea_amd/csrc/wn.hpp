@@ -98,6 +98,28 @@ struct MfOp {
   double* mask = nullptr;  // [3 n_nodes] 0 for a fixed DOF, else 1
   int2* chunk = nullptr;  // [n_chunks+1] {first node, first incidence} of each gather chunk
   int n_chunks = 0;
+  // Patch layout (affine meshes): the elements sorted along a Morton curve of
+  // their centroids and cut into patches of `patch` consecutive slots.  One
+  // workgroup per patch stages the patch's nodal values in LDS, keeps its
+  // element forces in LDS and writes one partial sum per (patch, local node)
+  // to slot p stride + j of the partial array; a node-ordered kernel then
+  // adds each node's partials in patch order.  Per-patch arrays have a fixed
+  // stride (> the most local nodes of one patch), so a patch's data is found
+  // without a header.  patch == 0: no patch layout (per-element
+  // contributions in `fe`, gathered by `chunk`).
+  int patch = 0, n_patch = 0, lmax = 0, stride = 0;  // elements per patch, patches, most local nodes, stride
+  int n_local = 0;                       // local nodes of all patches (= partial sums)
+  int gen = 0;                           // layout generation: smoothers built on another one fall back
+  int* pe = nullptr;                     // [M] element of each slot
+  int* pnode = nullptr;                  // [n_patch][stride] global node of each local node (-1: none)
+  unsigned short* pconn = nullptr;       // [M][10] local node of each element corner
+  unsigned short* poff = nullptr;        // [n_patch][stride] start of the local node's slot list (patch-relative)
+  unsigned short* pslot = nullptr;       // [M][10] per patch: the local nodes' slot lists (slot = 10 e + corner)
+  int4* nhead = nullptr;                 // [n_nodes] first three partial slots and the count (or overflow)
+  int* novf = nullptr;                   // overflow lists of nodes with more than three partials
+  double* qdp = nullptr;                 // [M][10] affine quadrature data, one record per slot
+  double* part = nullptr;                // [n_patch * stride][3]
+  std::vector<int> order;                // host: elements in Morton order of their centroids
 };
 // The finest level's smoothing and defect products of a V-cycle run
 // matrix-free on the equilibrated operator: y = s .* K (s .* x), with K the
@@ -112,6 +134,9 @@ struct MfSmooth {
   float* s32e = nullptr;
   float* qd32 = nullptr;  // rounded copy of the operator's qd (same layout)
   float* fe32 = nullptr;  // [10][M][3]
+  float* qdp32 = nullptr;   // [M][12] patch layout: rounded inverse Jacobians and w c11, w c12, w c44
+  float* part32 = nullptr;  // [n_patch * stride][3]
+  int gen = -1;             // the operator's layout generation the copies were made for
 };
 int mf_smooth_attach(MfSmooth& m, const MfOp* op, const double* scale, int want32, hipStream_t s);
 void mf_smooth_release(MfSmooth& m);
@@ -129,6 +154,15 @@ std::vector<int> mf_chunks(const std::vector<int>& n2e_ptr);
 int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s);
 void mf_release(MfOp& op);
 int mf_grid(const MfOp& op);  // blocks of the gather (= partial count of modes 1 and 2)
+// Build (elems > 0: elements per patch, a multiple of 64, at most 256) or drop
+// (elems == 0) the patch layout; smoothers attached before fall back to the
+// per-element path until attached again.
+int mf_set_patch(MfOp& op, int elems, hipStream_t s);
+// host set-up of the patch layout (setup.cpp)
+void morton_order(int M, const int* conn, const double* X, int* order);
+long long mf_patch_sizes(int M, int N, const int* conn, const int* order, int P, int* lmax);
+long long mf_patches(int M, int N, const int* conn, const int* order, int P, int stride, int* pnode,
+                     unsigned short* pconn, unsigned short* poff, unsigned short* pslot, int* nhead, int* novf);
 int mf_apply(const MfOp& op, const double* u, const double* f, double* y, double* partials, int mode,
              const int* flags, int guard, hipStream_t s);
 int launch_fint_elem(int M, const int* conn, const double* nodes, const double* E, const double* nu, const double* U,

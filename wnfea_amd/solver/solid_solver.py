@@ -408,6 +408,28 @@ class MatrixFreeOperator:
         amg.attach_matrix_free(self)
         return amg
 
+    def set_patch(self, elems: int):
+        """Rebuild the patch layout of the products with ``elems`` elements
+        per patch (a multiple of 64, at most 256), or drop it (0: per-element
+        contributions and a node gather).  Affine meshes only; the layout is
+        built with 64 at construction when it fits.  Hierarchies smoothing
+        on this operator are attached again (their FP32 copies follow the
+        layout)."""
+        _lib.check(self.lib.wnfea_mf_set_patch(self.handle, int(elems), _lib.stream_of()), "wnfea_mf_set_patch")
+        for amg in list(self._attached):
+            if getattr(amg, "handle", None):
+                amg.attach_matrix_free(self, getattr(amg, "_mf_scale", None))
+
+    @property
+    def patch_info(self) -> dict:
+        """Elements per patch (0: no patch layout), patches, the most local
+        nodes of one patch, the local nodes (= partial sums) of all and the
+        per-patch stride of the local-node arrays."""
+        out = (_lib.ctypes.c_longlong * 6)()
+        _lib.check(self.lib.wnfea_mf_info(self.handle, out), "wnfea_mf_info")
+        return {"elems": int(out[0]), "patches": int(out[1]), "lmax": int(out[2]), "local_nodes": int(out[3]),
+                "stride": int(out[5])}
+
     def apply(self, u, y=None):
         import torch
         y = torch.empty_like(u) if y is None else y
