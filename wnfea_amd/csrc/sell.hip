@@ -62,7 +62,10 @@ SellMatrix& SellMatrix::equilibrated() {
 // is four 3x3 blocks' worth of loads, so those rows split further, to about
 // two blocks per lane, up to 8 lanes per row (the 6-DOF beam lattice's
 // V-cycle: 177 instead of 189 us; beyond 8 lanes the 8-column SpMM of the
-// multi-RHS cycle loses more to its wider shuffle reduction than it gains).
+// multi-RHS cycle loses more to its wider shuffle reduction than it gains),
+// and 3x6 / 6x3 transfer blocks (two 3x3 blocks' worth) to about three per
+// lane (the I-beam's prolongation, about six blocks per row: 36.4 -> 34.1 us
+// with two lanes per row).
 static int auto_lanes(int n_rows, long long nnzb, int bb) {
   const double avg = n_rows > 0 ? double(nnzb) / n_rows : 0.0;
   int L = 1;
@@ -70,6 +73,8 @@ static int auto_lanes(int n_rows, long long nnzb, int bb) {
   while (L < 64 && 2 * L * 6 <= avg) L *= 2;
   if (bb >= 36)
     while (L < 8 && 2 * L * 2 <= avg) L *= 2;
+  else if (bb >= 18)
+    while (L < 8 && 2 * L * 3 <= avg) L *= 2;
   return L;
 }
 
