@@ -348,7 +348,8 @@ def mf_patch_bytes(n_elements: int, n_nodes: int, n_local: int, n_patch: int, st
     per_elem = 20 + 20 + (12 * s if folded else 10 * s + 20)
     patch = n_elements * per_elem + n_patch * stride * 6 + n_local * 3 * s + n_nodes * 6 * s
     node = n_nodes * (16 + 1) + n_local * 3 * s
-    return {"patch": patch, "node_cheb": node + n_nodes * 21 * s, "node_apply": node + n_nodes * 6 * s}
+    return {"patch": patch, "node_cheb": node + n_nodes * 21 * s, "node_defect": node + n_nodes * 12 * s,
+            "node_apply": node + n_nodes * 6 * s}
 
 
 def mf_only(system, model, reps: int, log, patch: int = -1) -> int:
@@ -789,6 +790,9 @@ def main() -> int:
                          "for rocprofv3 --pmc")
     ap.add_argument("--no-ir", action="store_true", help="skip the restart-IR (FP32 Krylov) scheme")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cycle-shape", default=None,
+                    help="fix the V-cycle's launch shapes (waves,unroll,xcd_remap for both, or 'fp32:fp64') "
+                         "instead of timing them on whole cycles")
     ap.add_argument("--mf-patch", type=int, default=-1,
                     help="elements per patch of the matrix-free operator's patch layout (0: per-element "
                          "contributions and node gather; default: the library's layout)")
@@ -879,9 +883,15 @@ def main() -> int:
         # their own launch shapes (timed on whole cycles)
         from wnfea_amd.solver.autotune import autotune_cycle
         cycle_tune = {}
-        for p in ("fp32", "fp64"):
-            c = autotune_cycle(amg, p)
-            cycle_tune[p] = {k: c[k] for k in ("waves", "unroll", "xcd_remap", "seconds")}
+        parts = args.cycle_shape.split(":") if args.cycle_shape else None
+        for p, text in zip(("fp32", "fp64"), (parts[0], parts[-1]) if parts else (None, None)):
+            if text:  # fixed, e.g. so that a profiled run launches the default run's kernels
+                w, u, x = (int(v) for v in text.split(","))
+                amg.set_tune(p, w, u, bool(x))
+                cycle_tune[p] = {"waves": w, "unroll": u, "xcd_remap": bool(x), "fixed": True}
+            else:
+                c = autotune_cycle(amg, p)
+                cycle_tune[p] = {k: c[k] for k in ("waves", "unroll", "xcd_remap", "seconds")}
         log(f"[bench] cycle launch shapes {cycle_tune}")
 
     if args.cheb_only:
