@@ -1067,15 +1067,19 @@ def main() -> int:
                  "note": "FP64 CG keeps its product and vector work in FP64 in both solvers; only the V-cycle "
                          "changes precision"}
     log(f"[bench] breakdown {breakdown}")
-    # the dominant kernel of the timed solve: the FP32 6x6 Chebyshev step of AMG level 1
-    lv1 = st["levels"][1]
-    b1 = lv1["block"]
-    nnzb1 = lv1["nnz"] // (b1 * b1)
-    t_ch32 = time_level_cheb(amg, 1, "fp32", args.spmv_reps)
-    t_ch64 = time_level_cheb(amg, 1, "fp64", args.spmv_reps)
-    ch32, ch64 = cheb_bytes(nnzb1, lv1["n"], b1, 4), cheb_bytes(nnzb1, lv1["n"], b1, 8)
-    log(f"[bench] level-1 Chebyshev step fp32 {t_ch32 * 1e6:.1f} us {ch32 / t_ch32 / 1e9:.0f} GB/s | "
-        f"fp64 {t_ch64 * 1e6:.1f} us {ch64 / t_ch64 / 1e9:.0f} GB/s")
+    # the dominant kernel of the timed solve: the FP32 6x6 Chebyshev step of AMG
+    # level 1 (a hierarchy with a smoothed level between the finest and the
+    # dense coarsest one; otherwise the fine K's FP32 SpMV below)
+    has_l1 = len(st["levels"]) >= 3
+    if has_l1:
+        lv1 = st["levels"][1]
+        b1 = lv1["block"]
+        nnzb1 = lv1["nnz"] // (b1 * b1)
+        t_ch32 = time_level_cheb(amg, 1, "fp32", args.spmv_reps)
+        t_ch64 = time_level_cheb(amg, 1, "fp64", args.spmv_reps)
+        ch32, ch64 = cheb_bytes(nnzb1, lv1["n"], b1, 4), cheb_bytes(nnzb1, lv1["n"], b1, 8)
+        log(f"[bench] level-1 Chebyshev step fp32 {t_ch32 * 1e6:.1f} us {ch32 / t_ch32 / 1e9:.0f} GB/s | "
+            f"fp64 {t_ch64 * 1e6:.1f} us {ch64 / t_ch64 / 1e9:.0f} GB/s")
     # SpMV bandwidth of the FP32 and FP64 3x3 kernel (the metric's SpMV figure)
     nnzb, nb = info["nnzb"], system.n_nodes
     t32 = time_spmv(system.K, "fp32", args.spmv_reps)
@@ -1157,18 +1161,25 @@ def main() -> int:
         "displacement_rel_err_mixed_vs_fp64": err_u,
         "spmv_gbs": {"fp32": round(gbs32, 1), "fp64": round(gbs64, 1), "fp32_us": round(t32 * 1e6, 2),
                      "fp64_us": round(t64 * 1e6, 2), "bytes_fp32": b32, "bytes_fp64": b64},
-        "roofline": {"bound": "hbm", "achieved": round(ch32 / t_ch32 / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ch32 / t_ch32 / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(CHEB_TRAFFIC_FILE, "fp32"),
-                     "traffic_source": "committed rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE summary "
-                                       "(profiles/amg_level1_cheb_ibeam_1.05M_hbm_traffic.md), not measured in this run",
-                     "algorithmic_bytes": ch32, "avg_launch_us": round(t_ch32 * 1e6, 2),
-                     "launches_per_iteration": 3,
-                     "kernel": f"sell_spmv_kernel<6,6,float,float,{cycle_tune['fp32']['waves'] if cycle_tune else tune['waves']},"
-                               f"{cycle_tune['fp32']['unroll'] if cycle_tune else tune['unroll']},OpCheb<6,float>> "
-                               f"(FP32 Chebyshev smoothing step of AMG level 1: {lv1['n']} rows, {nnzb1} 6x6 blocks)",
-                     "fp64_twin": {"avg_launch_us": round(t_ch64 * 1e6, 2), "algorithmic_bytes": ch64,
-                                   "frac": round(ch64 / t_ch64 / 1e9 / HBM_PEAK_GBS, 4)}},
+        "roofline": ({"bound": "hbm", "achieved": round(ch32 / t_ch32 / 1e9, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(ch32 / t_ch32 / 1e9 / HBM_PEAK_GBS, 4),
+                      "traffic": measured_traffic(CHEB_TRAFFIC_FILE, "fp32") if workload.startswith("c3d10_ibeam")
+                      else None,
+                      "traffic_source": "committed rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE summary "
+                                        "(profiles/amg_level1_cheb_ibeam_1.05M_hbm_traffic.md), not measured in this run",
+                      "algorithmic_bytes": ch32, "avg_launch_us": round(t_ch32 * 1e6, 2),
+                      "launches_per_iteration": 3,
+                      "kernel": f"sell_spmv_kernel<6,6,float,float,"
+                                f"{cycle_tune['fp32']['waves'] if cycle_tune else tune['waves']},"
+                                f"{cycle_tune['fp32']['unroll'] if cycle_tune else tune['unroll']},OpCheb<6,float>> "
+                                f"(FP32 Chebyshev smoothing step of AMG level 1: {lv1['n']} rows, {nnzb1} 6x6 blocks)",
+                      "fp64_twin": {"avg_launch_us": round(t_ch64 * 1e6, 2), "algorithmic_bytes": ch64,
+                                    "frac": round(ch64 / t_ch64 / 1e9 / HBM_PEAK_GBS, 4)}} if has_l1 else
+                     {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": b32,
+                      "avg_launch_us": round(t32 * 1e6, 2),
+                      "kernel": "FP32 3x3 SpMV of the fine K (the hierarchy has no smoothed level between the "
+                                "finest and the dense coarsest one)"}),
         "spmv_tune": tunes,
         "cycle_tune": cycle_tune,
         "matrix_free_operator": mf_info,

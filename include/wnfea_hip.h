@@ -250,8 +250,10 @@ int wnfea_host_bsr_spmv(int n, int b, const int *ptr, const int *idx, const doub
                         int threads);
 
 /* Host set-up of the matrix-free patch layout (host pointers; what
- * wnfea_mf_set_patch builds and uploads).  order[M] = the elements sorted by
- * the Morton code of their corner centroids (21 bits per axis on the mesh's
+ * wnfea_mf_set_patch builds and uploads).  order[M] = the elements grouped
+ * by connected component (elements sharing a corner; components in the
+ * order of their first element) and inside a component sorted by the Morton
+ * code of their corner centroids (21 bits per axis on the component's
  * bounding cube; ties in element order).  No reference counterpart (the
  * reference's matrix-free kernel walks elements in mesh order,
  * wnfea/solver/native/hip_kernels.cpp:283-425). */

@@ -1,6 +1,6 @@
 """Patch layout of the matrix-free C3D10 operator, on the host (no device
 call): the Morton order is a permutation sorted by the interleaved centroid
-code; every patch's local nodes, 16-bit local connectivity, slot lists and
+code, body by body; every patch's local nodes, 16-bit local connectivity, slot lists and
 each node's partial list are consistent with the mesh; and the two kernels'
 arithmetic, emulated in numpy with the oracle's element matrices (element
 forces summed per local node in slot order, then per node in patch order,
@@ -94,6 +94,25 @@ def test_morton_order_is_the_sorted_interleaved_centroid_code():
             code |= ((q[:, a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + a)
     ref = np.lexsort((np.arange(M), code))
     assert np.array_equal(order, ref)
+
+
+def test_morton_order_keeps_separate_bodies_apart():
+    """Two copies of one box in the same place (a batch of meshes in one
+    system): the order takes every element of the first copy, then every
+    element of the second, each copy in its own Morton order, so no patch
+    mixes the two bodies."""
+    L = _lib()
+    m = _box(6)
+    conn = np.ascontiguousarray(m.solid_elements, dtype=np.int32)
+    X = np.asarray(m.mesh_nodes, dtype=np.float64)[:, :3]
+    M, N = len(conn), len(X)
+    one = np.empty(M, dtype=np.int32)
+    assert L.wnfea_host_morton_order(M, _ptr(conn), _ptr(np.ascontiguousarray(X)), _ptr(one)) == 0
+    conn2 = np.ascontiguousarray(np.vstack([conn, conn + N]), dtype=np.int32)
+    X2 = np.ascontiguousarray(np.vstack([X, X]))
+    two = np.empty(2 * M, dtype=np.int32)
+    assert L.wnfea_host_morton_order(2 * M, _ptr(conn2), _ptr(X2), _ptr(two)) == 0
+    assert np.array_equal(two[:M], one) and np.array_equal(two[M:], one + M)
 
 
 @pytest.mark.parametrize("mesh,elems", [("box6", 64), ("box5", 128), ("ibeam", 128), ("ibeam", 256)])

@@ -79,15 +79,40 @@ __global__ __launch_bounds__(kVecBlock) void rho_kernel(long long n, const T* __
   }
 }
 
+// Sum of nb partials, the same bits in every block of a launch (same
+// threads, same order): a consumer kernel finalises a reduction itself
+// instead of a one-block launch before it.
+__device__ __forceinline__ double block_partial_sum(const double* __restrict__ part, int nb, double* sm) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) a += part[i];
+  return block_sum(a, sm);
+}
+
 // p = S z + beta p  (p = S z on the first iteration).  first < 0: the first
 // iteration is the one where the loop's iteration counter skip[1] is still 0,
-// so a captured iteration can be replayed unchanged.
+// so a captured iteration can be replayed unchanged.  With rho_part every
+// block sums the nb_rho partials of rho = r . z itself and takes
+// beta = rho / rho_prev; block 0 stores rho and beta (rho_prev is left for
+// the update of x and r, after every block here has read it).
 template <typename T, typename ZT>
-__global__ void update_p_kernel(long long n, const ZT* __restrict__ z, const double* __restrict__ scale,
-                                T* __restrict__ p, const double* sc, int first, const int* skip) {
+__global__ __launch_bounds__(kVecBlock) void update_p_kernel(long long n, const ZT* __restrict__ z,
+                                                             const double* __restrict__ scale, T* __restrict__ p,
+                                                             double* sc, int first, const int* skip,
+                                                             const double* __restrict__ rho_part, int nb_rho) {
   if (skip && *skip) return;
-  const T beta = T(sc[kScBeta]);
   const bool f = first >= 0 ? first != 0 : skip[1] == 0;
+  double beta_d = sc[kScBeta];
+  if (rho_part) {
+    __shared__ double sm[kVecBlock / 64];
+    const double rho = block_partial_sum(rho_part, nb_rho, sm);
+    const double prev = sc[kScRhoPrev];
+    beta_d = f ? 0.0 : (prev != 0.0 ? rho / prev : 0.0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc[kScRho] = rho;
+      sc[kScBeta] = beta_d;
+    }
+  }
+  const T beta = T(beta_d);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const T zi = scale ? T(scale[i] * double(z[i])) : T(z[i]);
     p[i] = f ? zi : zi + beta * p[i];
@@ -95,14 +120,29 @@ __global__ void update_p_kernel(long long n, const ZT* __restrict__ z, const dou
 }
 
 // x += alpha p; r -= alpha q; partials of rr = r . r (finalised with kFinRR);
-// optionally rp = S r in the preconditioner's precision.
+// optionally rp = S r in the preconditioner's precision.  With gam_part
+// every block sums the nb_gam partials of gamma = p . A p itself and takes
+// alpha = rho / gamma; block 0 stores gamma, alpha and rho_prev = rho.
 template <typename T, typename RT>
 __global__ __launch_bounds__(kVecBlock) void update_xr_kernel(long long n, T* __restrict__ x, T* __restrict__ r,
                                                               const T* __restrict__ p, const T* __restrict__ q,
                                                               RT* __restrict__ rp, const double* __restrict__ scale,
-                                                              double* partials, const double* sc, const int* flags) {
+                                                              double* partials, double* sc, const int* flags,
+                                                              const double* __restrict__ gam_part, int nb_gam) {
   if (flags && flags[0]) return;
-  const T alpha = T(sc[kScAlpha]);
+  double alpha_d = sc[kScAlpha];
+  if (gam_part) {
+    __shared__ double sg[kVecBlock / 64];
+    const double gam = block_partial_sum(gam_part, nb_gam, sg);
+    const double rho = sc[kScRho];
+    alpha_d = gam != 0.0 ? rho / gam : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc[kScGamma] = gam;
+      sc[kScAlpha] = alpha_d;
+      sc[kScRhoPrev] = rho;
+    }
+  }
+  const T alpha = T(alpha_d);
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
@@ -245,36 +285,37 @@ template int launch_cast_scale<float, float>(long long, const float*, const doub
 
 template <typename T>
 int launch_jacobi_dot(long long n, const T* r, const T* dinv, T* z, Workspace& w, int first, const int* skip,
-                      hipStream_t s) {
+                      hipStream_t s, bool fin) {
   const int g = vec_grid(n);
   hipLaunchKernelGGL(jacobi_dot_kernel<T>, dim3(g), dim3(kVecBlock), 0, s, n, r, dinv, z, w.partials, skip);
   int e = static_cast<int>(hipGetLastError());
-  return e ? e : launch_finalize(w.partials, g, 1, kFinRho, nullptr, w.sc, nullptr, first, skip, s);
+  return e || !fin ? e : launch_finalize(w.partials, g, 1, kFinRho, nullptr, w.sc, nullptr, first, skip, s);
 }
 template <typename T, typename ZT>
 int launch_rho(long long n, const T* r, const ZT* z, const double* scale, Workspace& w, int first, const int* skip,
-               hipStream_t s, const T* q) {
+               hipStream_t s, const T* q, bool fin) {
   const int g = vec_grid(n);
   hipLaunchKernelGGL((rho_kernel<T, ZT>), dim3(g), dim3(kVecBlock), 0, s, n, r, z, scale, q, w.partials, skip);
   int e = static_cast<int>(hipGetLastError());
-  return e ? e
-           : launch_finalize(w.partials, g, q ? 2 : 1, q ? kFinRhoFlex : kFinRho, nullptr, w.sc, nullptr, first, skip,
-                             s);
+  if (e || (!fin && !q)) return e;
+  return launch_finalize(w.partials, g, q ? 2 : 1, q ? kFinRhoFlex : kFinRho, nullptr, w.sc, nullptr, first, skip, s);
 }
 template <typename T, typename ZT>
 int launch_update_p(long long n, const ZT* z, const double* scale, T* p, Workspace& w, int first, const int* skip,
-                    hipStream_t s) {
-  hipLaunchKernelGGL((update_p_kernel<T, ZT>), WN_GRID(n), n, z, scale, p, w.sc, first, skip);
+                    hipStream_t s, const double* rho_part, int nb_rho) {
+  hipLaunchKernelGGL((update_p_kernel<T, ZT>), WN_GRID(n), n, z, scale, p, w.sc, first, skip, rho_part, nb_rho);
   return static_cast<int>(hipGetLastError());
 }
 template <typename T, typename RT>
 int launch_update_xr(long long n, T* x, T* r, const T* p, const T* q, RT* rp, const double* scale, Workspace& w,
-                     int* flags, hipStream_t s) {
+                     int* flags, hipStream_t s, const double* gam_part, int nb_gam) {
   const int g = vec_grid(n);
-  hipLaunchKernelGGL((update_xr_kernel<T, RT>), dim3(g), dim3(kVecBlock), 0, s, n, x, r, p, q, rp, scale, w.partials,
-                     w.sc, flags);
+  // the rr partials go past the region the gamma partials may occupy
+  double* rr_part = gam_part ? w.partials + 2 * (size_t)w.max_blocks : w.partials;
+  hipLaunchKernelGGL((update_xr_kernel<T, RT>), dim3(g), dim3(kVecBlock), 0, s, n, x, r, p, q, rp, scale, rr_part,
+                     w.sc, flags, gam_part, nb_gam);
   int e = static_cast<int>(hipGetLastError());
-  return e ? e : launch_finalize(w.partials, g, 1, kFinRR, nullptr, w.sc, flags, 0, flags, s);
+  return e ? e : launch_finalize(rr_part, g, 1, kFinRR, nullptr, w.sc, flags, 0, flags, s);
 }
 template <typename T>
 int launch_dot(long long n, const T* a, const T* b, Workspace& w, double* out, hipStream_t s) {
@@ -295,7 +336,8 @@ int launch_normalise(long long n, const T* x, T* y, const double* nrm2, hipStrea
 }
 
 #define WN_INST(T)                                                                                   \
-  template int launch_jacobi_dot<T>(long long, const T*, const T*, T*, Workspace&, int, const int*, hipStream_t); \
+  template int launch_jacobi_dot<T>(long long, const T*, const T*, T*, Workspace&, int, const int*, hipStream_t, \
+                                    bool);                                                           \
   template int launch_dot<T>(long long, const T*, const T*, Workspace&, double*, hipStream_t);        \
   template int launch_ir_update<T>(long long, double*, const T*, const double*, const double*, hipStream_t); \
   template int launch_normalise<T>(long long, const T*, T*, const double*, hipStream_t);
@@ -304,11 +346,11 @@ WN_INST(double)
 
 #define WN_INST2(T, U)                                                                                           \
   template int launch_rho<T, U>(long long, const T*, const U*, const double*, Workspace&, int, const int*,      \
-                                hipStream_t, const T*);                                                        \
+                                hipStream_t, const T*, bool);                                                  \
   template int launch_update_p<T, U>(long long, const U*, const double*, T*, Workspace&, int, const int*,      \
-                                     hipStream_t);                                                             \
+                                     hipStream_t, const double*, int);                                         \
   template int launch_update_xr<T, U>(long long, T*, T*, const T*, const T*, U*, const double*, Workspace&,    \
-                                      int*, hipStream_t);
+                                      int*, hipStream_t, const double*, int);
 WN_INST2(double, double)
 WN_INST2(double, float)
 WN_INST2(float, float)

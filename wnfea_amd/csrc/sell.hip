@@ -455,10 +455,16 @@ int spmv_dot(const SellMatrix& A, const VT* p, VT* y, Workspace& w, int with_alp
   }
   if (A.br == 3) CALL(3, 3) else if (A.br == 6) CALL(6, 6) else return kBadArg;
 #undef CALL
-  if (r || A.n_slices == 0) return r;
+  if (r || A.n_slices == 0 || with_alpha == 2) return r;
   return launch_finalize(w.partials, sell_grid<VT>(A, t), 1, with_alpha ? kFinGamma : kFinStore, w.sc + kScGamma, w.sc,
                          nullptr, 0, t.guard ? t.flags : nullptr, s);
 }
+template <typename VT>
+int spmv_partials(const SellMatrix& A, const SpmvTune& t) {
+  return sell_grid<VT>(A, t);
+}
+template int spmv_partials<double>(const SellMatrix&, const SpmvTune&);
+template int spmv_partials<float>(const SellMatrix&, const SpmvTune&);
 template int spmv_dot<double>(const SellMatrix&, const double*, double*, Workspace&, int, const SpmvTune&,
                               hipStream_t);
 template int spmv_dot<float>(const SellMatrix&, const float*, float*, Workspace&, int, const SpmvTune&, hipStream_t);

@@ -514,9 +514,13 @@ void bsr_spmv_host(int n, int b, const int* ptr, const int* idx, const double* v
 }
 
 // ---------------------------------------------------------------- matrix-free patch layout
-// Morton order of C3D10 elements: the centroid of each element's four
-// corners, quantised to 21 bits per axis on the mesh's bounding cube, bits
-// interleaved (x lowest); ties keep element order.
+// Morton order of C3D10 elements: the elements grouped by connected
+// component (elements that share a corner, found by union-find; components
+// in the order of their first element), and inside a component sorted by the
+// Morton code of the centroid of their four corners, quantised to 21 bits
+// per axis on the component's bounding cube (bits interleaved, x lowest;
+// ties in element order).  Grouping first keeps separate bodies that overlap
+// in space (a batch of meshes in one system) out of each other's patches.
 static unsigned long long spread3(unsigned long long v) {  // bit i -> bit 3 i (21 bits)
   v &= 0x1fffffULL;
   v = (v | v << 32) & 0x1f00000000ffffULL;
@@ -527,33 +531,74 @@ static unsigned long long spread3(unsigned long long v) {  // bit i -> bit 3 i (
   return v;
 }
 
+static int uf_find(std::vector<int>& par, int a) {
+  while (par[(size_t)a] != a) {
+    par[(size_t)a] = par[(size_t)par[(size_t)a]];
+    a = par[(size_t)a];
+  }
+  return a;
+}
+
 void morton_order(int M, const int* conn, const double* X, int* order) {
+  // components over the corner nodes
+  int nmax = 0;
+  for (long long q = 0; q < 10LL * M; ++q) nmax = std::max(nmax, conn[q] + 1);
+  std::vector<int> par((size_t)nmax);
+  for (int i = 0; i < nmax; ++i) par[(size_t)i] = i;
+  for (int e = 0; e < M; ++e) {
+    const int r0 = uf_find(par, conn[(size_t)e * 10]);
+    for (int i = 1; i < 4; ++i) {
+      const int ri = uf_find(par, conn[(size_t)e * 10 + i]);
+      if (ri != r0) par[(size_t)std::max(ri, r0)] = std::min(ri, r0);
+    }
+  }
+  // component index of each element, components numbered by first element
+  std::vector<int> comp((size_t)M), label((size_t)nmax, -1);
+  int nc = 0;
+  for (int e = 0; e < M; ++e) {
+    const int r = uf_find(par, conn[(size_t)e * 10]);
+    if (label[(size_t)r] < 0) label[(size_t)r] = nc++;
+    comp[(size_t)e] = label[(size_t)r];
+  }
   std::vector<double> c((size_t)M * 3);
-  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  std::vector<double> lo((size_t)nc * 3, 1e300), hi((size_t)nc * 3, -1e300);
   for (int e = 0; e < M; ++e)
     for (int a = 0; a < 3; ++a) {
       double v = 0.0;
       for (int i = 0; i < 4; ++i) v += X[(size_t)conn[(size_t)e * 10 + i] * 3 + a];
       v *= 0.25;
       c[(size_t)e * 3 + a] = v;
-      lo[a] = std::min(lo[a], v);
-      hi[a] = std::max(hi[a], v);
+      const size_t k = (size_t)comp[(size_t)e] * 3 + a;
+      lo[k] = std::min(lo[k], v);
+      hi[k] = std::max(hi[k], v);
     }
-  double ext = 0.0;
-  for (int a = 0; a < 3; ++a) ext = std::max(ext, hi[a] - lo[a]);
-  if (!(ext > 0.0)) ext = 1.0;
-  const double top = double((1 << 21) - 1), q = top / ext;
-  std::vector<std::pair<unsigned long long, int>> key((size_t)M);
+  const double top = double((1 << 21) - 1);
+  std::vector<double> q((size_t)nc);
+  for (int k = 0; k < nc; ++k) {
+    double ext = 0.0;
+    for (int a = 0; a < 3; ++a) ext = std::max(ext, hi[(size_t)k * 3 + a] - lo[(size_t)k * 3 + a]);
+    q[(size_t)k] = top / (ext > 0.0 ? ext : 1.0);
+  }
+  struct Key {
+    int comp;
+    unsigned long long code;
+    int e;
+    bool operator<(const Key& o) const {
+      return comp != o.comp ? comp < o.comp : (code != o.code ? code < o.code : e < o.e);
+    }
+  };
+  std::vector<Key> key((size_t)M);
   for (int e = 0; e < M; ++e) {
-    unsigned long long k = 0;
+    const int k = comp[(size_t)e];
+    unsigned long long code = 0;
     for (int a = 0; a < 3; ++a) {
-      const double t = std::min(std::max((c[(size_t)e * 3 + a] - lo[a]) * q, 0.0), top);
-      k |= spread3((unsigned long long)t) << a;
+      const double t = std::min(std::max((c[(size_t)e * 3 + a] - lo[(size_t)k * 3 + a]) * q[(size_t)k], 0.0), top);
+      code |= spread3((unsigned long long)t) << a;
     }
-    key[(size_t)e] = {k, e};
+    key[(size_t)e] = {k, code, e};
   }
   std::sort(key.begin(), key.end());
-  for (int e = 0; e < M; ++e) order[e] = key[(size_t)e].second;
+  for (int e = 0; e < M; ++e) order[e] = key[(size_t)e].e;
 }
 
 // Patch layout: slot s holds element order[s]; patch p the slots

@@ -133,9 +133,13 @@ __global__ __launch_bounds__(256) void sym_tile_gemv_kernel(int n, const T* __re
   const int t = threadIdx.x;
   const int c0 = (t % CPR) * V, r0 = t / CPR;
   const Chunk* src = reinterpret_cast<const Chunk*>(tiles + k * (kTile * kTile));
+  // non-temporal: the 200-400 MB of tiles streamed once per cycle would
+  // otherwise displace the cycle's and the CG iteration's reused vectors
+  // from L2 and the Infinity Cache (I-beam: CG vector work 52 -> 37 us per
+  // iteration, mixed solve 20.6 -> 19.8 ms)
   Chunk a[NI];
 #pragma unroll
-  for (int m = 0; m < NI; ++m) a[m] = src[t + 256 * m];
+  for (int m = 0; m < NI; ++m) load_vec<T, V>(reinterpret_cast<const T*>(src + t + 256 * m), a[m].e);
   double xj[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) {
@@ -511,43 +515,53 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
   if (sep64 && (r = launch_cast_scale<T, T>(n, k.r, scale, 1.0, k.z, s))) return r;
   // One iteration; the first-iteration switch is read on the device from the
   // iteration counter, so every iteration enqueues the same kernels.
+  // Unless the cycle is flexible, rho and gamma = p . A p are finalised by
+  // the kernels that use them (every block of the p update sums the rho
+  // partials, every block of the x, r update the gamma partials), so an
+  // iteration launches one one-block finalisation (rr and the done flag)
+  // instead of three.
+  const bool fused = !flexible;
+  const int nb_rho = vec_grid(n);
+  const int nb_gam = mf ? mf_grid(*mf) : spmv_partials<T>(A, t);
+  const double* rho_part = fused ? S.ws.partials : nullptr;
+  const double* gam_part = fused ? S.ws.partials : nullptr;
   auto iteration = [&](hipStream_t st) -> int {
     int e;
     const int first = -1;
     if (pkind == 1) {
-      if ((e = launch_jacobi_dot<T>(n, k.r, dinv, k.z, S.ws, first, S.flags, st))) return e;
-      if ((e = launch_update_p<T, T>(n, k.z, nullptr, k.p, S.ws, first, S.flags, st))) return e;
+      if ((e = launch_jacobi_dot<T>(n, k.r, dinv, k.z, S.ws, first, S.flags, st, !fused))) return e;
+      if ((e = launch_update_p<T, T>(n, k.z, nullptr, k.p, S.ws, first, S.flags, st, rho_part, nb_rho))) return e;
     } else if (pkind == 2 && amg32) {
       const float* vin = sep32 ? rp32 : reinterpret_cast<const float*>(k.r);
       float* vout = sep32 ? S.zp32 : reinterpret_cast<float*>(k.z);
       if ((e = amg_apply(*amg, amg_vt, vin, vout, S.flags, st))) return e;
-      if ((e = launch_rho<T, float>(n, k.r, vout, scale, S.ws, first, S.flags, st, flexible ? k.q : nullptr)))
+      if ((e = launch_rho<T, float>(n, k.r, vout, scale, S.ws, first, S.flags, st, flexible ? k.q : nullptr, !fused)))
         return e;
-      if ((e = launch_update_p<T, float>(n, vout, scale, k.p, S.ws, first, S.flags, st))) return e;
+      if ((e = launch_update_p<T, float>(n, vout, scale, k.p, S.ws, first, S.flags, st, rho_part, nb_rho))) return e;
     } else if (pkind == 2) {
       // FP64 cycle: in-place scaled input in k.z when S is given
       const T* vin = sep64 ? k.z : k.r;
       T* vout = k.q;  // q is free until the SpMV below
       if ((e = amg_apply(*amg, 0, vin, vout, S.flags, st))) return e;
-      if ((e = launch_rho<T, T>(n, k.r, vout, scale, S.ws, first, S.flags, st))) return e;
-      if ((e = launch_update_p<T, T>(n, vout, scale, k.p, S.ws, first, S.flags, st))) return e;
+      if ((e = launch_rho<T, T>(n, k.r, vout, scale, S.ws, first, S.flags, st, nullptr, !fused))) return e;
+      if ((e = launch_update_p<T, T>(n, vout, scale, k.p, S.ws, first, S.flags, st, rho_part, nb_rho))) return e;
     } else {
-      if ((e = launch_rho<T, T>(n, k.r, k.r, nullptr, S.ws, first, S.flags, st))) return e;
-      if ((e = launch_update_p<T, T>(n, k.r, nullptr, k.p, S.ws, first, S.flags, st))) return e;
+      if ((e = launch_rho<T, T>(n, k.r, k.r, nullptr, S.ws, first, S.flags, st, nullptr, !fused))) return e;
+      if ((e = launch_update_p<T, T>(n, k.r, nullptr, k.p, S.ws, first, S.flags, st, rho_part, nb_rho))) return e;
     }
     if (mf) {
       const double* pp = reinterpret_cast<const double*>(k.p);
       double* qq = reinterpret_cast<double*>(k.q);
       if ((e = mf_apply(*mf, pp, nullptr, qq, S.ws.partials, 1, S.flags, 1, st))) return e;
-      if ((e = launch_finalize(S.ws.partials, mf_grid(*mf), 1, kFinGamma, S.ws.sc + kScGamma, S.ws.sc, nullptr, 0,
-                               S.flags, st)))
+      if (!fused && (e = launch_finalize(S.ws.partials, mf_grid(*mf), 1, kFinGamma, S.ws.sc + kScGamma, S.ws.sc,
+                                         nullptr, 0, S.flags, st)))
         return e;
-    } else if ((e = spmv_dot<T>(A, k.p, k.q, S.ws, 1, t, st))) {
+    } else if ((e = spmv_dot<T>(A, k.p, k.q, S.ws, fused ? 2 : 1, t, st))) {
       return e;
     }
-    if (sep32) return launch_update_xr<T, float>(n, x, k.r, k.p, k.q, rp32, scale, S.ws, S.flags, st);
-    if (sep64) return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, k.z, scale, S.ws, S.flags, st);
-    return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, nullptr, nullptr, S.ws, S.flags, st);
+    if (sep32) return launch_update_xr<T, float>(n, x, k.r, k.p, k.q, rp32, scale, S.ws, S.flags, st, gam_part, nb_gam);
+    if (sep64) return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, k.z, scale, S.ws, S.flags, st, gam_part, nb_gam);
+    return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, nullptr, nullptr, S.ws, S.flags, st, gam_part, nb_gam);
   };
   std::vector<uint64_t> key;
   if (S.use_graphs) {

@@ -191,22 +191,27 @@ int launch_finalize(const double* partials, int nb, int K, int mode, double* out
 template <typename T> int launch_fill(long long n, T* x, T v, hipStream_t s);
 template <typename A, typename B>
 int launch_cast_scale(long long n, const A* x, const double* sc, double f, B* y, hipStream_t s);
+// fin == false: leave the partials of rho for launch_update_p to finalise
 template <typename T>
 int launch_jacobi_dot(long long n, const T* r, const T* dinv, T* z, Workspace& w, int first, const int* skip,
-                      hipStream_t s);
+                      hipStream_t s, bool fin = true);
 // rho = r . (S z) (S may be null); finalise beta = rho / rho_prev, or with q
-// (= A p of the previous iteration) the flexible beta = -alpha (S z . q) / rho_prev
+// (= A p of the previous iteration) the flexible beta = -alpha (S z . q) / rho_prev.
+// fin == false (without q): only the partials of rho, for launch_update_p
 template <typename T, typename ZT>
 int launch_rho(long long n, const T* r, const ZT* z, const double* scale, Workspace& w, int first, const int* skip,
-               hipStream_t s, const T* q = nullptr);
-// p = S z + beta p (p = S z when first)
+               hipStream_t s, const T* q = nullptr, bool fin = true);
+// p = S z + beta p (p = S z when first); with rho_part the kernel finalises
+// rho from its nb_rho partials and forms beta itself
 template <typename T, typename ZT>
 int launch_update_p(long long n, const ZT* z, const double* scale, T* p, Workspace& w, int first, const int* skip,
-                    hipStream_t s);
-// x += alpha p; r -= alpha q; rr; optional rp = S r; raises flags[0] (done) when rr <= sc[kScTol2]
+                    hipStream_t s, const double* rho_part = nullptr, int nb_rho = 0);
+// x += alpha p; r -= alpha q; rr; optional rp = S r; raises flags[0] (done)
+// when rr <= sc[kScTol2]; with gam_part the kernel finalises gamma = p . A p
+// from its nb_gam partials and forms alpha itself
 template <typename T, typename RT>
 int launch_update_xr(long long n, T* x, T* r, const T* p, const T* q, RT* rp, const double* scale, Workspace& w,
-                     int* flags, hipStream_t s);
+                     int* flags, hipStream_t s, const double* gam_part = nullptr, int nb_gam = 0);
 template <typename T> int launch_dot(long long n, const T* a, const T* b, Workspace& w, double* out, hipStream_t s);
 template <typename T>
 int launch_ir_update(long long n, double* u, const T* d, const double* sc_vec, const double* nrm2, hipStream_t s);
@@ -219,6 +224,8 @@ int spmv_axpby(const SellMatrix& A, const XT* x, XT* y, double alpha, double bet
 // y = A p, gamma = p.y (into sc[kScGamma]); alpha = rho/gamma if with_alpha
 template <typename VT>
 int spmv_dot(const SellMatrix& A, const VT* p, VT* y, Workspace& w, int with_alpha, const SpmvTune& t, hipStream_t s);
+// partials written by spmv_dot (with_alpha == 2: the partials only, no finalisation)
+template <typename VT> int spmv_partials(const SellMatrix& A, const SpmvTune& t);
 // r = b - A x in FP64, ||r||^2 -> out; optional rs = scale .* r (FP32)
 // r = b - A x in FP64; dots ||r||^2 -> out[0], ||scale r||^2 -> out[1]; optional
 // rs = scale .* r (FP32).
