@@ -1741,6 +1741,54 @@ int mf_smooth_defect(const MfSmooth& m, const T* x, const T* b, T* res, const in
   return mf_smooth_run<T>(m, x, o, skip, s);
 }
 
+// A Chebyshev step (as OpCheb) that also forms the partials of
+// rho = r . (S z), z = the step's x_out as stored: the V-cycle's last step
+// hands the CG iteration its rho without a pass over r, S and z of its own.
+// Per DOF the same product as the separate rho kernel, r (S z).
+template <typename T>
+struct OpChebRho {
+  static constexpr int kDots = 1;
+  OpCheb<3, T> c;
+  const double* r;
+  const double* scale;
+  double* partials;
+  struct Pre {
+    typename OpCheb<3, T>::Pre c;
+    double r, s;
+  };
+  __device__ __forceinline__ void load(long long k, Pre& p) const {
+    c.load(k, p.c);
+    p.r = r[k];
+    p.s = scale ? scale[k] : 1.0;
+  }
+  __device__ __forceinline__ void out(long long k, T a, double (&dots)[2], const Pre& p) const {
+    const T dn = c.c1 * p.c.d + c.c2 * p.c.dinv * (p.c.b - a);
+    const T xo = p.c.x + dn;
+    c.d[k] = dn;
+    c.x_out[k] = xo;
+    dots[0] += p.r * (double(xo) * p.s);
+  }
+};
+
+template <typename T>
+int mf_smooth_rho_blocks(const MfSmooth& m) {
+  return mf_smooth_ready<T>(m) && mf_qdp<T>(m) ? node_grid(m.op->n_nodes) : 0;
+}
+
+template <typename T>
+int mf_smooth_cheb_rho(const MfSmooth& m, const T* x_in, T* x_out, const T* b, const T* dinv, T* d, T c1, T c2,
+                       const RhoReq& rq, const int* skip, hipStream_t s) {
+  const T* qdp = mf_smooth_ready<T>(m) ? mf_qdp<T>(m) : nullptr;
+  if (!qdp || !rq.r || !rq.partials) return kBadMode;
+  const MfOp& op = *m.op;
+  T* part = mf_part<T>(m);
+  const OpChebRho<T> o{{b, dinv, x_in, x_out, d, c1, c2}, rq.r, rq.scale, rq.partials};
+  launch_patch<T, std::is_same<T, float>::value>(op, qdp, mf_scale_in<T>(m), x_in, part, skip, s);
+  hipLaunchKernelGGL((mf_node_kernel<T, OpChebRho<T>>), dim3(node_grid(op.n_nodes)), dim3(kNodeThreads), 0, s,
+                     op.n_nodes, op.nhead, op.novf, part, op.fix, mf_scale<T>(m), x_in, o, skip);
+  return static_cast<int>(hipGetLastError());
+}
+
 template int mf_smooth_cheb<double>(const MfSmooth&, const double*, double*, const double*, const double*, double*,
                                     double, double, const int*, hipStream_t);
 template int mf_smooth_cheb<float>(const MfSmooth&, const float*, float*, const float*, const float*, float*, float,
@@ -1748,5 +1796,11 @@ template int mf_smooth_cheb<float>(const MfSmooth&, const float*, float*, const 
 template int mf_smooth_defect<double>(const MfSmooth&, const double*, const double*, double*, const int*,
                                       hipStream_t);
 template int mf_smooth_defect<float>(const MfSmooth&, const float*, const float*, float*, const int*, hipStream_t);
+template int mf_smooth_rho_blocks<double>(const MfSmooth&);
+template int mf_smooth_rho_blocks<float>(const MfSmooth&);
+template int mf_smooth_cheb_rho<double>(const MfSmooth&, const double*, double*, const double*, const double*,
+                                        double*, double, double, const RhoReq&, const int*, hipStream_t);
+template int mf_smooth_cheb_rho<float>(const MfSmooth&, const float*, float*, const float*, const float*, float*,
+                                       float, float, const RhoReq&, const int*, hipStream_t);
 
 }  // namespace wn

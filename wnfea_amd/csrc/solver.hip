@@ -251,7 +251,7 @@ static int coarse_apply(const Amg& g, const T* b, T* x, const int* skip, hipStre
 // and prolongation use VT / VTC).  With VS != VT the cycle is not symmetric:
 // it serves flexible PCG.
 template <typename VT, typename VTC, typename T, typename VS = VT, typename VSC = VTC>
-static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t s) {
+static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t s, const RhoReq* rq = nullptr) {
   AmgLevel& L = *g.L[l];
   SpmvTune t = g.tune;
   t.flags = skip;
@@ -280,7 +280,11 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   T* oth = (cur == x) ? tmp : x;
   // the finest level of an FP32 / FP64 cycle may run its products matrix-free
   const bool mf = l == 0 && std::is_same<VT, T>::value && mf_smooth_ready<T>(g.mfs);
-  auto cheb = [&](const T* xi, T* xo, int k) -> int {
+  // rq: the last post-smoothing step (matrix-free, patch layout) also writes
+  // the partials of rho = r . (S x) for the Krylov loop
+  if (rq && !mf) return kBadMode;
+  auto cheb = [&](const T* xi, T* xo, int k, const RhoReq* q = nullptr) -> int {
+    if (mf && q) return mf_smooth_cheb_rho<T>(g.mfs, xi, xo, b, dinv, d, T(c.c1[k]), T(c.c2[k]), *q, skip, s);
     if (mf) return mf_smooth_cheb<T>(g.mfs, xi, xo, b, dinv, d, T(c.c1[k]), T(c.c2[k]), skip, s);
     return spmv_cheb<VS, T>(A, xi, xo, b, dinv, d, c.c1[k], c.c2[k], t, s);
   };
@@ -299,29 +303,42 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   if ((r = spmv_addto<VT, T>(L.P, nv[1], cur, t, s))) return r;  // cur == post here
   oth = (cur == x) ? tmp : x;
   for (int k = 0; k < deg; ++k) {
-    if ((r = cheb(cur, oth, k))) return r;
+    if ((r = cheb(cur, oth, k, k == deg - 1 ? rq : nullptr))) return r;
     std::swap(cur, oth);
   }
   return kOk;
 }
 
-// One V-cycle, z = M r.  vt: 0 FP64 values and vectors, 1 FP32 values, 2 FP16
+// Partials the cycle of kind vt writes for a RhoReq (0: that cycle cannot
+// form rho: its finest level is not smoothed matrix-free in the patch layout,
+// or vt is not 0 or 1).
+int amg_rho_blocks(const Amg& g, int vt) {
+  if (g.L.size() < 2) return 0;
+  if (vt == 0) return (g.store & 1) ? mf_smooth_rho_blocks<double>(g.mfs) : 0;
+  if (vt == 1) return (g.store & 2) ? mf_smooth_rho_blocks<float>(g.mfs) : 0;
+  return 0;
+}
+
+// One V-cycle, z = M r; with rq also the partials of rho = r . (S z) (see
+// amg_rho_blocks).  vt: 0 FP64 values and vectors, 1 FP32 values, 2 FP16
 // values on every level, 3 FP16 values on the finest level and FP32 below it,
 // 4 FP32 on the finest level and FP16 below it (vt >= 1: FP32 vectors and
 // arithmetic).  (FP16 values in the finest smoother with an FP32 defect make
 // the cycle nonsymmetric by O(eps16) and CG then stalls, so that split is not
 // offered.)
-int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s) {
+int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s, const RhoReq* rq) {
   if (g.L.empty()) return kNotReady;
   const float* r32 = static_cast<const float*>(r);
   float* z32 = static_cast<float*>(z);
+  if (rq && amg_rho_blocks(g, vt) <= 0) return kBadMode;
   switch (vt) {
     case 0:
       if (!(g.store & 1)) return kNotReady;
-      return vcycle<double, double, double>(g, 0, static_cast<const double*>(r), static_cast<double*>(z), skip, s);
+      return vcycle<double, double, double>(g, 0, static_cast<const double*>(r), static_cast<double*>(z), skip, s,
+                                            rq);
     case 1:
       if (!(g.store & 2)) return kNotReady;
-      return vcycle<float, float, float>(g, 0, r32, z32, skip, s);
+      return vcycle<float, float, float>(g, 0, r32, z32, skip, s, rq);
     case 2:
       if (!(g.store & 4)) return kNotReady;
       return vcycle<_Float16, _Float16, float>(g, 0, r32, z32, skip, s);
@@ -489,7 +506,15 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
   int r;
   if ((r = S.init())) return r;
   if ((r = k.ensure(n))) return r;
-  if ((r = S.ws.alloc((mf ? mf_grid(*mf) : A.n_slices) + 8))) return r;
+  // a nonsymmetric cycle needs the flexible (Polak-Ribiere) beta, which reads
+  // q = A p of the previous iteration in the rho kernel
+  const bool flexible = pkind == 2 && amg_vt == 5;
+  // rho from the cycle's last smoothing step (FP64 Krylov vectors, a
+  // symmetric cycle whose finest level is smoothed matrix-free in the patch
+  // layout): that step writes the rho partials, the rho pass is skipped
+  const int nb_cycle_rho =
+      (pkind == 2 && amg && !flexible && std::is_same<T, double>::value) ? amg_rho_blocks(*amg, amg_vt) : 0;
+  if ((r = S.ws.alloc(std::max((mf ? mf_grid(*mf) : A.n_slices), nb_cycle_rho) + 8))) return r;
   SpmvTune t = tune0;
   t.flags = S.flags;
   t.guard = 1;
@@ -499,9 +524,6 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
   const bool amg32 = (pkind == 2) && amg_vt != 0;
   if (pkind == 2 && !amg32 && !std::is_same<T, double>::value) return kBadMode;
   const bool sep32 = amg32 && (!std::is_same<T, float>::value || scale != nullptr);
-  // a nonsymmetric cycle needs the flexible (Polak-Ribiere) beta, which reads
-  // q = A p of the previous iteration in the rho kernel
-  const bool flexible = pkind == 2 && amg_vt == 5;
   const bool sep64 = (pkind == 2) && !amg32 && scale != nullptr;
   if (sep32 && (r = S.ensure_p(n))) return r;
   float* rp32 = sep32 ? S.rp32 : nullptr;
@@ -521,7 +543,9 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
   // iteration launches one one-block finalisation (rr and the done flag)
   // instead of three.
   const bool fused = !flexible;
-  const int nb_rho = vec_grid(n);
+  const RhoReq rq{reinterpret_cast<const double*>(k.r), scale, S.ws.partials};
+  const RhoReq* cycle_rho = fused && nb_cycle_rho > 0 ? &rq : nullptr;
+  const int nb_rho = cycle_rho ? nb_cycle_rho : vec_grid(n);
   const int nb_gam = mf ? mf_grid(*mf) : spmv_partials<T>(A, t);
   const double* rho_part = fused ? S.ws.partials : nullptr;
   const double* gam_part = fused ? S.ws.partials : nullptr;
@@ -534,16 +558,18 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
     } else if (pkind == 2 && amg32) {
       const float* vin = sep32 ? rp32 : reinterpret_cast<const float*>(k.r);
       float* vout = sep32 ? S.zp32 : reinterpret_cast<float*>(k.z);
-      if ((e = amg_apply(*amg, amg_vt, vin, vout, S.flags, st))) return e;
-      if ((e = launch_rho<T, float>(n, k.r, vout, scale, S.ws, first, S.flags, st, flexible ? k.q : nullptr, !fused)))
+      if ((e = amg_apply(*amg, amg_vt, vin, vout, S.flags, st, cycle_rho))) return e;
+      if (!cycle_rho &&
+          (e = launch_rho<T, float>(n, k.r, vout, scale, S.ws, first, S.flags, st, flexible ? k.q : nullptr, !fused)))
         return e;
       if ((e = launch_update_p<T, float>(n, vout, scale, k.p, S.ws, first, S.flags, st, rho_part, nb_rho))) return e;
     } else if (pkind == 2) {
       // FP64 cycle: in-place scaled input in k.z when S is given
       const T* vin = sep64 ? k.z : k.r;
       T* vout = k.q;  // q is free until the SpMV below
-      if ((e = amg_apply(*amg, 0, vin, vout, S.flags, st))) return e;
-      if ((e = launch_rho<T, T>(n, k.r, vout, scale, S.ws, first, S.flags, st, nullptr, !fused))) return e;
+      if ((e = amg_apply(*amg, 0, vin, vout, S.flags, st, cycle_rho))) return e;
+      if (!cycle_rho && (e = launch_rho<T, T>(n, k.r, vout, scale, S.ws, first, S.flags, st, nullptr, !fused)))
+        return e;
       if ((e = launch_update_p<T, T>(n, vout, scale, k.p, S.ws, first, S.flags, st, rho_part, nb_rho))) return e;
     } else {
       if ((e = launch_rho<T, T>(n, k.r, k.r, nullptr, S.ws, first, S.flags, st, nullptr, !fused))) return e;

@@ -190,6 +190,10 @@ struct Solver {
 };
 
 int amg_alloc_vectors(Amg& g, int prec_mask);
+// One V-cycle z = M r of kind vt; with rq also the partials of rho = r . (S z)
+// (amg_rho_blocks(g, vt) of them; kBadMode when that is 0).
+int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s, const RhoReq* rq = nullptr);
+int amg_rho_blocks(const Amg& g, int vt);
 
 // Coarsest-level inverse in the cycle's arithmetic type.
 template <typename T> const T* coarse_inv(const Amg& g);

@@ -150,6 +150,22 @@ int mf_smooth_cheb(const MfSmooth& m, const T* x_in, T* x_out, const T* b, const
 // res = b - A x
 template <typename T>
 int mf_smooth_defect(const MfSmooth& m, const T* x, const T* b, T* res, const int* skip, hipStream_t s);
+// Partials of rho = r . (S z) requested from the Chebyshev step that writes
+// a V-cycle's output z (scale null: S = I); nb partials, see
+// mf_smooth_rho_blocks.
+struct RhoReq {
+  const double* r;
+  const double* scale;
+  double* partials;
+};
+// The Chebyshev step above, also writing the RhoReq partials of its x_out
+// (patch layout only: kBadMode otherwise).
+template <typename T>
+int mf_smooth_cheb_rho(const MfSmooth& m, const T* x_in, T* x_out, const T* b, const T* dinv, T* d, T c1, T c2,
+                       const RhoReq& rq, const int* skip, hipStream_t s);
+// Partials mf_smooth_cheb_rho writes in precision T (0: it cannot run).
+template <typename T>
+int mf_smooth_rho_blocks(const MfSmooth& m);
 std::vector<int> mf_chunks(const std::vector<int>& n2e_ptr);
 int mf_setup(MfOp& op, const double* nodes, int* bad, hipStream_t s);
 void mf_release(MfOp& op);
