@@ -534,12 +534,27 @@ int spmv_restrict(const SellMatrix& A, const T* x, T* y, const SpmvTune& t, hipS
 #undef CALL
 }
 
+template <typename VT, typename T>
+int spmv_restrict_first(const SellMatrix& A, const T* x, T* y, const T* dinv, T* x1, T* d1, double c2,
+                        const SpmvTune& t, hipStream_t s) {
+  using AT = typename AccT<VT>::type;
+#define CALL(R, C)                                         \
+  {                                                        \
+    OpRestrictFirst<R, T, AT> op{y, dinv, x1, d1, T(c2)};  \
+    return launch_sell<R, C, VT, T>(A, x, op, t, s);       \
+  }
+  WN_BLOCK_DISPATCH(A, CALL);
+#undef CALL
+}
+
 #define WN_VINST(VT, T)                                                                                          \
   template int spmv_cheb<VT, T>(const SellMatrix&, const T*, T*, const T*, const T*, T*, double, double,       \
                                 const SpmvTune&, hipStream_t);                                                 \
   template int spmv_defect<VT, T>(const SellMatrix&, const T*, const T*, T*, const SpmvTune&, hipStream_t);   \
   template int spmv_addto<VT, T>(const SellMatrix&, const T*, T*, const SpmvTune&, hipStream_t);              \
-  template int spmv_restrict<VT, T>(const SellMatrix&, const T*, T*, const SpmvTune&, hipStream_t);
+  template int spmv_restrict<VT, T>(const SellMatrix&, const T*, T*, const SpmvTune&, hipStream_t);           \
+  template int spmv_restrict_first<VT, T>(const SellMatrix&, const T*, T*, const T*, T*, T*, double,            \
+                                          const SpmvTune&, hipStream_t);
 WN_VINST(double, double)
 WN_VINST(float, float)
 WN_VINST(_Float16, float)

@@ -281,6 +281,37 @@ struct OpAxpby {
   }
 };
 
+// y = R x (restriction) and, from it, the coarse level's first Chebyshev
+// iterate from x = 0: x1 = d1 = c2 dinv .* y (the expression of the
+// separate first-step kernel, so the bits are the same).
+template <int BR, typename T, typename AT>
+struct OpRestrictFirst {
+  static constexpr int kDots = 0;
+  T* y;
+  const T* dinv;
+  T* x1;
+  T* d1;
+  T c2;
+  double* partials = nullptr;
+  struct Pre { T dinv; };
+  __device__ __forceinline__ void load(long long k, Pre& p) const { p.dinv = dinv[k]; }
+  __device__ __forceinline__ void put(long long k, T b, T dv) const {
+    y[k] = b;
+    const T v = c2 * dv * b;
+    d1[k] = v;
+    x1[k] = v;
+  }
+  __device__ __forceinline__ void out(long long k, AT a, double (&)[2], const Pre& p) const { put(k, T(a), p.dinv); }
+  __device__ __forceinline__ void row(int i, const AT (&acc)[BR], double (&)[2], int q, int L) const {
+#pragma unroll
+    for (int r = 0; r < BR; ++r) {
+      if ((r & (L - 1)) != q) continue;
+      const long long k = (long long)i * BR + r;
+      put(k, T(acc[r]), dinv[k]);
+    }
+  }
+};
+
 // y = A p and gamma = p . (A p) (block partials; the finaliser forms alpha = rho / gamma).
 template <int BR, typename YT, typename AT>
 struct OpSpmvDot {

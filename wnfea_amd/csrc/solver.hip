@@ -250,8 +250,22 @@ static int coarse_apply(const Amg& g, const T* b, T* x, const int* skip, hipStre
 // smoothing steps on this level / the coarser ones (the defect, restriction
 // and prolongation use VT / VTC).  With VS != VT the cycle is not symmetric:
 // it serves flexible PCG.
+// The buffer a level's pre-smoother starts in (its first iterate): the
+// post-smoother (deg steps, ping-pong) must end in x, so it starts in
+// `post`; the pre-smoother (1 elementwise + deg-1 product steps) is started
+// so that it ends in `post` too, and the prolongation adds in place there.
+template <typename T>
+static T* first_iterate(int deg, T* x, T* tmp) {
+  T* post = (deg % 2 == 0) ? x : tmp;
+  return ((deg - 1) % 2 == 0) ? post : (post == x ? tmp : x);
+}
+static int level_degree(const AmgLevel& L) { return std::max(1, std::min(L.degree, 16)); }
+
+// first_done: the caller's restriction wrote this level's first Chebyshev
+// iterate already (see spmv_restrict_first).
 template <typename VT, typename VTC, typename T, typename VS = VT, typename VSC = VTC>
-static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t s, const RhoReq* rq = nullptr) {
+static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t s, const RhoReq* rq = nullptr,
+                  bool first_done = false) {
   AmgLevel& L = *g.L[l];
   SpmvTune t = g.tune;
   t.flags = skip;
@@ -267,16 +281,11 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   T* res = v[4];
   const T* dinv = level_dinv<T>(L);
   if (!tmp || !dinv || !L.has_PR) return kNotReady;
-  const int deg = std::max(1, std::min(L.degree, 16));
+  const int deg = level_degree(L);
   const ChebCoef c = cheb_coefs(L.lmax, L.ratio, deg);
   const SellMatrix& A = *L.A;
   int r;
-  // Buffer plan: the post-smoother (deg SpMV steps, ping-pong) must end in x,
-  // so it starts in `post`; the pre-smoother (1 elementwise + deg-1 SpMV
-  // steps) is started so that it ends in `post` too, and the prolongation
-  // adds in place there.  No copies.
-  T* post = (deg % 2 == 0) ? x : tmp;
-  T* cur = ((deg - 1) % 2 == 0) ? post : (post == x ? tmp : x);
+  T* cur = first_iterate(deg, x, tmp);  // no copies: see first_iterate
   T* oth = (cur == x) ? tmp : x;
   // the finest level of an FP32 / FP64 cycle may run its products matrix-free
   const bool mf = l == 0 && std::is_same<VT, T>::value && mf_smooth_ready<T>(g.mfs);
@@ -288,8 +297,11 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
     if (mf) return mf_smooth_cheb<T>(g.mfs, xi, xo, b, dinv, d, T(c.c1[k]), T(c.c2[k]), skip, s);
     return spmv_cheb<VS, T>(A, xi, xo, b, dinv, d, c.c1[k], c.c2[k], t, s);
   };
-  hipLaunchKernelGGL(cheb_first_kernel<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, b, dinv, d, cur, T(c.c2[0]), skip);
-  if ((r = static_cast<int>(hipGetLastError()))) return r;
+  if (!first_done) {
+    hipLaunchKernelGGL(cheb_first_kernel<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, b, dinv, d, cur, T(c.c2[0]),
+                       skip);
+    if ((r = static_cast<int>(hipGetLastError()))) return r;
+  }
   for (int k = 1; k < deg; ++k) {
     if ((r = cheb(cur, oth, k))) return r;
     std::swap(cur, oth);
@@ -298,8 +310,20 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   if ((r = mf ? mf_smooth_defect<T>(g.mfs, cur, b, res, skip, s) : spmv_defect<VT, T>(A, cur, b, res, t, s))) return r;
   AmgLevel& N = *g.L[l + 1];
   T** nv = level_vecs<T>(N);
-  if ((r = spmv_restrict<VT, T>(L.R, res, nv[0], t, s))) return r;
-  if ((r = vcycle<VTC, VTC, T, VSC, VSC>(g, l + 1, nv[0], nv[1], skip, s))) return r;
+  // a smoothed coarser level gets its first Chebyshev iterate from the
+  // restriction's epilogue (one launch fewer per cycle)
+  const T* dinv_n = level_dinv<T>(N);
+  const bool fuse_first = l + 1 < (int)g.L.size() - 1 && nv[1] && nv[2] && nv[3] && dinv_n;
+  if (fuse_first) {
+    const int deg_n = level_degree(N);
+    const ChebCoef cn = cheb_coefs(N.lmax, N.ratio, deg_n);
+    if ((r = spmv_restrict_first<VT, T>(L.R, res, nv[0], dinv_n, first_iterate(deg_n, nv[1], nv[2]), nv[3],
+                                        cn.c2[0], t, s)))
+      return r;
+  } else if ((r = spmv_restrict<VT, T>(L.R, res, nv[0], t, s))) {
+    return r;
+  }
+  if ((r = vcycle<VTC, VTC, T, VSC, VSC>(g, l + 1, nv[0], nv[1], skip, s, nullptr, fuse_first))) return r;
   if ((r = spmv_addto<VT, T>(L.P, nv[1], cur, t, s))) return r;  // cur == post here
   oth = (cur == x) ? tmp : x;
   for (int k = 0; k < deg; ++k) {

@@ -258,6 +258,10 @@ template <typename VT, typename T> int spmv_addto(const SellMatrix& A, const T* 
                                                   hipStream_t s);
 template <typename VT, typename T>
 int spmv_restrict(const SellMatrix& A, const T* x, T* y, const SpmvTune& t, hipStream_t s);
+// y = R x and the next level's first Chebyshev iterate x1 = d1 = c2 dinv .* y
+template <typename VT, typename T>
+int spmv_restrict_first(const SellMatrix& A, const T* x, T* y, const T* dinv, T* x1, T* d1, double c2,
+                        const SpmvTune& t, hipStream_t s);
 
 // standard block layout <-> sliced layout, and assembly
 int sell_from_std(SellMatrix& A, const double* vals_std, int to64, int to32, hipStream_t s, int to16 = 0);
