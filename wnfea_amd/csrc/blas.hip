@@ -141,7 +141,9 @@ __global__ __launch_bounds__(kVecBlock) void update_xr_kernel(long long n, T* __
                                                               const T* __restrict__ p, const T* __restrict__ q,
                                                               RT* __restrict__ rp, const double* __restrict__ scale,
                                                               double* partials, double* sc, const int* flags,
-                                                              const double* __restrict__ gam_part, int nb_gam) {
+                                                              const double* __restrict__ gam_part, int nb_gam,
+                                                              const RT* __restrict__ fdinv, RT* __restrict__ fx1,
+                                                              RT* __restrict__ fd1, RT fc2) {
   if (flags && flags[0]) return;
   double alpha_d = sc[kScAlpha];
   if (gam_part) {
@@ -161,7 +163,15 @@ __global__ __launch_bounds__(kVecBlock) void update_xr_kernel(long long n, T* __
     x[i] += alpha * p[i];
     const T ri = r[i] - alpha * q[i];
     r[i] = ri;
-    if (rp) rp[i] = RT(scale ? scale[i] * double(ri) : double(ri));
+    if (rp) {
+      const RT b = RT(scale ? scale[i] * double(ri) : double(ri));
+      rp[i] = b;
+      if (fx1) {  // the next cycle's first iterate, as the first-step kernel forms it
+        const RT v = fc2 * fdinv[i] * b;
+        fd1[i] = v;
+        fx1[i] = v;
+      }
+    }
     acc += double(ri) * double(ri);
   }
   __shared__ double sm[kVecBlock / 64];
@@ -321,12 +331,14 @@ int launch_update_p(long long n, const ZT* z, const double* scale, T* p, Workspa
 }
 template <typename T, typename RT>
 int launch_update_xr(long long n, T* x, T* r, const T* p, const T* q, RT* rp, const double* scale, Workspace& w,
-                     int* flags, hipStream_t s, const double* gam_part, int nb_gam) {
+                     int* flags, hipStream_t s, const double* gam_part, int nb_gam, const FirstIter<RT>* fi) {
   const int g = vec_grid(n);
   // the rr partials go past the region the gamma partials may occupy
   double* rr_part = gam_part ? w.partials + 2 * (size_t)w.max_blocks : w.partials;
+  const bool f = fi && rp && fi->x1;
   hipLaunchKernelGGL((update_xr_kernel<T, RT>), dim3(g), dim3(kVecBlock), 0, s, n, x, r, p, q, rp, scale, rr_part,
-                     w.sc, flags, gam_part, nb_gam);
+                     w.sc, flags, gam_part, nb_gam, f ? fi->dinv : nullptr, f ? fi->x1 : nullptr,
+                     f ? fi->d1 : nullptr, f ? fi->c2 : RT(0));
   int e = static_cast<int>(hipGetLastError());
   return e ? e : launch_finalize(rr_part, g, 1, kFinRR, nullptr, w.sc, flags, 0, flags, s);
 }
@@ -363,7 +375,7 @@ WN_INST(double)
   template int launch_update_p<T, U>(long long, const U*, const double*, T*, Workspace&, int, const int*,      \
                                      hipStream_t, const double*, int);                                         \
   template int launch_update_xr<T, U>(long long, T*, T*, const T*, const T*, U*, const double*, Workspace&,    \
-                                      int*, hipStream_t, const double*, int);
+                                      int*, hipStream_t, const double*, int, const FirstIter<U>*);
 WN_INST2(double, double)
 WN_INST2(double, float)
 WN_INST2(float, float)

@@ -36,7 +36,8 @@ int sa_tentative(int n_nodes, int b, int k, const int* agg, int n_agg, const dou
 struct Amg;
 struct AmgLevel;
 struct Solver;
-int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s, const RhoReq* rq);
+int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s, const RhoReq* rq,
+              bool first_done);
 int amg_level_cheb(Amg& g, int l, int vt, int reps, hipStream_t s);
 int amg_set_coarse_fp64(Amg& g, int fp64);
 template <typename T>
@@ -438,7 +439,7 @@ int wnfea_amg_tune_precision(void* h, int prec, int waves, int xcd_remap, int un
 // finest level and FP32 below, 4 = FP32 finest and FP16 below (vt >= 1: FP32 vectors)
 int wnfea_amg_vcycle(void* h, int vt, const void* r, void* z, void* stream) {
   if (!h || !r || !z) return kBadArg;
-  return amg_apply(*static_cast<Amg*>(h), vt, r, z, nullptr, S(stream), nullptr);
+  return amg_apply(*static_cast<Amg*>(h), vt, r, z, nullptr, S(stream), nullptr, false);
 }
 
 // FP32 cycles: coarsest solve with the FP64 inverse (fp64 != 0) or its FP32 copy

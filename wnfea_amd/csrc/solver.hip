@@ -333,6 +333,33 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   return kOk;
 }
 
+template <typename T>
+bool amg_first_iter(Amg& g, int vt, T* z, FirstIter<T>& f) {
+  const bool kind = std::is_same<T, double>::value ? (vt == 0 && (g.store & 1)) : (vt == 1 && (g.store & 2));
+  if (!kind || g.L.size() < 2 || !z) return false;
+  AmgLevel& L = *g.L[0];
+  T** v = level_vecs<T>(L);
+  const T* dinv = level_dinv<T>(L);
+  if (!v[2] || !v[3] || !dinv || !L.has_PR) return false;
+  const int deg = level_degree(L);
+  const ChebCoef c = cheb_coefs(L.lmax, L.ratio, deg);
+  f.dinv = dinv;
+  f.x1 = first_iterate(deg, z, v[2]);
+  f.d1 = v[3];
+  f.c2 = T(c.c2[0]);
+  return true;
+}
+template bool amg_first_iter<double>(Amg&, int, double*, FirstIter<double>&);
+template bool amg_first_iter<float>(Amg&, int, float*, FirstIter<float>&);
+
+template <typename T>
+int amg_first_launch(const FirstIter<T>& f, long long n, const T* b, const int* skip, hipStream_t s) {
+  hipLaunchKernelGGL(cheb_first_kernel<T>, dim3(vec_grid(n)), dim3(256), 0, s, n, b, f.dinv, f.d1, f.x1, f.c2, skip);
+  return static_cast<int>(hipGetLastError());
+}
+template int amg_first_launch<double>(const FirstIter<double>&, long long, const double*, const int*, hipStream_t);
+template int amg_first_launch<float>(const FirstIter<float>&, long long, const float*, const int*, hipStream_t);
+
 // Partials the cycle of kind vt writes for a RhoReq (0: that cycle cannot
 // form rho: its finest level is not smoothed matrix-free in the patch layout,
 // or vt is not 0 or 1).
@@ -350,7 +377,8 @@ int amg_rho_blocks(const Amg& g, int vt) {
 // arithmetic).  (FP16 values in the finest smoother with an FP32 defect make
 // the cycle nonsymmetric by O(eps16) and CG then stalls, so that split is not
 // offered.)
-int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s, const RhoReq* rq) {
+int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s, const RhoReq* rq,
+              bool first_done) {
   if (g.L.empty()) return kNotReady;
   const float* r32 = static_cast<const float*>(r);
   float* z32 = static_cast<float*>(z);
@@ -359,10 +387,10 @@ int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream
     case 0:
       if (!(g.store & 1)) return kNotReady;
       return vcycle<double, double, double>(g, 0, static_cast<const double*>(r), static_cast<double*>(z), skip, s,
-                                            rq);
+                                            rq, first_done);
     case 1:
       if (!(g.store & 2)) return kNotReady;
-      return vcycle<float, float, float>(g, 0, r32, z32, skip, s, rq);
+      return vcycle<float, float, float>(g, 0, r32, z32, skip, s, rq, first_done);
     case 2:
       if (!(g.store & 4)) return kNotReady;
       return vcycle<_Float16, _Float16, float>(g, 0, r32, z32, skip, s);
@@ -559,6 +587,19 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
   hipLaunchKernelGGL(pcg_start_kernel, dim3(1), dim3(64), 0, s, S.ws.sc, S.flags, rtol * rtol);
   if (sep32 && (r = launch_cast_scale<T, float>(n, k.r, scale, 1.0, rp32, s))) return r;
   if (sep64 && (r = launch_cast_scale<T, T>(n, k.r, scale, 1.0, k.z, s))) return r;
+  // The finest level's first Chebyshev iterate of each cycle is formed by
+  // the kernel that writes the cycle's input, S r: the x/r update of the
+  // previous iteration, and one launch here for the first iteration.
+  FirstIter<float> fi32;
+  FirstIter<T> fi64;
+  bool first_done = false;
+  if (pkind == 2 && !flexible && sep32) {
+    first_done = amg_first_iter<float>(*amg, amg_vt, S.zp32, fi32) &&
+                 !(r = amg_first_launch<float>(fi32, n, rp32, nullptr, s));
+  } else if (pkind == 2 && sep64 && std::is_same<T, double>::value) {
+    first_done = amg_first_iter<T>(*amg, 0, k.q, fi64) && !(r = amg_first_launch<T>(fi64, n, k.z, nullptr, s));
+  }
+  if (r) return r;
   // One iteration; the first-iteration switch is read on the device from the
   // iteration counter, so every iteration enqueues the same kernels.
   // Unless the cycle is flexible, rho and gamma = p . A p are finalised by
@@ -582,7 +623,7 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
     } else if (pkind == 2 && amg32) {
       const float* vin = sep32 ? rp32 : reinterpret_cast<const float*>(k.r);
       float* vout = sep32 ? S.zp32 : reinterpret_cast<float*>(k.z);
-      if ((e = amg_apply(*amg, amg_vt, vin, vout, S.flags, st, cycle_rho))) return e;
+      if ((e = amg_apply(*amg, amg_vt, vin, vout, S.flags, st, cycle_rho, first_done))) return e;
       if (!cycle_rho &&
           (e = launch_rho<T, float>(n, k.r, vout, scale, S.ws, first, S.flags, st, flexible ? k.q : nullptr, !fused)))
         return e;
@@ -591,7 +632,7 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
       // FP64 cycle: in-place scaled input in k.z when S is given
       const T* vin = sep64 ? k.z : k.r;
       T* vout = k.q;  // q is free until the SpMV below
-      if ((e = amg_apply(*amg, 0, vin, vout, S.flags, st, cycle_rho))) return e;
+      if ((e = amg_apply(*amg, 0, vin, vout, S.flags, st, cycle_rho, first_done))) return e;
       if (!cycle_rho && (e = launch_rho<T, T>(n, k.r, vout, scale, S.ws, first, S.flags, st, nullptr, !fused)))
         return e;
       if ((e = launch_update_p<T, T>(n, vout, scale, k.p, S.ws, first, S.flags, st, rho_part, nb_rho))) return e;
@@ -609,8 +650,12 @@ int pcg_run(Solver& S, const SellMatrix& A, int pkind, const T* dinv, Amg* amg, 
     } else if ((e = spmv_dot<T>(A, k.p, k.q, S.ws, fused ? 2 : 1, t, st))) {
       return e;
     }
-    if (sep32) return launch_update_xr<T, float>(n, x, k.r, k.p, k.q, rp32, scale, S.ws, S.flags, st, gam_part, nb_gam);
-    if (sep64) return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, k.z, scale, S.ws, S.flags, st, gam_part, nb_gam);
+    if (sep32)
+      return launch_update_xr<T, float>(n, x, k.r, k.p, k.q, rp32, scale, S.ws, S.flags, st, gam_part, nb_gam,
+                                        first_done ? &fi32 : nullptr);
+    if (sep64)
+      return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, k.z, scale, S.ws, S.flags, st, gam_part, nb_gam,
+                                    first_done ? &fi64 : nullptr);
     return launch_update_xr<T, T>(n, x, k.r, k.p, k.q, nullptr, nullptr, S.ws, S.flags, st, gam_part, nb_gam);
   };
   std::vector<uint64_t> key;

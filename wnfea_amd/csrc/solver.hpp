@@ -192,8 +192,19 @@ struct Solver {
 int amg_alloc_vectors(Amg& g, int prec_mask);
 // One V-cycle z = M r of kind vt; with rq also the partials of rho = r . (S z)
 // (amg_rho_blocks(g, vt) of them; kBadMode when that is 0).
-int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s, const RhoReq* rq = nullptr);
+// first_done: the finest level's first Chebyshev iterate is in place already
+// (see amg_first_iter).
+int amg_apply(Amg& g, int vt, const void* r, void* z, const int* skip, hipStream_t s, const RhoReq* rq = nullptr,
+              bool first_done = false);
 int amg_rho_blocks(const Amg& g, int vt);
+// Where the finest level's first Chebyshev iterate of a cycle of kind vt
+// (0: T double, 1: T float) with output z goes, so that the kernel writing
+// the cycle's input can form it (false: no such cycle).
+template <typename T>
+bool amg_first_iter(Amg& g, int vt, T* z, FirstIter<T>& f);
+// That first iterate from the input b, as the cycle's first-step kernel.
+template <typename T>
+int amg_first_launch(const FirstIter<T>& f, long long n, const T* b, const int* skip, hipStream_t s);
 
 // Coarsest-level inverse in the cycle's arithmetic type.
 template <typename T> const T* coarse_inv(const Amg& g);

@@ -222,12 +222,23 @@ int launch_rho(long long n, const T* r, const ZT* z, const double* scale, Worksp
 template <typename T, typename ZT>
 int launch_update_p(long long n, const ZT* z, const double* scale, T* p, Workspace& w, int first, const int* skip,
                     hipStream_t s, const double* rho_part = nullptr, int nb_rho = 0);
-// x += alpha p; r -= alpha q; rr; optional rp = S r; raises flags[0] (done)
-// when rr <= sc[kScTol2]; with gam_part the kernel finalises gamma = p . A p
-// from its nb_gam partials and forms alpha itself
+// The finest level's first Chebyshev iterate of the next V-cycle, written by
+// the kernel that writes the cycle's input b: x1 = d1 = c2 dinv .* b.
+template <typename T>
+struct FirstIter {
+  const T* dinv = nullptr;
+  T* x1 = nullptr;
+  T* d1 = nullptr;
+  T c2 = T(0);
+};
+// x += alpha p; r -= alpha q; rr; optional rp = S r (and with fi the next
+// cycle's first iterate from it); raises flags[0] (done) when
+// rr <= sc[kScTol2]; with gam_part the kernel finalises gamma = p . A p from
+// its nb_gam partials and forms alpha itself
 template <typename T, typename RT>
 int launch_update_xr(long long n, T* x, T* r, const T* p, const T* q, RT* rp, const double* scale, Workspace& w,
-                     int* flags, hipStream_t s, const double* gam_part = nullptr, int nb_gam = 0);
+                     int* flags, hipStream_t s, const double* gam_part = nullptr, int nb_gam = 0,
+                     const FirstIter<RT>* fi = nullptr);
 template <typename T> int launch_dot(long long n, const T* a, const T* b, Workspace& w, double* out, hipStream_t s);
 template <typename T>
 int launch_ir_update(long long n, double* u, const T* d, const double* sc_vec, const double* nrm2, hipStream_t s);
