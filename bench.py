@@ -14,7 +14,8 @@ the same solve with the all-FP64 AMG-PCG and the speedup with a per-part
 breakdown, the FP32 / FP64 3x3 BSR SpMV bandwidth (the metric's SpMV
 figure), a roofline object for the dominant kernel of the timed solve (the
 FP32 6x6 Chebyshev smoothing step of AMG level 1, three launches per
-iteration), the displacement and von Mises errors against the CPU FP64
+iteration; with the read rate a plain streaming kernel reaches on the same
+device, the practical ceiling next to the datasheet peak), the displacement and von Mises errors against the CPU FP64
 oracle, and a CPU baseline (the FP64 oracle on the host cores: a bounded
 sample on the I-beam, extrapolated, and a measured solve to tolerance of the
 19 683-DOF box).
@@ -65,6 +66,41 @@ def time_level_cheb(amg, level: int, precision: str, reps: int) -> float:
     en.record(stream)
     torch.cuda.synchronize()
     return st.elapsed_time(en) / 1e3 / reps
+
+
+def stream_ceiling(reps: int = 10) -> dict:
+    """Read rate of the library's HBM streaming-read probe (eight 16-byte
+    non-temporal loads in flight per thread, 512-thread blocks) over a 1 GiB
+    buffer, the best of three grid sizes, HIP events on the launch stream:
+    the practical ceiling of a bandwidth-bound kernel on this device, next
+    to the datasheet's 8 TB/s."""
+    import torch
+    from wnfea_amd import _lib
+    L = _lib.load()
+    nbytes = 1 << 30
+    a = torch.zeros(nbytes // 4, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    per_grid = {}
+    for grid in (2048, 4096, 8192):
+        out = torch.empty(grid * 512, dtype=torch.float32, device="cuda")
+
+        def run():
+            _lib.check(L.wnfea_hbm_stream_read(_lib.dptr(a), nbytes, _lib.dptr(out), grid, _lib.stream_of()),
+                       "wnfea_hbm_stream_read")
+        run()
+        run()
+        st, en = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        st.record(stream)
+        for _ in range(reps):
+            run()
+        en.record(stream)
+        torch.cuda.synchronize()
+        per_grid[grid] = round(nbytes / (st.elapsed_time(en) / 1e3 / reps) / 1e9, 1)
+    del a
+    torch.cuda.empty_cache()
+    best = max(per_grid, key=per_grid.get)
+    return {"gbs": per_grid[best], "grid": best, "per_grid_gbs": per_grid, "bytes": nbytes}
 
 
 # Element stiffness kernel cost per C3D10 element, the dense convention of a
@@ -1080,6 +1116,9 @@ def main() -> int:
         ch32, ch64 = cheb_bytes(nnzb1, lv1["n"], b1, 4), cheb_bytes(nnzb1, lv1["n"], b1, 8)
         log(f"[bench] level-1 Chebyshev step fp32 {t_ch32 * 1e6:.1f} us {ch32 / t_ch32 / 1e9:.0f} GB/s | "
             f"fp64 {t_ch64 * 1e6:.1f} us {ch64 / t_ch64 / 1e9:.0f} GB/s")
+    # the practical streaming-read ceiling of this device (plain read probe)
+    ceiling = stream_ceiling()
+    log(f"[bench] HBM streaming-read probe {ceiling['gbs']} GB/s (grid {ceiling['grid']}; {ceiling['per_grid_gbs']})")
     # SpMV bandwidth of the FP32 and FP64 3x3 kernel (the metric's SpMV figure)
     nnzb, nb = info["nnzb"], system.n_nodes
     t32 = time_spmv(system.K, "fp32", args.spmv_reps)
@@ -1168,6 +1207,8 @@ def main() -> int:
                       "traffic_source": "committed rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE summary "
                                         "(profiles/amg_level1_cheb_ibeam_1.05M_hbm_traffic.md), not measured in this run",
                       "algorithmic_bytes": ch32, "avg_launch_us": round(t_ch32 * 1e6, 2),
+                      "stream_ceiling_gbs": ceiling["gbs"],
+                      "frac_of_stream_ceiling": round(ch32 / t_ch32 / 1e9 / ceiling["gbs"], 4),
                       "launches_per_iteration": 3,
                       "kernel": f"sell_spmv_kernel<6,6,float,float,"
                                 f"{cycle_tune['fp32']['waves'] if cycle_tune else tune['waves']},"
@@ -1177,6 +1218,8 @@ def main() -> int:
                                     "frac": round(ch64 / t_ch64 / 1e9 / HBM_PEAK_GBS, 4)}} if has_l1 else
                      {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": b32,
+                      "stream_ceiling_gbs": ceiling["gbs"],
+                      "frac_of_stream_ceiling": round(gbs32 / ceiling["gbs"], 4),
                       "avg_launch_us": round(t32 * 1e6, 2),
                       "kernel": "FP32 3x3 SpMV of the fine K (the hierarchy has no smoothed level between the "
                                 "finest and the dense coarsest one)"}),

@@ -33,6 +33,14 @@ int wnfea_abi_version(void);
  * Replaces hip_get_device_summary (wnfea/solver/native/hip_kernels.cpp:1541). */
 int wnfea_device_info(long long *out);
 
+/* HBM streaming-read probe: a kernel of 512-thread blocks that reads the
+ * buffer a (device, bytes a multiple of 16, 16-byte aligned) once with
+ * 16-byte non-temporal loads, eight in flight per thread, and writes each
+ * thread's sum of the floats it read to out[grid * 512] (device).  The rate
+ * it reaches is the practical ceiling of the bandwidth-bound kernels, next
+ * to the datasheet peak.  No reference counterpart (a measurement tool). */
+int wnfea_hbm_stream_read(const float *a, long long bytes, float *out, int grid, void *stream);
+
 /* ---------------------------------------------------------------- elements */
 
 /* Batched C3D10 element stiffness, FP64.  conn[M][10] int32 (device),

@@ -153,3 +153,26 @@ def test_spmv_rect_blocks(br, bc):
     back = m.get_values().cpu().numpy()
     assert np.array_equal(back, vals)
     m.close()
+
+
+def test_hbm_stream_read_probe_reads_every_value_once():
+    """The streaming-read probe reads the whole buffer exactly once: with
+    small integer values every per-thread sum is exact in FP32, so the sums
+    over all threads add up to the buffer's sum exactly (one grid below and
+    one above the buffer's 16-byte chunk count); bad sizes and alignment are
+    refused on the host."""
+    import torch
+    from wnfea_amd import _lib
+    L = _lib.load()
+    n = (1 << 20) + 12  # a multiple of four floats, not of the unrolled round
+    a = torch.randint(0, 4, (n,), device="cuda", dtype=torch.int32).float()
+    ref = float(a.double().sum())
+    for grid in (37, 1024):
+        out = torch.empty(grid * 512, dtype=torch.float32, device="cuda")
+        _lib.check(L.wnfea_hbm_stream_read(_lib.dptr(a), 4 * n, _lib.dptr(out), grid, _lib.stream_of()),
+                   "wnfea_hbm_stream_read")
+        torch.cuda.synchronize()
+        assert float(out.double().sum()) == ref, grid
+    out = torch.empty(512, dtype=torch.float32, device="cuda")
+    assert L.wnfea_hbm_stream_read(_lib.dptr(a), 4 * n - 4, _lib.dptr(out), 1, _lib.stream_of()) != 0
+    assert L.wnfea_hbm_stream_read(_lib.dptr(a) + 4, 4 * n - 16, _lib.dptr(out), 1, _lib.stream_of()) != 0

@@ -27,6 +27,7 @@ _pll = ctypes.POINTER(ctypes.c_longlong)
 SIGNATURES = {
     "wnfea_abi_version": (_i, []),
     "wnfea_device_info": (_i, [_pll]),
+    "wnfea_hbm_stream_read": (_i, [_p, _ll, _p, _i, _p]),
     "wnfea_c3d10_ke": (_i, [_i, _p, _p, _p, _p, _p, _p, _p]),
     "wnfea_c3d10_ke_variant": (_i, [_i]),
     "wnfea_c3d10_fint": (_i, [_i, _i, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p]),

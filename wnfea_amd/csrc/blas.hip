@@ -360,6 +360,39 @@ int launch_normalise(long long n, const T* x, T* y, const double* nrm2, hipStrea
   return static_cast<int>(hipGetLastError());
 }
 
+// HBM streaming-read probe: every thread sums the floats of eight 16-byte
+// non-temporal loads per round (grid-stride over the buffer) and stores its
+// sum.  It measures the read rate a plain stream reaches on this device, the
+// practical ceiling for the bandwidth-bound kernels; out[t] summed over all
+// threads is the sum of the buffer.
+constexpr int kStreamBlock = 512, kStreamU = 8;
+__global__ __launch_bounds__(kStreamBlock) void stream_read_kernel(const float* __restrict__ a, long long n4,
+                                                                   float* __restrict__ out) {
+  using F4 = float __attribute__((ext_vector_type(4)));
+  const F4* __restrict__ v = reinterpret_cast<const F4*>(a);
+  const long long stride = (long long)gridDim.x * kStreamBlock;
+  long long i = blockIdx.x * (long long)kStreamBlock + threadIdx.x;
+  float s = 0.f;
+  for (; i + (kStreamU - 1) * stride < n4; i += kStreamU * stride) {
+    F4 w[kStreamU];
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u) w[u] = __builtin_nontemporal_load(v + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u) s += (w[u].x + w[u].y) + (w[u].z + w[u].w);
+  }
+  for (; i < n4; i += stride) {
+    const F4 w = __builtin_nontemporal_load(v + i);
+    s += (w.x + w.y) + (w.z + w.w);
+  }
+  out[blockIdx.x * (long long)kStreamBlock + threadIdx.x] = s;
+}
+
+int launch_stream_read(const float* a, long long bytes, float* out, int grid, hipStream_t s) {
+  if (!a || !out || bytes < 0 || bytes % 16 || grid <= 0 || (reinterpret_cast<uintptr_t>(a) & 15)) return kBadArg;
+  hipLaunchKernelGGL(stream_read_kernel, dim3(grid), dim3(kStreamBlock), 0, s, a, bytes / 16, out);
+  return static_cast<int>(hipGetLastError());
+}
+
 #define WN_INST(T)                                                                                   \
   template int launch_jacobi_dot<T>(long long, const T*, const T*, T*, Workspace&, int, const int*, hipStream_t, \
                                     bool);                                                           \

@@ -81,6 +81,11 @@ extern "C" {
 
 int wnfea_abi_version(void) { return 1; }
 
+// HBM streaming-read probe: out[grid * 512] per-thread sums of a[bytes / 4]
+int wnfea_hbm_stream_read(const float* a, long long bytes, float* out, int grid, void* stream) {
+  return launch_stream_read(a, bytes, out, grid, S(stream));
+}
+
 // out[0] CUs, out[1] wavefront size, out[2] gfx arch number (e.g. 950),
 // out[3] L2 bytes, out[4] LDS bytes per block, out[5] runtime version.
 int wnfea_device_info(long long* out) {

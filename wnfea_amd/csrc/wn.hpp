@@ -243,6 +243,9 @@ template <typename T> int launch_dot(long long n, const T* a, const T* b, Worksp
 template <typename T>
 int launch_ir_update(long long n, double* u, const T* d, const double* sc_vec, const double* nrm2, hipStream_t s);
 template <typename T> int launch_normalise(long long n, const T* x, T* y, const double* nrm2, hipStream_t s);
+// HBM streaming-read probe (blas.hip): a[bytes / 4] floats, 16-byte aligned,
+// bytes a multiple of 16; out[grid * 512] per-thread sums.
+int launch_stream_read(const float* a, long long bytes, float* out, int grid, hipStream_t s);
 
 // -------- sliced-ELL kernels (sell.hip)
 // y = alpha A x + beta y
