@@ -114,6 +114,15 @@ __device__ __forceinline__ double dpp_row16_sum(double v) {
   return v;
 }
 
+// x[i] for i < n, else 0 (the tile's padding rows and columns): the load
+// reads an in-bounds element either way, so that no branch around it makes
+// the wavefront wait for every load in flight before the next one (n >= 1)
+template <typename XT>
+__device__ __forceinline__ double x_or_zero(const XT* __restrict__ x, int i, int n) {
+  const double v = double(x[i < n ? i : n - 1]);
+  return i < n ? v : 0.0;
+}
+
 // Thread t of the block holds V = 16 B / sizeof(T) consecutive columns
 // col0 = (t % CPR) * V of NI rows t / CPR + RPI * m: every load instruction
 // of the block covers 4 KiB of the tile contiguously.  T: tile values, XT:
@@ -144,7 +153,7 @@ __global__ __launch_bounds__(256) void sym_tile_gemv_kernel(int n, const T* __re
 #pragma unroll
   for (int v = 0; v < V; ++v) {
     const int j = ij.y * kTile + c0 + v;
-    xj[v] = j < n ? double(x[j]) : 0.0;
+    xj[v] = x_or_zero(x, j, n);
   }
   double rowp[NI], colp[V];
 #pragma unroll
@@ -152,7 +161,7 @@ __global__ __launch_bounds__(256) void sym_tile_gemv_kernel(int n, const T* __re
 #pragma unroll
   for (int m = 0; m < NI; ++m) {
     const int i = ij.x * kTile + r0 + RPI * m;
-    const double xi = i < n ? double(x[i]) : 0.0;
+    const double xi = x_or_zero(x, i, n);
     double s = 0.0;
 #pragma unroll
     for (int v = 0; v < V; ++v) {
