@@ -456,7 +456,7 @@ def run_batch(args, log) -> int:
     models = [b[0] for b in batch]
     n_dofs = sum(b[1]["n_dofs"] for b in batch)
     log(f"[bench] batch: 64 boxes, {n_dofs} DOF total ({time.perf_counter() - t0:.1f}s)")
-    bs = BatchedSolids(models, amg_kw={"ratio": args.amg_ratio})
+    bs = BatchedSolids(models, amg_kw={"ratio": args.amg_ratio, **omega_kw(args)})
     log(f"[bench] batched setup {bs.timings}")
     for _ in range(args.warmup):
         bs.solve(1e-8)
@@ -527,7 +527,7 @@ def run_adjoint(args, log) -> int:
     tip = o2n[meta["tip_nodes"]]
     c = tip_vector(model, tip)
     t0 = time.perf_counter()
-    prob = C3D10Adjoint(model, tol=1e-10, amg_precision="fp32")
+    prob = C3D10Adjoint(model, tol=1e-10, amg_precision="fp32", amg_kw=omega_kw(args))
     prob.set_moduli(E)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
@@ -589,6 +589,12 @@ def run_adjoint(args, log) -> int:
     return 0
 
 
+def omega_kw(args) -> dict:
+    """The SA prolongator smoothing factor when --amg-omega is given (else
+    the solver's default for the problem type)."""
+    return {} if args.amg_omega is None else {"omega": args.amg_omega}
+
+
 def tune_spmv(K, shape: str | None, log) -> dict:
     """SpMV launch shapes for the FP32 and the FP64 values of K: fixed by
     `shape` ("waves,unroll,xcd_remap" for both, or "fp32 shape:fp64 shape"),
@@ -636,7 +642,8 @@ def run_multirhs(args, log) -> int:
     amg = build_amg(system, storage=("fp32",), waves=tune["waves"], unroll=tune["unroll"],
                     xcd_remap=tune["xcd_remap"],
                     theta=args.amg_theta if args.amg_theta is not None else 0.1,
-                    theta_decay=args.amg_theta_decay if args.amg_theta_decay is not None else 0.4)
+                    theta_decay=args.amg_theta_decay if args.amg_theta_decay is not None else 0.4,
+                    **omega_kw(args))
     apply_tune(amg, tunes)
     B = ibeam_tip_load_cases(model.mesh_nodes, o2n[meta["tip_nodes"]], n_cases=32, seed=0)
     B[system.fixed_mask] = 0.0
@@ -723,6 +730,7 @@ def run_beams(args, log) -> int:
         amg_kw["theta"] = args.amg_theta
     if args.amg_theta_decay is not None:
         amg_kw["theta_decay"] = args.amg_theta_decay
+    amg_kw.update(omega_kw(args))
     bs.build_amg(**amg_kw)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
@@ -808,6 +816,8 @@ def main() -> int:
                     help="SA strength threshold on the finest level (I-beam default 0.1, box 0.08, beam lattice 0.02)")
     ap.add_argument("--amg-ratio", type=float, default=30.0,
                     help="Chebyshev smoother interval [lmax/ratio, lmax] of the I-beam, box and batch hierarchies")
+    ap.add_argument("--amg-omega", type=float, default=None,
+                    help="SA prolongator smoothing factor (default: 1.6 for the solids, 4/3 for the beam lattice)")
     ap.add_argument("--amg-theta-decay", type=float, default=None,
                     help="factor on the threshold per coarser level (I-beam default 0.4, box 1.0)")
     ap.add_argument("--smoother-operator", choices=["matrix_free", "assembled"], default="matrix_free",
@@ -841,8 +851,8 @@ def main() -> int:
     from wnfea_amd.mesh.rcm import apply_rcm_to_model
     from wnfea_amd.mesh.solid_generator import generate_c3d10_ibeam, generate_c3d10_structured_block
     from wnfea_amd.solver.autotune import autotune_spmv
-    from wnfea_amd.solver.solid_solver import (assemble_c3d10_bsr, build_amg, ir_device, pcg_device, set_graphs,
-                                               solve_mixed, true_residual)
+    from wnfea_amd.solver.solid_solver import (SOLID_SA_OMEGA, assemble_c3d10_bsr, build_amg, ir_device, pcg_device,
+                                               set_graphs, solve_mixed, true_residual)
 
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     torch.cuda.set_device(0)
@@ -895,7 +905,7 @@ def main() -> int:
     th = args.amg_theta if args.amg_theta is not None else (0.1 if args.config == "ibeam" else 0.08)
     thd = args.amg_theta_decay if args.amg_theta_decay is not None else (0.4 if args.config == "ibeam" else 1.0)
     amg = build_amg(system, waves=tune["waves"], unroll=tune["unroll"], xcd_remap=tune["xcd_remap"],
-                    theta=th, theta_decay=thd, ratio=args.amg_ratio)
+                    theta=th, theta_decay=thd, ratio=args.amg_ratio, **omega_kw(args))
     apply_tune(amg, tunes)
     t_amg = time.perf_counter() - t0
     st = amg.stats()
@@ -1177,7 +1187,9 @@ def main() -> int:
         "config": {"workload": workload, "n_dofs": meta["n_dofs"], "n_elements": meta["n_elements"],
                    "nnzb": nnzb, "tol": 1e-8, "solver": "FP64 CG + low-precision SA-AMG V-cycle",
                    "cg_operator": args.operator, "smoother_operator": args.smoother_operator,
-                   "amg_theta": th, "amg_theta_decay": thd, "parallelism": "single-gpu"},
+                   "amg_theta": th, "amg_theta_decay": thd,
+                   "amg_omega": args.amg_omega if args.amg_omega is not None else SOLID_SA_OMEGA,
+                   "parallelism": "single-gpu"},
         "dof_per_s": round(meta["n_dofs"] / t_mixed, 1),
         "rel_res_fp64_verified": rel_true,
         "iterations": iters,

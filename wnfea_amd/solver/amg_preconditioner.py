@@ -11,7 +11,8 @@ the reference it is a *smoothed* aggregation AMG:
 * strength of connection ||A_ij||_F >= theta sqrt(||A_ii||_F ||A_jj||_F) on
   block norms, three-phase aggregation (native, ``wnfea_sa_aggregate``);
 * the tentative prolongator is smoothed, P = (I - omega/lambda D^-1 A) P_tent
-  with omega = 4/3 and lambda the power-iteration estimate of rho(D^-1 A);
+  with omega = 4/3 by default (the solid solver's hierarchies take 1.6) and
+  lambda the power-iteration estimate of rho(D^-1 A);
 * the coarse near-null-space is the R factor of each aggregate's QR;
 * Chebyshev smoothing on [lambda/ratio, lambda] replaces damped Jacobi.
 
@@ -292,7 +293,7 @@ def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None
 
 def _coarse_inverse(Acd: np.ndarray) -> np.ndarray:
     """Symmetric inverse of the coarsest operator by Cholesky, in FP64 on the
-    GPU when one is visible (rocSOLVER through torch.linalg: the 9 978-row
+    GPU when one is visible (rocSOLVER through torch.linalg: the 9 762-row
     I-beam level factorises in a fraction of a second there), else on the
     host (scipy).  A matrix that is only semidefinite in floating point gets
     the reference's relative 1e-12 diagonal shift (amg_preconditioner.py:314)
@@ -351,10 +352,10 @@ class SmoothedAggregationAMG:
                  degree: int = 2, ratio: float = 30.0, coarse_size: int = 12000, max_levels: int = 10,
                  coarse_degree: int | None = None, level_degrees: list | None = None,
                  storage: tuple = ("fp64", "fp32", "fp16"), sigma: int = 4096, waves: int = 1,
-                 xcd_remap: bool = True, unroll: int = 1):
+                 xcd_remap: bool = True, unroll: int = 1, omega: float = 4.0 / 3.0):
         t0 = time.perf_counter()
         self.lib = _lib.load()
-        self.levels, inv = sa_setup(A_hat, coords, fixed_mask, theta=theta, theta_decay=theta_decay,
+        self.levels, inv = sa_setup(A_hat, coords, fixed_mask, theta=theta, theta_decay=theta_decay, omega=omega,
                                     coarse_size=coarse_size, max_levels=max_levels, dof_scale=dof_scale)
         self.setup_host_sec = time.perf_counter() - t0
         self.degree, self.ratio = degree, ratio

@@ -166,13 +166,15 @@ class BeamSystem:
         theta 0.01-0.03, 3 460 at 0.08, 8 152 at 0.15), and the Chebyshev
         smoother's interval [lmax/ratio, lmax] defaults to ratio 20 (2 931
         iterations, 5 % faster than the solids' 30; 10-20 are alike, 60 takes
-        3 652).  Unless a launch shape is given, the cycle's SpMV launch shape
-        is then tuned on whole cycles."""
+        3 652), and the prolongator smoothing factor to the textbook 4/3 (the
+        solids' 1.6 takes 3 460 iterations).  Unless a launch shape is given,
+        the cycle's SpMV launch shape is then tuned on whole cycles."""
         from .autotune import autotune_cycle
         from .solid_solver import build_amg
         self.close()
         kw.setdefault("theta", 0.02)
         kw.setdefault("ratio", 20.0)
+        kw.setdefault("omega", 4.0 / 3.0)
         self.amg = build_amg(self.system, storage=tuple(storage), **kw)
         if self.amg.autotuned:
             self.cycle_tune = autotune_cycle(self.amg, "fp32" if "fp32" in storage else "fp64")
