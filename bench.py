@@ -527,6 +527,9 @@ def run_adjoint(args, log) -> int:
     tip = o2n[meta["tip_nodes"]]
     c = tip_vector(model, tip)
     t0 = time.perf_counter()
+    # the library's strength threshold: with per-element moduli spread over
+    # a factor of five the I-beam's thresholds (0.1, decay 0.5) are slower
+    # here (27.7 / 52.9 ms against 26.8 / 51.0 ms)
     prob = C3D10Adjoint(model, tol=1e-10, amg_precision="fp32", amg_kw=omega_kw(args))
     prob.set_moduli(E)
     torch.cuda.synchronize()
@@ -589,6 +592,14 @@ def run_adjoint(args, log) -> int:
     return 0
 
 
+# SA strength threshold on the I-beam: 0.1 on the finest level times 0.5 per
+# coarser level (sweep with the prolongator smoothing factor 1.6: 40
+# iterations; 0.3 / 0.4 / 0.45 / 0.55 / 0.6 take 44 / 41 / 41 / 40 / 39, the
+# last one with a coarsest level of 11 694 rows, close to the 12 000 rows
+# above which a fourth level appears and the iterations rise to 67)
+IBEAM_THETA_DECAY = 0.5
+
+
 def omega_kw(args) -> dict:
     """The SA prolongator smoothing factor when --amg-omega is given (else
     the solver's default for the problem type)."""
@@ -642,7 +653,7 @@ def run_multirhs(args, log) -> int:
     amg = build_amg(system, storage=("fp32",), waves=tune["waves"], unroll=tune["unroll"],
                     xcd_remap=tune["xcd_remap"],
                     theta=args.amg_theta if args.amg_theta is not None else 0.1,
-                    theta_decay=args.amg_theta_decay if args.amg_theta_decay is not None else 0.4,
+                    theta_decay=args.amg_theta_decay if args.amg_theta_decay is not None else IBEAM_THETA_DECAY,
                     **omega_kw(args))
     apply_tune(amg, tunes)
     B = ibeam_tip_load_cases(model.mesh_nodes, o2n[meta["tip_nodes"]], n_cases=32, seed=0)
@@ -819,7 +830,7 @@ def main() -> int:
     ap.add_argument("--amg-omega", type=float, default=None,
                     help="SA prolongator smoothing factor (default: 1.6 for the solids, 4/3 for the beam lattice)")
     ap.add_argument("--amg-theta-decay", type=float, default=None,
-                    help="factor on the threshold per coarser level (I-beam default 0.4, box 1.0)")
+                    help="factor on the threshold per coarser level (I-beam default 0.5, box 1.0)")
     ap.add_argument("--smoother-operator", choices=["matrix_free", "assembled"], default="matrix_free",
                     help="operator of the finest level's smoothing steps and defect in the V-cycle")
     ap.add_argument("--spmv-shape", default=None,
@@ -903,7 +914,8 @@ def main() -> int:
     # strength thresholds per workload (sweep on the I-beam: 0.1 on the finest
     # level and 0.04 below it take 44 instead of 50 iterations)
     th = args.amg_theta if args.amg_theta is not None else (0.1 if args.config == "ibeam" else 0.08)
-    thd = args.amg_theta_decay if args.amg_theta_decay is not None else (0.4 if args.config == "ibeam" else 1.0)
+    thd = args.amg_theta_decay if args.amg_theta_decay is not None else (IBEAM_THETA_DECAY if args.config == "ibeam"
+                                                                         else 1.0)
     amg = build_amg(system, waves=tune["waves"], unroll=tune["unroll"], xcd_remap=tune["xcd_remap"],
                     theta=th, theta_decay=thd, ratio=args.amg_ratio, **omega_kw(args))
     apply_tune(amg, tunes)

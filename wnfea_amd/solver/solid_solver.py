@@ -635,7 +635,7 @@ def build_amg(system: SolidSystem, storage: tuple = ("fp64", "fp32", "fp16"), au
     threshold (``theta`` on the finest level, times ``theta_decay`` per level)
     is mesh-dependent: 0.08 on every level suits the cantilever boxes, the
     1.05M-DOF I-beam converges in 44 instead of 50 iterations with 0.1 and
-    decay 0.4.  The prolongator smoothing P = (I - omega/lambda D^-1 A) P_tent
+    decay 0.4 (40 with decay 0.5 and omega 1.6).  The prolongator smoothing P = (I - omega/lambda D^-1 A) P_tent
     takes ``omega`` = 1.6 for solids: on the I-beam 41 iterations against 44
     at the textbook 4/3 (43 at 1.4-1.5 and 1.8, 48 at 1.9), the 64-box batch
     33 against 36, the I-beam's 32 tip load cases 38-48 against 41-51."""
