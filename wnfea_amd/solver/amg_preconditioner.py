@@ -267,9 +267,11 @@ def sa_setup(A: sp.bsr_matrix, coords: np.ndarray, fixed_mask: np.ndarray | None
         _lib.check(L.wnfea_sa_tentative(n, b, k, _lib.hptr(agg), na, _lib.hptr(Bc_in), _lib.hptr(p_rowptr),
                                         _lib.hptr(p_colidx), _lib.hptr(p_vals), _lib.hptr(Bc)), "wnfea_sa_tentative")
         P0 = sp.bsr_matrix((p_vals[:n_blk].reshape(-1, b, k), p_colidx[:n_blk], p_rowptr), shape=(n * b, na * k))
+        # omega: one value, or one per coarsening step (the last repeated)
+        om = (omega[min(len(levels), len(omega) - 1)] if isinstance(omega, (list, tuple)) else omega)
         # P = P0 - (omega / lmax) D^-1 A P0;  Ac = R A P with R = P^T, on host
         # threads (wnfea_host_bsr_*), symmetrised
-        P = _smooth_prolongator(A, P0, dinv, omega / lmax)
+        P = _smooth_prolongator(A, P0, dinv, om / lmax)
         R = _transpose(P)
         Ac = _spgemm(R, _spgemm(A, P))
         # An aggregate that cannot carry all k near-null-space modes (one or
