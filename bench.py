@@ -456,7 +456,12 @@ def run_batch(args, log) -> int:
     models = [b[0] for b in batch]
     n_dofs = sum(b[1]["n_dofs"] for b in batch)
     log(f"[bench] batch: 64 boxes, {n_dofs} DOF total ({time.perf_counter() - t0:.1f}s)")
-    bs = BatchedSolids(models, amg_kw={"ratio": args.amg_ratio, **omega_kw(args)})
+    kw = {"ratio": args.amg_ratio, **omega_kw(args)}
+    if args.amg_theta is not None:
+        kw["theta"] = args.amg_theta
+    if args.amg_theta_decay is not None:
+        kw["theta_decay"] = args.amg_theta_decay
+    bs = BatchedSolids(models, amg_kw=kw)
     log(f"[bench] batched setup {bs.timings}")
     for _ in range(args.warmup):
         bs.solve(1e-8)
