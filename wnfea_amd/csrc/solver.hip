@@ -880,6 +880,8 @@ int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const i
 }
 
 int amg_set_coarse(Amg& g, int n, const double* inv) {
+  // the tile product reads x[n - 1] for its padding lanes: n >= 1
+  if (n < 1 || !inv) return kBadArg;
   g.release_coarse();
   const size_t nn = (size_t)n * n;
   WN_HIP(hipMalloc(&g.inv, sizeof(double) * nn));
