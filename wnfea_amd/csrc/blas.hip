@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kVecBlock) void update_xr_kernel(long long n, T* __
       const RT b = RT(scale ? scale[i] * double(ri) : double(ri));
       rp[i] = b;
       if (fx1) {  // the next cycle's first iterate, as the first-step kernel forms it
-        const RT v = fc2 * fdinv[i] * b;
+        const RT v = fc2 * (fdinv ? fdinv[i] : RT(1)) * b;
         fd1[i] = v;
         fx1[i] = v;
       }
@@ -337,7 +337,7 @@ int launch_update_xr(long long n, T* x, T* r, const T* p, const T* q, RT* rp, co
   double* rr_part = gam_part ? w.partials + 2 * (size_t)w.max_blocks : w.partials;
   const bool f = fi && rp && fi->x1;
   hipLaunchKernelGGL((update_xr_kernel<T, RT>), dim3(g), dim3(kVecBlock), 0, s, n, x, r, p, q, rp, scale, rr_part,
-                     w.sc, flags, gam_part, nb_gam, f ? fi->dinv : nullptr, f ? fi->x1 : nullptr,
+                     w.sc, flags, gam_part, nb_gam, f && !fi->unit ? fi->dinv : nullptr, f ? fi->x1 : nullptr,
                      f ? fi->d1 : nullptr, f ? fi->c2 : RT(0));
   int e = static_cast<int>(hipGetLastError());
   return e ? e : launch_finalize(rr_part, g, 1, kFinRR, nullptr, w.sc, flags, 0, flags, s);

@@ -1731,7 +1731,11 @@ static int mf_smooth_run(const MfSmooth& m, const T* x, const Op& o, const int* 
 template <typename T>
 int mf_smooth_cheb(const MfSmooth& m, const T* x_in, T* x_out, const T* b, const T* dinv, T* d, T c1, T c2,
                    const int* skip, hipStream_t s) {
-  OpCheb<3, T> o{b, dinv, x_in, x_out, d, c1, c2};
+  if (!dinv) {
+    const OpCheb<3, T, true> o{b, nullptr, x_in, x_out, d, c1, c2};
+    return mf_smooth_run<T>(m, x_in, o, skip, s);
+  }
+  const OpCheb<3, T> o{b, dinv, x_in, x_out, d, c1, c2};
   return mf_smooth_run<T>(m, x_in, o, skip, s);
 }
 
@@ -1745,15 +1749,15 @@ int mf_smooth_defect(const MfSmooth& m, const T* x, const T* b, T* res, const in
 // rho = r . (S z), z = the step's x_out as stored: the V-cycle's last step
 // hands the CG iteration its rho without a pass over r, S and z of its own.
 // Per DOF the same product as the separate rho kernel, r (S z).
-template <typename T>
+template <typename T, bool Unit = false>
 struct OpChebRho {
   static constexpr int kDots = 1;
-  OpCheb<3, T> c;
+  OpCheb<3, T, Unit> c;
   const double* r;
   const double* scale;
   double* partials;
   struct Pre {
-    typename OpCheb<3, T>::Pre c;
+    typename OpCheb<3, T, Unit>::Pre c;
     double r, s;
   };
   __device__ __forceinline__ void load(long long k, Pre& p) const {
@@ -1782,10 +1786,17 @@ int mf_smooth_cheb_rho(const MfSmooth& m, const T* x_in, T* x_out, const T* b, c
   if (!qdp || !rq.r || !rq.partials) return kBadMode;
   const MfOp& op = *m.op;
   T* part = mf_part<T>(m);
-  const OpChebRho<T> o{{b, dinv, x_in, x_out, d, c1, c2}, rq.r, rq.scale, rq.partials};
   launch_patch<T, std::is_same<T, float>::value>(op, qdp, mf_scale_in<T>(m), x_in, part, skip, s);
-  hipLaunchKernelGGL((mf_node_kernel<T, OpChebRho<T>>), dim3(node_grid(op.n_nodes)), dim3(kNodeThreads), 0, s,
-                     op.n_nodes, op.nhead, op.novf, part, op.fix, mf_scale<T>(m), x_in, o, skip);
+  const dim3 grid(node_grid(op.n_nodes));
+  if (!dinv) {
+    const OpChebRho<T, true> o{{b, nullptr, x_in, x_out, d, c1, c2}, rq.r, rq.scale, rq.partials};
+    hipLaunchKernelGGL((mf_node_kernel<T, OpChebRho<T, true>>), grid, dim3(kNodeThreads), 0, s, op.n_nodes,
+                       op.nhead, op.novf, part, op.fix, mf_scale<T>(m), x_in, o, skip);
+  } else {
+    const OpChebRho<T> o{{b, dinv, x_in, x_out, d, c1, c2}, rq.r, rq.scale, rq.partials};
+    hipLaunchKernelGGL((mf_node_kernel<T, OpChebRho<T>>), grid, dim3(kNodeThreads), 0, s, op.n_nodes, op.nhead,
+                       op.novf, part, op.fix, mf_scale<T>(m), x_in, o, skip);
+  }
   return static_cast<int>(hipGetLastError());
 }
 

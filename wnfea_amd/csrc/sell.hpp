@@ -376,7 +376,8 @@ struct OpResidual {
 
 // One Chebyshev smoothing step on D^-1 A (D^-1 given as `dinv`):
 //   res = b - A x_in;  d = c1 * d + c2 * dinv .* res;  x_out = x_in + d
-template <int BR, typename T>
+// Unit: D = I (an equilibrated operator's unit diagonal); dinv is not read.
+template <int BR, typename T, bool Unit = false>
 struct OpCheb {
   static constexpr int kDots = 0;
   const T* b;
@@ -389,7 +390,11 @@ struct OpCheb {
   struct Pre { T b, dinv, x, d; };
   __device__ __forceinline__ void load(long long k, Pre& p) const {
     p.b = b[k];
-    p.dinv = dinv[k];
+    if constexpr (Unit) {
+      p.dinv = T(1);
+    } else {
+      p.dinv = dinv[k];
+    }
     p.x = x_in[k];
     p.d = d[k];
   }
@@ -404,7 +409,7 @@ struct OpCheb {
       if ((r & (L - 1)) != q) continue;
       const long long k = (long long)i * BR + r;
       const T res = b[k] - acc[r];
-      const T dn = c1 * d[k] + c2 * dinv[k] * res;
+      const T dn = c1 * d[k] + c2 * (Unit ? T(1) : dinv[k]) * res;
       d[k] = dn;
       x_out[k] = x_in[k] + dn;
     }

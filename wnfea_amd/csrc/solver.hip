@@ -301,9 +301,11 @@ static int vcycle(Amg& g, int l, const T* b, T* x, const int* skip, hipStream_t 
   // rq: the last post-smoothing step (matrix-free, patch layout) also writes
   // the partials of rho = r . (S x) for the Krylov loop
   if (rq && !mf) return kBadMode;
+  // a unit inverse diagonal is passed to the matrix-free steps as null
+  const T* dinv_mf = level_unit<T>(L) ? nullptr : dinv;
   auto cheb = [&](const T* xi, T* xo, int k, const RhoReq* q = nullptr) -> int {
-    if (mf && q) return mf_smooth_cheb_rho<T>(g.mfs, xi, xo, b, dinv, d, T(c.c1[k]), T(c.c2[k]), *q, skip, s);
-    if (mf) return mf_smooth_cheb<T>(g.mfs, xi, xo, b, dinv, d, T(c.c1[k]), T(c.c2[k]), skip, s);
+    if (mf && q) return mf_smooth_cheb_rho<T>(g.mfs, xi, xo, b, dinv_mf, d, T(c.c1[k]), T(c.c2[k]), *q, skip, s);
+    if (mf) return mf_smooth_cheb<T>(g.mfs, xi, xo, b, dinv_mf, d, T(c.c1[k]), T(c.c2[k]), skip, s);
     return spmv_cheb<VS, T>(A, xi, xo, b, dinv, d, c.c1[k], c.c2[k], t, s);
   };
   if (!first_done) {
@@ -353,6 +355,7 @@ bool amg_first_iter(Amg& g, int vt, T* z, FirstIter<T>& f) {
   const int deg = level_degree(L);
   const ChebCoef c = cheb_coefs(L.lmax, L.ratio, deg);
   f.dinv = dinv;
+  f.unit = level_unit<T>(L);
   f.x1 = first_iterate(deg, z, v[2]);
   f.d1 = v[3];
   f.c2 = T(c.c2[0]);
@@ -853,7 +856,12 @@ int amg_set_level(Amg& g, int lvl, SellMatrix* fine, int br, int n_rows, const i
   if (!L.dinv64) WN_HIP(hipMalloc(&L.dinv64, sizeof(double) * std::max(L.n, 1)));
   if (!L.dinv32) WN_HIP(hipMalloc(&L.dinv32, sizeof(float) * std::max(L.n, 1)));
   std::vector<float> d32(L.n);
-  for (int i = 0; i < L.n; ++i) d32[i] = static_cast<float>(dinv[i]);
+  L.unit64 = L.unit32 = true;
+  for (int i = 0; i < L.n; ++i) {
+    d32[i] = static_cast<float>(dinv[i]);
+    L.unit64 = L.unit64 && dinv[i] == 1.0;
+    L.unit32 = L.unit32 && d32[i] == 1.0f;
+  }
   WN_HIP(hipMemcpy(L.dinv64, dinv, sizeof(double) * L.n, hipMemcpyHostToDevice));
   WN_HIP(hipMemcpy(L.dinv32, d32.data(), sizeof(float) * L.n, hipMemcpyHostToDevice));
   L.has_PR = false;

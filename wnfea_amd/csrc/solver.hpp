@@ -17,6 +17,9 @@ struct AmgLevel {
   int n = 0;  // scalar rows
   double* dinv64 = nullptr;
   float* dinv32 = nullptr;
+  // the inverse diagonal is exactly 1 in that precision (an equilibrated
+  // level): matrix-free smoothing steps skip its load
+  bool unit64 = false, unit32 = false;
   double lmax = 2.0, ratio = 30.0;
   int degree = 2;
   // level vectors (b, x, tmp, d, res) per arithmetic precision
@@ -82,6 +85,9 @@ template <> inline float** level_vecs<float>(AmgLevel& l) { return l.v32; }
 template <typename T> T* level_dinv(AmgLevel& l);
 template <> inline double* level_dinv<double>(AmgLevel& l) { return l.dinv64; }
 template <> inline float* level_dinv<float>(AmgLevel& l) { return l.dinv32; }
+template <typename T> inline bool level_unit(const AmgLevel& l);
+template <> inline bool level_unit<double>(const AmgLevel& l) { return l.unit64; }
+template <> inline bool level_unit<float>(const AmgLevel& l) { return l.unit32; }
 
 struct ChebCoef {
   double c1[16], c2[16];

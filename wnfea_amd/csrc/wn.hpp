@@ -143,7 +143,8 @@ void mf_smooth_release(MfSmooth& m);
 template <typename T> bool mf_smooth_ready(const MfSmooth& m);
 template <> bool mf_smooth_ready<double>(const MfSmooth& m);
 template <> bool mf_smooth_ready<float>(const MfSmooth& m);
-// x_out = x_in + d',  d' = c1 d + c2 dinv (b - A x_in)
+// x_out = x_in + d',  d' = c1 d + c2 dinv (b - A x_in); dinv null: D = I
+// (the equilibrated finest level, whose inverse diagonal is 1 everywhere)
 template <typename T>
 int mf_smooth_cheb(const MfSmooth& m, const T* x_in, T* x_out, const T* b, const T* dinv, T* d, T c1, T c2,
                    const int* skip, hipStream_t s);
@@ -227,6 +228,7 @@ int launch_update_p(long long n, const ZT* z, const double* scale, T* p, Workspa
 template <typename T>
 struct FirstIter {
   const T* dinv = nullptr;
+  bool unit = false;  // dinv is 1 everywhere: the x/r update does not read it
   T* x1 = nullptr;
   T* d1 = nullptr;
   T c2 = T(0);
