@@ -403,3 +403,19 @@ def test_structured_block_conformity(tets_per_hex):
     boundary_area = 2 * (L * W + L * H + W * H)
     area = 0.5 * np.linalg.norm(np.cross(c[:, 1] - c[:, 0], c[:, 2] - c[:, 0]), axis=1).sum()
     assert abs(area - boundary_area) < 1e-12
+
+
+def test_coarse_inverse_rejects_an_empty_level():
+    """The coarsest-level set-up refuses n < 1 (the packed tile product reads
+    the vector's last entry for its padding lanes) before touching the GPU."""
+    import ctypes
+    from wnfea_amd import _lib
+    L = _lib.load()
+    h = L.wnfea_amg_create(2, 3)
+    assert h
+    try:
+        inv = (ctypes.c_double * 1)(1.0)
+        assert L.wnfea_amg_set_coarse(h, 0, inv) == -1
+        assert L.wnfea_amg_set_coarse(h, 1, None) == -1
+    finally:
+        L.wnfea_amg_destroy(h)
