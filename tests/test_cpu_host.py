@@ -419,3 +419,23 @@ def test_coarse_inverse_rejects_an_empty_level():
         assert L.wnfea_amg_set_coarse(h, 1, None) == -1
     finally:
         L.wnfea_amg_destroy(h)
+
+
+def test_equilibrated_finest_level_has_a_unit_fp32_inverse_diagonal():
+    """On S K S with S = diag(K)^-1/2 every diagonal entry is K_ii / K_ii:
+    the finest level's point-Jacobi inverse rounds to exactly 1.0 in FP32
+    (the matrix-free Chebyshev steps then skip its load), and a level below
+    it does not."""
+    from oracle import oracle as O
+    from wnfea_amd.solver.amg_preconditioner import sa_setup
+    model, _ = G.generate_c3d10_structured_block(1.0, 0.1, 0.1, 24, 3, 3, tets_per_hex=6, load_direction="z")
+    E, nu = model.element_moduli()
+    fixed = model.fixed_dof_mask(3)
+    K, _ = O.assemble_csr(model.mesh_nodes, model.solid_elements, E, nu, fixed, model.load_vector(3))
+    d = K.diagonal()
+    s = 1.0 / np.sqrt(np.abs(d))
+    A = (sp.diags(s) @ K @ sp.diags(s)).tobsr(blocksize=(3, 3))
+    levels, _ = sa_setup(A, model.mesh_nodes, fixed, coarse_size=300, dof_scale=np.sqrt(np.abs(d)))
+    assert np.all(levels[0].dinv.astype(np.float32) == np.float32(1.0))
+    assert np.abs(levels[0].dinv - 1.0).max() < 1e-15
+    assert not np.all(levels[1].dinv.astype(np.float32) == np.float32(1.0))
